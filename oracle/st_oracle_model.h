@@ -1,0 +1,297 @@
+// Scene model and geometry set-up of the CPU oracle.  TEST INFRASTRUCTURE
+// ONLY (see st_oracle_rng.h for who may load it).
+//
+// Each function cites the reference lines it restates.  Floating-point
+// expressions keep the reference's evaluation order so that, compiled with
+// the reference's own flags, the oracle reproduces the reference bit for bit
+// (checked against oracle/_ref by tests/test_oracle_vs_ref.py).
+#pragma once
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace oracle {
+
+static const double kDegToRad = 0.017453292519943295;  // types.h:64
+
+struct TableEntry {
+    double angle;
+    double value;
+};
+
+// coretrace/types.h:85-109
+struct OpticFace {
+    char dist = 'g';
+    int optnum = 1, apgr = 0, order = 0;
+    double refr[4] = {0, 0, 0, 0};
+    double ab12[4] = {0, 0, 0, 0};
+    double reflectivity = 0, transmissivity = 0;
+    double slope = 0, spec = 0;
+    bool use_refl_table = false, use_trans_table = false;
+    std::vector<TableEntry> refl_table, trans_table;
+};
+
+struct OpticSet {
+    std::string name;
+    OpticFace front, back;
+};
+
+// coretrace/types.h:117-193 (only the fields the in-scope surfaces use)
+struct Element {
+    bool enabled = true;
+    double origin[3] = {0, 0, 0}, aim[3] = {0, 0, 0}, zrot = 0;
+    double euler[3] = {0, 0, 0};
+    double R[3][3] = {{0}}, RT[3][3] = {{0}};  // ref->loc, loc->ref
+    double pos_sun[3] = {0, 0, 0};
+    char shape = ' ';
+    double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // ParameterA..H
+    double area = 0, zap = 0;
+    char surf = ' ';
+    int stype = 0;
+    double kappa = 0, alpha[5] = {0, 0, 0, 0, 0};
+    double vcx = 0, vcy = 0, annular_r = 0, cross_r = 0, cone_half = 0, curv_rev = 0;
+    int fit_order = 0;
+    int bc_n = 0;                 // BCoefficients is bc_n x bc_n
+    std::vector<double> bc;       // row-major (i, j)
+    std::vector<double> poly;     // .ply coefficients
+    std::vector<double> sx, sy, sy2;  // cubic spline data
+    int interaction = 0;
+    OpticSet *optics = nullptr;
+    std::string optic_name;
+    double B(int i, int j) const { return bc[(size_t)i * bc_n + j]; }
+};
+
+struct Record {
+    double pos[3];
+    double cos[3];
+    int element;
+    int stage;
+    uint32_t raynum;
+};
+
+// coretrace/types.h:239-315
+struct Stage {
+    bool multihit = true, virt = false, tracethrough = false;
+    double origin[3] = {0, 0, 0}, aim[3] = {0, 0, 0}, zrot = 0;
+    double euler[3] = {0, 0, 0};
+    double R[3][3] = {{0}}, RT[3][3] = {{0}};
+    std::vector<Element *> elements;
+    std::vector<Record> recs;
+    ~Stage() {
+        for (Element *e : elements) delete e;
+    }
+};
+
+// coretrace/types.h:195-221
+struct Sun {
+    char shape = ' ';
+    double sigma = 0;
+    bool point_source = false;
+    std::vector<double> angle, intensity;
+    double max_angle = 0, max_intensity = 0;
+    double origin[3] = {0, 0, 0};
+    double euler[3] = {0, 0, 0};
+    double RT[3][3] = {{0}};
+    double maxrad = 0, xcm = 0, ycm = 0;
+    double minx = 0, maxx = 0, miny = 0, maxy = 0;
+};
+
+// coretrace/types.h:317-343, defaults types.cpp:439-448
+struct System {
+    Sun sun;
+    std::vector<OpticSet *> optics;
+    std::vector<Stage *> stages;
+    int raycount = 1000, raymax = 100000;
+    bool dyn_group = false, err_sunshape = true, err_optical = true;
+    std::vector<Record> all;
+    uint64_t sunraycount = 0;
+    std::vector<std::string> messages;
+    int rng_mode = 0;                  // 0 reference stream, 1 counter streams
+    std::vector<uint64_t> ray_sun_index;  // raynum-1 -> 0-based sun ray index
+    uint64_t draws_total = 0;
+
+    ~System() {
+        for (Stage *s : stages) delete s;
+        for (OpticSet *o : optics) delete o;
+    }
+    void errlog(const char *fmt, ...) {
+        char buf[513];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, 512, fmt, ap);
+        va_end(ap);
+        messages.push_back(buf);
+    }
+};
+
+// ------------------------------------------------------------ math ---------
+// coretrace/mathproc.cpp:58-72
+inline void mat_vec(const double M[3][3], const double v[3], double o[3]) {
+    o[0] = M[0][0] * v[0] + M[0][1] * v[1] + M[0][2] * v[2];
+    o[1] = M[1][0] * v[0] + M[1][1] * v[1] + M[1][2] * v[2];
+    o[2] = M[2][0] * v[0] + M[2][1] * v[1] + M[2][2] * v[2];
+}
+// coretrace/mathproc.cpp:84-95
+inline double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+// coretrace/mathproc.cpp:115-142
+inline void to_local(const double pref[3], const double cref[3], const double org[3], const double R[3][3],
+                     double ploc[3], double cloc[3]) {
+    double d[3] = {pref[0] - org[0], pref[1] - org[1], pref[2] - org[2]};
+    double c[3] = {cref[0], cref[1], cref[2]};
+    mat_vec(R, d, ploc);
+    mat_vec(R, c, cloc);
+}
+// coretrace/mathproc.cpp:146-173
+inline void to_ref(const double ploc[3], const double cloc[3], const double org[3], const double RT[3][3],
+                   double pref[3], double cref[3]) {
+    double t[3], c[3] = {cloc[0], cloc[1], cloc[2]};
+    mat_vec(RT, ploc, t);
+    mat_vec(RT, c, cref);
+    for (int i = 0; i < 3; i++) pref[i] = t[i] + org[i];
+}
+// coretrace/mathproc.cpp:175-217 (Spencer & Murty eq. 2)
+inline void euler_matrices(const double e[3], double R[3][3], double RT[3][3]) {
+    double ca = cos(e[0]), cb = cos(e[1]), cg = cos(e[2]);
+    double sa = sin(e[0]), sb = sin(e[1]), sg = sin(e[2]);
+    R[0][0] = ca * cg + sa * sb * sg;
+    R[0][1] = -cb * sg;
+    R[0][2] = -sa * cg + ca * sb * sg;
+    R[1][0] = ca * sg - sa * sb * cg;
+    R[1][1] = cb * cg;
+    R[1][2] = -sa * sg - ca * sb * cg;
+    R[2][0] = sa * cb;
+    R[2][1] = sb;
+    R[2][2] = ca * cb;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) RT[i][j] = R[j][i];
+}
+// coretrace/input.cpp:158-176 and stapi.cpp:737-752
+inline bool euler_from_aim(const double org[3], const double aim[3], double zrot, double e[3]) {
+    double dx = aim[0] - org[0], dy = aim[1] - org[1], dz = aim[2] - org[2];
+    double dtot = sqrt(dx * dx + dy * dy + dz * dz);
+    if (dtot == 0.0) return false;
+    dx = dx / dtot;
+    dy = dy / dtot;
+    dz = dz / dtot;
+    e[0] = atan2(dx, dz);
+    e[1] = asin(dy);
+    e[2] = zrot * kDegToRad;
+    return true;
+}
+
+// Numerical-Recipes cubic spline, coretrace/mathproc.cpp:244-330
+inline bool splint(const std::vector<double> &xa, const std::vector<double> &ya, const std::vector<double> &y2a,
+                   int n, double x, double *y, double *dydx) {
+    int klo = 0, khi = n - 1;
+    while (khi - klo > 1) {
+        int k = (khi + klo) / 2;
+        if (xa[k] > x)
+            khi = k;
+        else
+            klo = k;
+    }
+    double h = xa[khi] - xa[klo];
+    if (h == 0.0) return false;
+    double a = (xa[khi] - x) / h;
+    double b = (x - xa[klo]) / h;
+    *y = a * ya[klo] + b * ya[khi] + ((a * a * a - a) * y2a[klo] + (b * b * b - b) * y2a[khi]) * (h * h) / 6.0;
+    *dydx = (ya[khi] - ya[klo]) / (xa[khi] - xa[klo]) - (3.0 * a * a - 1.0) * (xa[khi] - xa[klo]) * y2a[klo] / 6.0 +
+            (3.0 * b * b - 1.0) * (xa[khi] - xa[klo]) * y2a[khi] / 6.0;
+    return true;
+}
+
+inline void spline(const std::vector<double> &x, const std::vector<double> &y, int n, double yp1, double ypn,
+                   std::vector<double> &y2) {
+    std::vector<double> u(x.size());
+    if (yp1 > 0.99e30) {
+        y2[0] = 0.0;
+        u[0] = 0.0;
+    } else {
+        y2[0] = -0.5;
+        u[0] = (3.0 / (x[1] - x[0])) * ((y[1] - y[0]) / (x[1] - x[0]) - yp1);
+    }
+    for (int i = 1; i < n - 1; i++) {
+        double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+        double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        u[i] = (y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1]);
+        u[i] = (6.0 * u[i] / (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / p;
+    }
+    double qn, un;
+    if (ypn > 0.99e30) {
+        qn = 0.0;
+        un = 0.0;
+    } else {
+        qn = 0.5;
+        un = (3.0 / (x[n - 1] - x[n - 2])) * (ypn - (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]));
+    }
+    y2[n - 1] = (un - qn * u[n - 2]) / (qn * y2[n - 2] + 1.0);
+    for (int k = n - 2; k >= 0; k--) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
+// Insertion sort over slots 0..n-1 with the reference's slot-0 behaviour:
+// a new minimum lands in slot 1 and slot 0 keeps its value
+// (coretrace/mathproc.cpp:333-348).
+inline void piksrt_ref(int n, double arr[5]) {
+    for (int j = 1; j < n; j++) {
+        double a = arr[j];
+        int i = j - 1;
+        bool stopped = false;
+        for (; i >= 0; i--) {
+            if (arr[i] <= a) {
+                stopped = true;
+                break;
+            }
+            arr[i + 1] = arr[i];
+        }
+        if (!stopped) i = 0;
+        arr[i + 1] = a;
+    }
+}
+
+// Zernike monomials, coretrace/vshot.cpp:59-139
+inline double eval_mono(double ax, double ay, const Element &e) {
+    double z = 0.0;
+    for (int i = 0; i <= e.fit_order; i++) {
+        for (int j = 0; j <= i; j++) {
+            double xx = (j == 0) ? 1.0 : pow(ax, j);
+            double yy = (i - j == 0) ? 1.0 : pow(ay, (i - j));
+            z = z + e.B(i, j) * xx * yy;
+        }
+    }
+    return z;
+}
+inline void mono_slope(const Element &e, double sxp, double syp, double *dzdx, double *dzdy) {
+    if (sxp == 0.0) sxp = sxp + 1.0e-10;
+    if (syp == 0.0) syp = syp + 1.0e-10;
+    *dzdx = 0.0;
+    *dzdy = 0.0;
+    for (int i = 1; i <= e.fit_order; i++) {
+        for (int j = 0; j <= i; j++) {
+            *dzdx = *dzdx + j * e.B(i, j) * pow(sxp, (j - 1)) * pow(syp, (i - j));
+            *dzdy = *dzdy + (i - j) * e.B(i, j) * pow(sxp, j) * pow(syp, (i - j - 1));
+        }
+    }
+}
+// Rotationally symmetric polynomial, coretrace/mathproc.cpp:218-241
+inline double eval_poly(double ax, double ay, const Element &e) {
+    double az = 0.0;
+    double r = sqrt(ax * ax + ay * ay);
+    for (int i = 0; i <= e.fit_order; i++) az = az + e.poly[i] * pow(r, i);
+    return az;
+}
+inline void poly_slope(const Element &e, double ax, double ay, double *dzdx, double *dzdy) {
+    double dzdr = 0.0;
+    double r = sqrt(ax * ax + ay * ay);
+    double drdx = ax / r, drdy = ay / r;
+    for (int i = 1; i <= e.fit_order; i++) dzdr = dzdr + i * e.poly[i] * pow(r, i - 1);
+    *dzdx = dzdr * drdx;
+    *dzdy = dzdr * drdy;
+}
+
+}  // namespace oracle

@@ -1,0 +1,164 @@
+"""Synthetic workloads: the trough, tower and faceted-field scenes the
+benchmarks and parity tests run, built only from their stated parameters.
+
+* :func:`trough` -- parabolic trough, 5.77 m x 12 m aperture, f = 1.71 m,
+  70 mm absorber tube on the focal line, pillbox sun 4.65 mrad (layout as in
+  app/deploy/api/examples/trough-demo.py:36-66).
+* :func:`tower` -- N spherically focused 10 x 10 m heliostats on a radially
+  staggered layout within 1 km, flat 15 x 15 m receiver at 150 m, Buie CSR
+  sun, sun at 45 deg elevation and 180 deg azimuth (aim = bisector of the sun
+  and receiver vectors, as app/deploy/api/examples/tower-demo.py:171-178).
+* :func:`faceted_field` -- N heliostats x 4 flat 5 x 5 m facets plus a
+  cylindrical external receiver (as the reference's tower sample's receiver:
+  element on the wall, aimed along +y, 't' surface with an 'l' aperture).
+* :func:`buie_profile` -- the Buie et al. (2003) sunshape, tabulated for the
+  'd' user sunshape (angles in mrad).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .scene import Element, Optic, OpticFace, Scene, Stage
+
+# ------------------------------------------------------------------ sun ----
+
+
+def buie_profile(csr: float, n_disk: int = 48, n_aureole: int = 48, theta_max: float = 43.6):
+    """Buie, Monger & Dey, Solar Energy 74 (2003) 113-122, eqs. (3)-(6):
+    phi(t) = cos(0.326 t)/cos(0.308 t) for t <= 4.65 mrad,
+    phi(t) = exp(kappa) t^gamma beyond, with
+    kappa = 0.9 ln(13.5 chi) chi^-0.3, gamma = 2.2 ln(0.52 chi) chi^0.43 - 0.1.
+    Returns (angles_mrad, intensity normalised to 1 at t = 0)."""
+    chi = float(csr)
+    kappa = 0.9 * math.log(13.5 * chi) * chi ** -0.3
+    gamma = 2.2 * math.log(0.52 * chi) * chi ** 0.43 - 0.1
+    disk = np.linspace(0.0, 4.65, n_disk)
+    aur = np.geomspace(4.65 + 1e-3, theta_max, n_aureole)
+    a = np.concatenate([disk, aur])
+    phi = np.where(a <= 4.65, np.cos(0.326 * a) / np.cos(0.308 * a),
+                   np.exp(kappa) * np.power(np.maximum(a, 1e-9), gamma))
+    phi = phi / phi[0]
+    return a, phi
+
+
+# ---------------------------------------------------------------- trough ---
+
+
+def trough(raycount: int = 1_000_000, aperture: float = 5.77, length: float = 12.0, focal: float = 1.71,
+           tube_d: float = 0.070, refl: float = 0.94, slope: float = 2.0, spec: float = 0.5,
+           absorptivity: float = 0.95, sun_half_angle: float = 4.65, sun_shape: str = "p") -> Scene:
+    """Parabolic-cylinder mirror + absorber tube on the focal line."""
+    r = tube_d / 2.0
+    s = Scene(sun_shape=sun_shape, sun_sigma=sun_half_angle, sun_xyz=(0.0, 0.0, 100.0),
+              raycount=raycount, maxcount=max(100 * raycount, 1000), dyn_group=False)
+    s.optics = [Optic("mirror", OpticFace(dist="g", refl=refl, slope=slope, spec=spec),
+                      OpticFace(dist="g", refl=0.0, slope=slope, spec=spec)),
+                Optic("absorber", OpticFace(dist="g", refl=1.0 - absorptivity),
+                      OpticFace(dist="g", refl=1.0 - absorptivity))]
+    mirror = Element(xyz=(0, 0, 0), aim=(0, 0, 1), aperture="l",
+                     ap_params=(-aperture / 2, aperture / 2, length), surface="p",
+                     surf_params=(1.0 / (2.0 * focal), 0.0), optic="mirror")
+    tube = Element(xyz=(0, 0, focal - r), aim=(0, 0, 100), aperture="l", ap_params=(0, 0, length),
+                   surface="t", surf_params=(1.0 / r,), optic="absorber")
+    s.stages = [Stage(elements=[mirror]), Stage(elements=[tube])]
+    return s
+
+
+# ----------------------------------------------------------------- tower ---
+
+
+def staggered_layout(n: int, r_min: float, r_max: float, pitch: float,
+                     north_only: bool = False) -> np.ndarray:
+    """Radially staggered rings: heliostats spaced ``pitch`` along each ring,
+    odd rings rotated by half a pitch, ring spacing chosen so that ``n``
+    positions fill [r_min, r_max] (the last ring may be partial)."""
+    span = math.pi if north_only else 2 * math.pi
+    dr = span * (r_max ** 2 - r_min ** 2) / (2.0 * pitch * n)
+    pts = []
+    k = 0
+    while len(pts) < n:
+        r = r_min + (k + 0.5) * dr
+        m = max(1, int(span * r / pitch))
+        off = 0.5 if (k % 2) else 0.0
+        for j in range(m):
+            a = (j + off) / m * span
+            pts.append((r * math.cos(a), r * math.sin(a)))
+        k += 1
+    return np.array(pts[:n], dtype=np.float64)
+
+
+def sun_vector(elev_deg: float, azim_deg: float) -> np.ndarray:
+    """Unit vector toward the sun, z up, y north, x east, azimuth from north."""
+    e, a = math.radians(elev_deg), math.radians(azim_deg)
+    return np.array([math.sin(a) * math.cos(e), math.cos(a) * math.cos(e), math.sin(e)])
+
+
+def tower(n_helio: int = 1000, raycount: int = 10_000_000, helio_size: float = 10.0,
+          rec_size: float = 15.0, tower_h: float = 150.0, r_max: float = 1000.0,
+          sun_elev: float = 45.0, sun_azim: float = 180.0, csr: float = 0.1, slope: float = 1.5,
+          refl: float = 0.9, rec_abs: float = 0.95, sun: str = "buie", dyn_group: bool = True) -> Scene:
+    """Heliostat field in the north half-plane, flat receiver facing it."""
+    sv = sun_vector(sun_elev, sun_azim)
+    pts = staggered_layout(n_helio, 100.0, r_max, 2.2 * helio_size, north_only=True)
+    rec = np.array([0.0, 0.0, tower_h])
+    s = Scene(sun_xyz=tuple(sv * 1000.0), raycount=raycount, maxcount=2**31 - 1, dyn_group=dyn_group)
+    if sun == "buie":
+        a, v = buie_profile(csr)
+        s.sun_shape, s.sun_sigma, s.sun_table = "d", 4.65, (a, v)
+    else:
+        s.sun_shape, s.sun_sigma = "p", 4.65
+    s.optics = [Optic("helio", OpticFace(dist="g", refl=refl, slope=slope, spec=0.0),
+                      OpticFace(dist="g", refl=0.0, slope=slope, spec=0.0)),
+                Optic("receiver", OpticFace(dist="g", refl=1.0 - rec_abs), OpticFace(dist="g", refl=1.0 - rec_abs))]
+    els = []
+    for x, y in pts:
+        p = np.array([x, y, 0.0])
+        to_rec = rec - p
+        slant = float(np.linalg.norm(to_rec))
+        nrm = to_rec / slant + sv
+        nrm /= np.linalg.norm(nrm)
+        els.append(Element(xyz=tuple(p), aim=tuple(p + 100.0 * nrm), aperture="r",
+                           ap_params=(helio_size, helio_size), surface="s",
+                           surf_params=(1.0 / (2.0 * slant),), optic="helio"))
+    centroid = np.array([pts[:, 0].mean(), pts[:, 1].mean(), 0.0])
+    recel = Element(xyz=tuple(rec), aim=tuple(centroid), aperture="r", ap_params=(rec_size, rec_size),
+                    surface="f", optic="receiver")
+    s.stages = [Stage(elements=els), Stage(elements=[recel])]
+    return s
+
+
+def faceted_field(n_helio: int = 10_000, raycount: int = 50_000_000, facet: float = 5.0,
+                  rec_radius: float = 8.5, rec_height: float = 20.0, tower_h: float = 200.0,
+                  r_max: float = 1500.0, sun_elev: float = 45.0, sun_azim: float = 180.0,
+                  sun_sigma: float = 2.5, slope: float = 1.5, refl: float = 0.9,
+                  rec_abs: float = 0.95) -> Scene:
+    """Surround field of 2x2-facet heliostats and a cylindrical receiver."""
+    sv = sun_vector(sun_elev, sun_azim)
+    pts = staggered_layout(n_helio, 120.0, r_max, 2.6 * facet * 2, north_only=False)
+    rec = np.array([0.0, 0.0, tower_h])
+    s = Scene(sun_shape="g", sun_sigma=sun_sigma, sun_xyz=tuple(sv * 1000.0), raycount=raycount,
+              maxcount=2**31 - 1, dyn_group=True)
+    s.optics = [Optic("helio", OpticFace(dist="g", refl=refl, slope=slope, spec=0.0),
+                      OpticFace(dist="g", refl=0.0, slope=slope, spec=0.0)),
+                Optic("receiver", OpticFace(dist="g", refl=1.0 - rec_abs), OpticFace(dist="g", refl=1.0 - rec_abs))]
+    els = []
+    for x, y in pts:
+        p = np.array([x, y, 0.0])
+        to_rec = rec - p
+        nrm = to_rec / np.linalg.norm(to_rec) + sv
+        nrm /= np.linalg.norm(nrm)
+        # in-plane axes: u horizontal, w = n x u
+        u = np.cross([0.0, 0.0, 1.0], nrm)
+        u /= np.linalg.norm(u)
+        w = np.cross(nrm, u)
+        for du, dw in ((-0.5, -0.5), (0.5, -0.5), (-0.5, 0.5), (0.5, 0.5)):
+            c = p + (du * u + dw * w) * (facet + 0.05)
+            els.append(Element(xyz=tuple(c), aim=tuple(c + 100.0 * nrm), aperture="r",
+                               ap_params=(facet, facet), surface="f", optic="helio"))
+    recel = Element(xyz=(0.0, -rec_radius, tower_h), aim=(0.0, 1000.0 - rec_radius, tower_h), aperture="l",
+                    ap_params=(0.0, 0.0, rec_height), surface="t", surf_params=(1.0 / rec_radius,),
+                    optic="receiver")
+    s.stages = [Stage(elements=els), Stage(elements=[recel])]
+    return s
