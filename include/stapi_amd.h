@@ -1,0 +1,55 @@
+/*
+ * Extensions of the MI355X trace backend.  None of these has a counterpart
+ * in the reference's coretrace/stapi.h; callers that only use the reference
+ * interface never need them.
+ */
+#ifndef SOLTRACE_AMD_STAPI_AMD_H
+#define SOLTRACE_AMD_STAPI_AMD_H
+
+#include "stapi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arithmetic of the bulk ray algebra: 64 (default, the reference's FP64
+ * path) or 32 (FP32 bulk; frame transforms, sun-plane draws, the torus
+ * quartic and Newton start planes stay FP64).  Returns 1, or -1 for other
+ * values. */
+int st_amd_set_precision(st_context_t pcxt, int bits);
+
+/* 1 (default): stage-1 sun rays use the sun-plane grid and other rays a
+ * per-stage BVH; 0: every ray tests every element.  Both give the same hits
+ * (ties break to the lower element index, as a full scan does). */
+int st_amd_set_cull(st_context_t pcxt, int on);
+
+/* Bin, during the next runs, the last record of each ray on element `elem`
+ * (0-based) of stage `stage` (0-based) into an nx x ny map in that element's
+ * frame, the way the GUI flux map does (app/src/project.cpp:1139-1436):
+ * flat ('f') elements bin (x, y) over [xmin,xmax] x [ymin,ymax]; cylinders
+ * ('t') and partial cylinders ('s' + 'l') unroll x to arc length, which fixes
+ * the x range.  absorbed_only keeps only absorbed final records.  stage < 0
+ * switches binning off.  Returns 1, or -1 for an element that cannot be
+ * binned. */
+int st_amd_flux_request(st_context_t pcxt, int stage, int elem, int nx, int ny, int absorbed_only, double xmin,
+                        double xmax, double ymin, double ymax);
+
+/* Integer counts of the last run's map, counts[ix*ny + iy]; bin_dims (may be
+ * NULL) receives xmin, ymin, bin width, bin height.  Returns nx*ny or -1. */
+int st_amd_flux_result(st_context_t pcxt, unsigned int *counts, double *bin_dims);
+
+/* 0-based sun-ray index of rays 1..n of the last run (out[raynum-1]).
+ * Returns the number written, or -1. */
+int st_amd_ray_sun_index(st_context_t pcxt, unsigned long long *out, int n);
+
+/* Device timing of the last run (HIP events): [0] sun-mask ms,
+ * [1..16] trace ms per stage, [17] compaction ms, [18] wall ms of st_sim_run,
+ * [19] sun rays drawn, [20..35] rays entering each stage, [36..51] records
+ * per stage.  Writes min(n, 52) values; returns 52. */
+int st_amd_timing(st_context_t pcxt, double *out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

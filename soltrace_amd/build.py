@@ -1,0 +1,74 @@
+"""Build the HIP backend in-tree for gfx950.
+
+Produces ``soltrace_amd/coretrace_api.so`` (the name the reference's callers
+load, coretrace/CMakeLists.txt:89-94, no ``lib`` prefix) and the static
+``soltrace_amd/libcoretrace.a`` (coretrace/CMakeLists.txt:105-110).
+
+Usage: ``python -m soltrace_amd.build`` (or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+ARCH = os.environ.get("SOLTRACE_AMD_ARCH", "gfx950")
+
+SOURCES = ["st_kernels.hip", "st_scene.cpp", "st_context.cpp"]
+
+# -packed-fp32-ops off: packed FP32 forms are excluded from every kernel.
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+    "-munsafe-fp-atomics",
+    "-Wno-unused-command-line-argument",
+    "-I" + CSRC,
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found (ROCm is required to build the HIP backend)")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd[:3]) + " ...")
+    return r
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    cc = hipcc()
+    objs = []
+    for src in SOURCES:
+        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+        cmd = [cc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        if src.endswith(".cpp"):
+            cmd = [cc, "-x", "hip", *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run(cmd)
+        objs.append(obj)
+    so = os.path.join(HERE, "coretrace_api.so")
+    tmp = so + ".tmp"
+    _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, so)
+    lib = os.path.join(HERE, "libcoretrace.a")
+    if os.path.exists(lib):
+        os.remove(lib)
+    _run(["ar", "rcs", lib, *objs])
+    return so
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,1016 @@
+// C-ABI of the HIP backend: the 55 entry points of the reference's
+// coretrace/stapi.h (declared in include/stapi.h) plus the st_amd_*
+// extensions of include/stapi_amd.h.
+//
+// A context owns its scene model, a HIP stream (default priority), the
+// device tables of the last compile, and the device-resident records of the
+// last run.  st_sim_run (coretrace/stapi.cpp:680-734) compiles the scene,
+// traces every stage on the GPU and leaves the records in HBM; the result
+// getters copy them out in the reference's TRayData::Merge order
+// (coretrace/types.cpp:311-355).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "st_kernels.h"
+#include "st_scene.h"
+
+#define STCORE_API extern "C" __attribute__((visibility("default")))
+
+typedef unsigned long st_uint_t;
+typedef void *st_context_t;
+typedef int (*st_callback_t)(st_uint_t, st_uint_t, st_uint_t, st_uint_t, st_uint_t, void *);
+
+namespace st {
+
+static void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        char b[256];
+        snprintf(b, sizeof b, "HIP error in %s: %s", what, hipGetErrorString(e));
+        throw std::runtime_error(b);
+    }
+}
+#define HC(x) hip_check((x), #x)
+
+// Device buffer that grows on demand and keeps its capacity.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void *ensure(size_t bytes) {
+        if (bytes <= cap && p) return p;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes, 256);
+        HC(hipMalloc(&p, want));
+        cap = want;
+        return p;
+    }
+    template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+template <typename T>
+static T *upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
+    b.ensure(std::max<size_t>(1, v.size()) * sizeof(T));
+    if (!v.empty()) HC(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return b.as<T>();
+}
+
+struct StageRecords {
+    std::vector<std::shared_ptr<DevBuf>> chunks;
+    std::vector<size_t> counts;
+    size_t total = 0;
+};
+
+struct Timing {
+    double mask_ms = 0, trace_ms[kMaxStages] = {0}, other_ms = 0, total_ms = 0;
+    uint64_t sun_rays = 0, rays_in[kMaxStages] = {0}, records[kMaxStages] = {0};
+};
+
+struct Context {
+    SceneModel model;
+    int raycount = 1000, raymax = 100000;
+    bool dyn_group = false, err_sun = true, err_opt = true;
+    int precision = 64;
+    bool cull = true;
+    std::vector<std::string> messages;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    CompiledScene cs;
+    DevBuf d_elems, d_faces, d_aux, d_stages, d_bvh, d_bvh_items, d_grid_cells, d_grid_items;
+    // run scratch
+    DevBuf d_mask, d_popc, d_offs, d_scan_tmp, d_total, d_hits, d_rec_slots, d_rec_count, d_rec_offs, d_ovf_rec,
+        d_ovf_meta, d_ovf_count, d_out_slots, d_out_flag, d_out_offs, d_err, d_flux;
+    DevBuf d_rays_a, d_rays_b;
+
+    // results
+    std::vector<StageRecords> recs;
+    uint64_t sun_ray_count = 0;
+    bool host_valid = false;
+    std::vector<DevRecord> host_recs;  // merged order
+    uint64_t n_hits_done = 0;
+
+    FluxSpec flux{};
+    std::vector<uint32_t> flux_host;
+    Timing timing;
+
+    Context() {
+        HC(hipGetDevice(&device));
+        HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    }
+    ~Context() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void errlog(const char *fmt, ...) {
+        char b[513];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(b, 512, fmt, ap);
+        va_end(ap);
+        messages.push_back(b);
+    }
+    void clear_results() {
+        recs.clear();
+        host_valid = false;
+        host_recs.clear();
+        n_hits_done = 0;
+        sun_ray_count = 0;
+    }
+    int run(uint32_t seed, st_callback_t cb, void *cbdata);
+    template <typename R> int run_t(uint32_t seed, st_callback_t cb, void *cbdata);
+    void materialize();
+    size_t total_records() const {
+        size_t n = 0;
+        for (const auto &s : recs) n += s.total;
+        return n;
+    }
+};
+
+static Context *ctx_of(st_context_t p) { return reinterpret_cast<Context *>(p); }
+
+static const uint32_t kMaxChunk = 1u << 22;   // rays per trace launch
+static const int kSlots = 4;                  // record slots per ray per launch
+
+void Context::materialize() {
+    if (host_valid) return;
+    // TRayData::Merge applied stage by stage: full 8192-record blocks first,
+    // partial blocks re-appended after (coretrace/types.cpp:311-355).
+    const size_t B = 8192;
+    std::vector<std::vector<DevRecord>> per(recs.size());
+    for (size_t s = 0; s < recs.size(); s++) {
+        per[s].resize(recs[s].total);
+        size_t off = 0;
+        for (size_t c = 0; c < recs[s].chunks.size(); c++) {
+            if (recs[s].counts[c])
+                HC(hipMemcpyAsync(per[s].data() + off, recs[s].chunks[c]->p, recs[s].counts[c] * sizeof(DevRecord),
+                                  hipMemcpyDeviceToHost, stream));
+            off += recs[s].counts[c];
+        }
+    }
+    HC(hipStreamSynchronize(stream));
+    std::vector<DevRecord> all;
+    for (size_t s = 0; s < per.size(); s++) {
+        const std::vector<DevRecord> &src = per[s];
+        size_t fa = (all.size() / B) * B, fs = (src.size() / B) * B;
+        std::vector<DevRecord> out;
+        out.reserve(all.size() + src.size());
+        out.insert(out.end(), all.begin(), all.begin() + fa);
+        out.insert(out.end(), src.begin(), src.begin() + fs);
+        out.insert(out.end(), all.begin() + fa, all.end());
+        out.insert(out.end(), src.begin() + fs, src.end());
+        all.swap(out);
+    }
+    host_recs.swap(all);
+    host_valid = true;
+}
+
+int Context::run(uint32_t seed, st_callback_t cb, void *cbdata) {
+    try {
+        return precision == 32 ? run_t<float>(seed, cb, cbdata) : run_t<double>(seed, cb, cbdata);
+    } catch (const std::exception &e) {
+        errlog("trace error: %s", e.what());
+        return -1;
+    }
+}
+
+template <typename R>
+int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
+    auto t_start = std::chrono::steady_clock::now();
+    clear_results();
+    timing = Timing();
+    std::string msg;
+    if (!compile_scene(model, cull, cs, msg)) {
+        errlog("%s", msg.c_str());
+        return -1;
+    }
+    const uint64_t N = (uint64_t)(int64_t)raycount;
+    const uint64_t maxrays = (uint64_t)(int64_t)raymax;
+    if ((int64_t)raycount < 1) {
+        errlog("invalid number of rays: %d", raycount);
+        return -1;
+    }
+    if (model.stages.empty()) {
+        errlog("no stages defined.");
+        return -1;
+    }
+    if ((int)model.stages.size() > kMaxStages) {
+        errlog("at most %d stages are supported", (int)kMaxStages);
+        return -1;
+    }
+    if (!compile_sun(model, cs, msg)) {
+        errlog("%s", msg.c_str());
+        return -1;
+    }
+    hipStream_t s = stream;
+    SceneView sv{};
+    sv.elems = upload(d_elems, cs.elems, s);
+    sv.faces = upload(d_faces, cs.faces, s);
+    sv.aux = upload(d_aux, cs.aux, s);
+    sv.stages = upload(d_stages, cs.stages, s);
+    sv.bvh = upload(d_bvh, cs.bvh, s);
+    sv.bvh_items = upload(d_bvh_items, cs.bvh_items, s);
+    sv.grid_cells = upload(d_grid_cells, cs.grid_cells, s);
+    sv.grid_items = upload(d_grid_items, cs.grid_items, s);
+    sv.grid = cs.grid;
+    sv.use_grid = cs.use_grid ? 1 : 0;
+    sv.nstages = (int)cs.stages.size();
+    sv.sun = cs.sun;
+
+    uint32_t *err = (uint32_t *)d_err.ensure(sizeof(uint32_t));
+    launch_fill_u32(err, 1, 0u, s);
+    uint32_t *ovf_count = (uint32_t *)d_ovf_count.ensure(sizeof(uint32_t));
+    uint32_t *total = (uint32_t *)d_total.ensure(4 * sizeof(uint32_t));
+    const uint32_t ovf_cap = kMaxChunk / 2;
+    d_ovf_rec.ensure((size_t)ovf_cap * sizeof(DevRecord));
+    d_ovf_meta.ensure((size_t)ovf_cap * 2 * sizeof(uint32_t));
+    d_rec_slots.ensure((size_t)kMaxChunk * kSlots * sizeof(DevRecord));
+    d_rec_count.ensure((size_t)kMaxChunk * sizeof(uint32_t));
+    d_rec_offs.ensure((size_t)kMaxChunk * sizeof(uint32_t));
+    d_out_slots.ensure((size_t)kMaxChunk * sizeof(DevRay));
+    d_out_flag.ensure((size_t)kMaxChunk * sizeof(uint32_t));
+    d_out_offs.ensure((size_t)kMaxChunk * sizeof(uint32_t));
+    d_scan_tmp.ensure(scan_tmp_words((size_t)1 << 26) * sizeof(uint32_t));
+
+    FluxSpec fs = flux;
+    if (fs.active) {
+        d_flux.ensure((size_t)fs.nx * fs.ny * sizeof(uint32_t));
+        launch_fill_u32(d_flux.as<uint32_t>(), (size_t)fs.nx * fs.ny, 0u, s);
+    }
+
+    hipEvent_t ev0, ev1;
+    HC(hipEventCreate(&ev0));
+    HC(hipEventCreate(&ev1));
+    auto timed = [&](double &acc, auto &&fn) {
+        HC(hipEventRecord(ev0, s));
+        fn();
+        HC(hipEventRecord(ev1, s));
+        HC(hipEventSynchronize(ev1));
+        float ms = 0;
+        HC(hipEventElapsedTime(&ms, ev0, ev1));
+        acc += ms;
+    };
+    auto read_u32 = [&](const uint32_t *p) {
+        uint32_t v = 0;
+        HC(hipMemcpyAsync(&v, p, sizeof v, hipMemcpyDeviceToHost, s));
+        HC(hipStreamSynchronize(s));
+        return v;
+    };
+    auto check_err = [&]() -> bool {
+        uint32_t e = read_u32(err);
+        if (e & 1u) {
+            errlog("Bad optical interaction type (stage)");
+            return false;
+        }
+        if (e & 2u) {
+            errlog("internal error: sun-ray hit test disagreed between launches");
+            return false;
+        }
+        if (e & 4u) {
+            errlog("internal error: record overflow buffer full");
+            return false;
+        }
+        return true;
+    };
+
+    recs.assign(model.stages.size(), StageRecords());
+    const size_t nst = model.stages.size();
+    uint64_t traced_total = 0;
+    bool cancelled = false;
+
+    // One trace launch over n rays (chunk), then record and ray compaction.
+    // Appends the passed rays to next_rays at *n_next.
+    auto trace_chunk = [&](int si, int n, const uint64_t *sun_idx, uint32_t raynum_base, const DevRay *in,
+                           uint32_t last_raynum, DevRay *next_rays, uint64_t *n_next) -> bool {
+        TraceArgs a{};
+        a.sv = sv;
+        a.si = si;
+        a.seed = seed;
+        a.inc_sun = err_sun;
+        a.inc_err = err_opt;
+        a.n_rays = n;
+        a.slots = kSlots;
+        a.sun_index = sun_idx;
+        a.raynum_base = raynum_base;
+        a.in_rays = in;
+        a.last_raynum = last_raynum;
+        a.rec_slots = d_rec_slots.as<DevRecord>();
+        a.rec_count = d_rec_count.as<uint32_t>();
+        a.ovf_rec = d_ovf_rec.as<DevRecord>();
+        a.ovf_meta = d_ovf_meta.as<uint32_t>();
+        a.ovf_count = ovf_count;
+        a.ovf_cap = ovf_cap;
+        a.out_slots = d_out_slots.as<DevRay>();
+        a.out_flag = d_out_flag.as<uint32_t>();
+        a.err_flags = err;
+        a.flux = fs;
+        a.flux_bins = fs.active ? d_flux.as<uint32_t>() : nullptr;
+        launch_fill_u32(ovf_count, 1, 0u, s);
+        timed(timing.trace_ms[si], [&]() { launch_trace_stage<R>(a, s); });
+        timing.rays_in[si] += (uint64_t)n;
+        auto t0 = std::chrono::steady_clock::now();
+        launch_exclusive_scan(a.rec_count, d_rec_offs.as<uint32_t>(), (size_t)n, d_scan_tmp.as<uint32_t>(), total, s);
+        launch_exclusive_scan(a.out_flag, d_out_offs.as<uint32_t>(), (size_t)n, d_scan_tmp.as<uint32_t>(), total + 1,
+                              s);
+        uint32_t tot[2];
+        HC(hipMemcpyAsync(tot, total, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HC(hipStreamSynchronize(s));
+        if (!check_err()) return false;
+        auto chunk = std::make_shared<DevBuf>();
+        chunk->ensure((size_t)std::max<uint32_t>(1, tot[0]) * sizeof(DevRecord));
+        launch_compact_records(a.rec_slots, a.rec_count, d_rec_offs.as<uint32_t>(), n, kSlots,
+                               chunk->as<DevRecord>(), s);
+        launch_compact_overflow(a.ovf_rec, a.ovf_meta, ovf_count, ovf_cap, d_rec_offs.as<uint32_t>(),
+                                chunk->as<DevRecord>(), s);
+        if (tot[1] && next_rays)
+            launch_compact_rays(a.out_slots, a.out_flag, d_out_offs.as<uint32_t>(), n, next_rays + *n_next, s);
+        HC(hipStreamSynchronize(s));
+        recs[si].chunks.push_back(chunk);
+        recs[si].counts.push_back(tot[0]);
+        recs[si].total += tot[0];
+        timing.records[si] += tot[0];
+        if (n_next) *n_next += tot[1];
+        timing.other_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        traced_total += (uint64_t)n;
+        return true;
+    };
+
+    // ------------------------------------------------------ stage 0 -------
+    DevRay *next = (DevRay *)d_rays_a.ensure((size_t)N * sizeof(DevRay));
+    uint64_t *sun_index_all = (uint64_t *)d_hits.ensure((size_t)N * sizeof(uint64_t));
+    uint64_t n_next = 0;
+    uint64_t hits = 0, s0 = 0;
+    double rate = 0.0;  // hits per sun ray observed so far
+    while (hits < N) {
+        uint64_t want = N - hits;
+        uint64_t batch = (rate > 0) ? (uint64_t)(want / rate * 1.05) + 4096 : std::min<uint64_t>(1u << 22, want * 4 + 4096);
+        batch = std::min<uint64_t>(batch, 1u << 25);
+        batch = std::max<uint64_t>(batch, 4096);
+        if (s0 + batch > maxrays) batch = maxrays - s0;
+        if (batch == 0) {
+            errlog("generated sun rays reached maximum count: %d", raymax);
+            return -1;
+        }
+        const size_t nwords = (batch + 63) / 64;
+        uint64_t *mask = (uint64_t *)d_mask.ensure(nwords * sizeof(uint64_t));
+        uint32_t *popc = (uint32_t *)d_popc.ensure(nwords * sizeof(uint32_t));
+        uint32_t *offs = (uint32_t *)d_offs.ensure(nwords * sizeof(uint32_t));
+        d_scan_tmp.ensure(scan_tmp_words(nwords) * sizeof(uint32_t));
+        MaskArgs ma{};
+        ma.sv = sv;
+        ma.seed = seed;
+        ma.s0 = s0;
+        ma.n = (uint32_t)batch;
+        ma.mask = mask;
+        timed(timing.mask_ms, [&]() { launch_sun_mask<R>(ma, s); });
+        timing.sun_rays += batch;
+        launch_mask_popc(mask, popc, nwords, s);
+        launch_exclusive_scan(popc, offs, nwords, d_scan_tmp.as<uint32_t>(), total, s);
+        uint32_t h = read_u32(total);
+        uint32_t take = (uint32_t)std::min<uint64_t>(h, want);
+        uint64_t *hit_list = sun_index_all + hits;
+        if (take) launch_mask_compact(mask, offs, nwords, s0, take, hit_list, s);
+        if (take == want) {
+            uint64_t last = 0;
+            HC(hipMemcpyAsync(&last, hit_list + take - 1, sizeof last, hipMemcpyDeviceToHost, s));
+            HC(hipStreamSynchronize(s));
+            sun_ray_count = last + 1;
+        } else {
+            sun_ray_count = s0 + batch;
+        }
+        for (uint32_t c = 0; c < take; c += kMaxChunk) {
+            int n = (int)std::min<uint32_t>(kMaxChunk, take - c);
+            if (!trace_chunk(0, n, hit_list + c, (uint32_t)(hits + c), nullptr, 0, next, &n_next)) return -1;
+        }
+        hits += take;
+        n_hits_done = hits;
+        s0 += batch;
+        rate = (double)hits / (double)s0;
+        if (hits < N && s0 >= maxrays) {
+            errlog("generated sun rays reached maximum count: %d", raymax);
+            return -1;
+        }
+        if (cb && !cb((st_uint_t)traced_total, (st_uint_t)hits, (st_uint_t)N, 1, (st_uint_t)nst, cbdata)) {
+            cancelled = true;
+            break;
+        }
+    }
+
+    // stage-0 pass-on rule: the last ray's pass counts only when an earlier
+    // ray was passed too (coretrace/raytrace.cpp:786-797, 1081-1085)
+    auto last_raynum_of = [&](DevRay *rays, uint64_t n) -> uint32_t {
+        DevRay r;
+        HC(hipMemcpyAsync(&r, rays + n - 1, sizeof r, hipMemcpyDeviceToHost, s));
+        HC(hipStreamSynchronize(s));
+        return r.raynum;
+    };
+    DevRay *cur = next;
+    uint64_t n_cur = n_next;
+    bool prev_has = false;
+    uint32_t last_prev = 0;
+    if (!cancelled && n_cur > 0) {
+        last_prev = last_raynum_of(cur, n_cur);
+        uint64_t below = (last_prev == N) ? n_cur - 1 : n_cur;
+        prev_has = below > 0;
+    }
+
+    // ------------------------------------------------- later stages -------
+    DevBuf *bufs[2] = {&d_rays_a, &d_rays_b};
+    int which = 0;
+    for (size_t si = 1; si < nst && prev_has && !cancelled; si++) {
+        DevRay *nxt = (DevRay *)bufs[which ^ 1]->ensure(std::max<uint64_t>(n_cur, 1) * sizeof(DevRay));
+        uint64_t n_out = 0;
+        for (uint64_t c = 0; c < n_cur; c += kMaxChunk) {
+            int n = (int)std::min<uint64_t>(kMaxChunk, n_cur - c);
+            if (!trace_chunk((int)si, n, nullptr, 0, cur + c, last_prev, nxt, &n_out)) return -1;
+            if (cb && !cb((st_uint_t)traced_total, (st_uint_t)(c + n), (st_uint_t)last_prev, (st_uint_t)si + 1,
+                          (st_uint_t)nst, cbdata)) {
+                cancelled = true;
+                break;
+            }
+        }
+        which ^= 1;
+        cur = nxt;
+        n_cur = n_out;
+        prev_has = n_cur > 0;
+        if (prev_has) last_prev = last_raynum_of(cur, n_cur);
+    }
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (fs.active) {
+        flux_host.assign((size_t)fs.nx * fs.ny, 0u);
+        HC(hipMemcpyAsync(flux_host.data(), d_flux.p, flux_host.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          s));
+        HC(hipStreamSynchronize(s));
+    }
+    timing.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return (int)total_records();
+}
+
+}  // namespace st
+
+using namespace st;
+
+#define CTX(p, r)                       \
+    Context *c = ctx_of(p);             \
+    if (!c) return r;
+#define STAGE(i)                                        \
+    if ((i) >= c->model.stages.size()) return -1;      \
+    HStage &stg = c->model.stages[i];
+#define ELEM(i)                                         \
+    if ((i) >= stg.elements.size()) return -1;         \
+    HElement &el = stg.elements[i];
+
+// ------------------------------------------------------------- contexts ---
+STCORE_API st_context_t st_create_context() {
+    try {
+        return reinterpret_cast<st_context_t>(new Context);
+    } catch (const std::exception &e) {
+        fprintf(stderr, "st_create_context: %s\n", e.what());
+        return nullptr;
+    }
+}
+STCORE_API int st_free_context(st_context_t p) {
+    CTX(p, -1);
+    delete c;
+    return 1;
+}
+STCORE_API int st_num_messages(st_context_t p) {
+    CTX(p, -1);
+    return (int)c->messages.size();
+}
+STCORE_API const char *st_message(st_context_t p, st_uint_t idx) {
+    CTX(p, nullptr);
+    return idx < c->messages.size() ? c->messages[idx].c_str() : nullptr;
+}
+STCORE_API void st_clear_messages(st_context_t p) {
+    Context *c = ctx_of(p);
+    if (c) c->messages.clear();
+}
+
+// --------------------------------------------------------------- optics ---
+STCORE_API int st_num_optics(st_context_t p) {
+    CTX(p, -1);
+    return (int)c->model.optics.size();
+}
+STCORE_API int st_add_optic(st_context_t p, const char *name) {
+    CTX(p, -1);
+    c->model.optics.emplace_back();
+    c->model.optics.back().name = name ? name : "";
+    return (int)c->model.optics.size() - 1;
+}
+STCORE_API int st_delete_optic(st_context_t p, st_uint_t idx) {
+    CTX(p, -1);
+    if (idx >= c->model.optics.size()) return -1;
+    c->model.optics.erase(c->model.optics.begin() + idx);
+    return 1;
+}
+STCORE_API int st_clear_optics(st_context_t p) {
+    CTX(p, -1);
+    c->model.optics.clear();
+    return 1;
+}
+STCORE_API int st_optic(st_context_t p, st_uint_t idx, int fb, char dist, int optnum, int apgr, int order,
+                        double rreal, double rimag, double ref, double tra, double gratingab12[3], double rmsslope,
+                        double rmsspec, int userefltable, int refl_npoints, double *refl_angles, double *refls,
+                        int usetranstable, int trans_npoints, double *trans_angles, double *transs) {
+    CTX(p, -1);
+    if (idx >= c->model.optics.size()) return -1;
+    HFace &f = (fb == 2) ? c->model.optics[idx].back : c->model.optics[idx].front;
+    f.dist = dist;
+    f.optnum = optnum;
+    f.apgr = apgr;
+    f.order = order;
+    f.refr[0] = rreal;
+    f.refr[1] = rimag;
+    f.reflectivity = ref;
+    f.transmissivity = tra;
+    // the reference copies four grating coefficients (coretrace/stapi.cpp:190-191)
+    for (int i = 0; i < 4; i++) f.ab12[i] = gratingab12 ? gratingab12[i] : 0.0;
+    f.slope = rmsslope;
+    f.spec = rmsspec;
+    if (userefltable && refl_npoints > 0 && refl_angles && refls) {
+        f.use_refl = true;
+        f.refl_ang.assign(refl_angles, refl_angles + refl_npoints);
+        f.refl_val.assign(refls, refls + refl_npoints);
+    }
+    if (usetranstable && trans_npoints > 0 && trans_angles && transs) {
+        f.use_trans = true;
+        f.trans_ang.assign(trans_angles, trans_angles + trans_npoints);
+        f.trans_val.assign(transs, transs + trans_npoints);
+    }
+    return 1;
+}
+
+// --------------------------------------------------------------- stages ---
+STCORE_API int st_num_stages(st_context_t p) {
+    CTX(p, -1);
+    return (int)c->model.stages.size();
+}
+STCORE_API int st_add_stage(st_context_t p) {
+    CTX(p, -1);
+    c->model.stages.emplace_back();
+    return (int)c->model.stages.size() - 1;
+}
+STCORE_API int st_add_stages(st_context_t p, st_uint_t num) {
+    CTX(p, -1);
+    for (st_uint_t i = 0; i < num; i++) c->model.stages.emplace_back();
+    return (int)num;
+}
+STCORE_API int st_delete_stage(st_context_t p, st_uint_t idx) {
+    CTX(p, -1);
+    if (idx >= c->model.stages.size()) return -1;
+    c->model.stages.erase(c->model.stages.begin() + idx);
+    return 1;
+}
+STCORE_API int st_clear_stages(st_context_t p) {
+    CTX(p, -1);
+    c->model.stages.clear();
+    return 1;
+}
+STCORE_API int st_stage_xyz(st_context_t p, st_uint_t i, double x, double y, double z) {
+    CTX(p, -1);
+    STAGE(i);
+    stg.origin[0] = x; stg.origin[1] = y; stg.origin[2] = z;
+    return 1;
+}
+STCORE_API int st_stage_aim(st_context_t p, st_uint_t i, double x, double y, double z) {
+    CTX(p, -1);
+    STAGE(i);
+    stg.aim[0] = x; stg.aim[1] = y; stg.aim[2] = z;
+    return 1;
+}
+STCORE_API int st_stage_zrot(st_context_t p, st_uint_t i, double z) {
+    CTX(p, -1);
+    STAGE(i);
+    stg.zrot = z;
+    return 1;
+}
+STCORE_API int st_stage_flags(st_context_t p, st_uint_t i, int virt, int multihit, int tracethrough) {
+    CTX(p, -1);
+    STAGE(i);
+    stg.virt = virt != 0;
+    stg.multihit = multihit != 0;
+    stg.tracethrough = tracethrough != 0;
+    return 1;
+}
+
+// ------------------------------------------------------------- elements ---
+STCORE_API int st_num_elements(st_context_t p, st_uint_t i) {
+    CTX(p, -1);
+    STAGE(i);
+    return (int)stg.elements.size();
+}
+STCORE_API int st_add_element(st_context_t p, st_uint_t i) {
+    CTX(p, -1);
+    STAGE(i);
+    stg.elements.emplace_back();
+    return (int)stg.elements.size() - 1;
+}
+STCORE_API int st_add_elements(st_context_t p, st_uint_t i, st_uint_t num) {
+    CTX(p, -1);
+    if (num < 1) return -1;
+    STAGE(i);
+    stg.elements.resize(stg.elements.size() + num);
+    return (int)num;
+}
+STCORE_API int st_delete_element(st_context_t p, st_uint_t i, st_uint_t idx) {
+    CTX(p, -1);
+    STAGE(i);
+    if (idx >= stg.elements.size()) return -1;
+    stg.elements.erase(stg.elements.begin() + idx);
+    return 1;
+}
+STCORE_API int st_clear_elements(st_context_t p, st_uint_t i) {
+    CTX(p, -1);
+    STAGE(i);
+    stg.elements.clear();
+    return 1;
+}
+STCORE_API int st_element_enabled(st_context_t p, st_uint_t i, st_uint_t j, int en) {
+    CTX(p, -1);
+    STAGE(i);
+    ELEM(j);
+    el.enabled = en != 0;
+    return 1;
+}
+STCORE_API int st_element_xyz(st_context_t p, st_uint_t i, st_uint_t j, double x, double y, double z) {
+    CTX(p, -1);
+    STAGE(i);
+    ELEM(j);
+    el.origin[0] = x; el.origin[1] = y; el.origin[2] = z;
+    return 1;
+}
+STCORE_API int st_element_aim(st_context_t p, st_uint_t i, st_uint_t j, double x, double y, double z) {
+    CTX(p, -1);
+    STAGE(i);
+    ELEM(j);
+    el.aim[0] = x; el.aim[1] = y; el.aim[2] = z;
+    return 1;
+}
+STCORE_API int st_element_zrot(st_context_t p, st_uint_t i, st_uint_t j, double z) {
+    CTX(p, -1);
+    STAGE(i);
+    ELEM(j);
+    el.zrot = z;
+    return 1;
+}
+STCORE_API int st_element_aperture(st_context_t p, st_uint_t i, st_uint_t j, char ap) {
+    CTX(p, -1);
+    STAGE(i);
+    ELEM(j);
+    el.shape = ap;
+    return 1;
+}
+STCORE_API int st_element_aperture_params(st_context_t p, st_uint_t i, st_uint_t j, double params[8]) {
+    CTX(p, -1);
+    STAGE(i);
+    ELEM(j);
+    for (int k = 0; k < 8; k++) el.p[k] = params[k];
+    return 1;
+}
+STCORE_API int st_element_surface(st_context_t p, st_uint_t i, st_uint_t j, char surf) {
+    CTX(p, -1);
+    STAGE(i);
+    ELEM(j);
+    el.surf = surf;
+    return 1;
+}
+STCORE_API int st_element_surface_params(st_context_t p, st_uint_t i, st_uint_t j, double params[8]) {
+    CTX(p, -1);
+    STAGE(i);
+    ELEM(j);
+    std::string msg;
+    if (!translate_surface_params(el, params, msg)) {
+        c->errlog("%s", msg.c_str());
+        return -1;
+    }
+    return 1;
+}
+STCORE_API int st_element_surface_file(st_context_t p, st_uint_t i, st_uint_t j, const char *file) {
+    CTX(p, -1);
+    STAGE(i);
+    ELEM(j);
+    std::string msg;
+    if (!read_surface_file(el, file, msg)) {
+        c->errlog("%s", msg.c_str());
+        return -1;
+    }
+    return 1;
+}
+STCORE_API int st_element_interaction(st_context_t p, st_uint_t i, st_uint_t j, int type) {
+    CTX(p, -1);
+    STAGE(i);
+    ELEM(j);
+    el.interaction = type;
+    return 1;
+}
+STCORE_API int st_element_optic(st_context_t p, st_uint_t i, st_uint_t j, const char *name) {
+    CTX(p, -1);
+    STAGE(i);
+    ELEM(j);
+    el.optic_name = name ? name : "";
+    return 1;
+}
+
+// ------------------------------------------------------------------ sun ---
+STCORE_API int st_sun(st_context_t p, int point_source, char shape, double sigma) {
+    CTX(p, -1);
+    c->model.sun.point_source = point_source != 0;
+    c->model.sun.shape = shape;
+    c->model.sun.sigma = sigma;
+    return 1;
+}
+STCORE_API int st_sun_xyz(st_context_t p, double x, double y, double z) {
+    CTX(p, -1);
+    c->model.sun.origin[0] = x; c->model.sun.origin[1] = y; c->model.sun.origin[2] = z;
+    return 1;
+}
+// coretrace/stapi.cpp:497-527
+STCORE_API int st_sun_position(st_context_t p, double lat, double day, double hour, double *x, double *y,
+                               double *z) {
+    (void)p;
+    const double d2r = M_PI / 180;
+    double decl = 180 / M_PI * asin(0.39795 * cos(0.98563 * M_PI / 180 * (day - 173)));
+    double ha = 15 * (hour - 12);
+    double el = 180 / M_PI * asin(sin(decl * d2r) * sin(lat * d2r) + cos(decl * d2r) * cos(ha * d2r) * cos(lat * d2r));
+    double az = 180 / M_PI *
+                acos((sin(d2r * decl) * cos(d2r * lat) - cos(d2r * decl) * sin(d2r * lat) * cos(d2r * ha)) /
+                         cos(d2r * el) +
+                     0.0000000001);
+    if (sin(ha * d2r) > 0.0) az = 360 - az;
+    *x = -sin(az * d2r) * cos(el * d2r);
+    *y = sin(el * d2r);
+    *z = cos(az * d2r) * cos(el * d2r);
+    return 1;
+}
+// coretrace/stapi.cpp:529-560 (table mirrored about zero)
+STCORE_API int st_sun_userdata(st_context_t p, st_uint_t n, double angle[], double intensity[]) {
+    CTX(p, -1);
+    if (n <= 2) return -1;
+    HSun &s = c->model.sun;
+    s.max_angle = 0;
+    s.max_intensity = 0;
+    s.angle.assign(2 * n - 1, 0.0);
+    s.intensity.assign(2 * n - 1, 0.0);
+    for (st_uint_t i = 0; i < n; i++) {
+        s.angle[n + i - 1] = angle[i];
+        s.intensity[n + i - 1] = intensity[i];
+        if (angle[i] > s.max_angle) s.max_angle = angle[i];
+        if (intensity[i] > s.max_intensity) s.max_intensity = intensity[i];
+    }
+    for (st_uint_t i = 0; i < n - 1; i++) {
+        s.angle[i] = -angle[n - i - 1];
+        s.intensity[i] = intensity[n - i - 1];
+    }
+    return (int)n;
+}
+
+// -------------------------------------------------------------- results ---
+STCORE_API int st_num_intersections(st_context_t p) {
+    CTX(p, -1);
+    return (int)c->total_records();
+}
+#define MATERIALIZE()                                      \
+    try {                                                  \
+        c->materialize();                                  \
+    } catch (const std::exception &e) {                    \
+        c->errlog("result copy failed: %s", e.what());     \
+        return -2;                                         \
+    }
+STCORE_API int st_locations(st_context_t p, double *x, double *y, double *z) {
+    CTX(p, -1);
+    MATERIALIZE();
+    for (size_t i = 0; i < c->host_recs.size(); i++) {
+        x[i] = c->host_recs[i].pos[0]; y[i] = c->host_recs[i].pos[1]; z[i] = c->host_recs[i].pos[2];
+    }
+    return 1;
+}
+STCORE_API int st_cosines(st_context_t p, double *x, double *y, double *z) {
+    CTX(p, -1);
+    MATERIALIZE();
+    for (size_t i = 0; i < c->host_recs.size(); i++) {
+        x[i] = c->host_recs[i].cos[0]; y[i] = c->host_recs[i].cos[1]; z[i] = c->host_recs[i].cos[2];
+    }
+    return 1;
+}
+STCORE_API int st_elementmap(st_context_t p, int *m) {
+    CTX(p, -1);
+    MATERIALIZE();
+    for (size_t i = 0; i < c->host_recs.size(); i++) m[i] = c->host_recs[i].element;
+    return 1;
+}
+STCORE_API int st_stagemap(st_context_t p, int *m) {
+    CTX(p, -1);
+    MATERIALIZE();
+    // stage of each merged record: recover from the merge layout
+    const size_t B = 8192;
+    std::vector<int> st_of;
+    for (size_t s = 0; s < c->recs.size(); s++) {
+        std::vector<int> src(c->recs[s].total, (int)s + 1);
+        size_t fa = (st_of.size() / B) * B, fs = (src.size() / B) * B;
+        std::vector<int> out;
+        out.reserve(st_of.size() + src.size());
+        out.insert(out.end(), st_of.begin(), st_of.begin() + fa);
+        out.insert(out.end(), src.begin(), src.begin() + fs);
+        out.insert(out.end(), st_of.begin() + fa, st_of.end());
+        out.insert(out.end(), src.begin() + fs, src.end());
+        st_of.swap(out);
+    }
+    for (size_t i = 0; i < st_of.size(); i++) m[i] = st_of[i];
+    return 1;
+}
+STCORE_API int st_raynumbers(st_context_t p, int *m) {
+    CTX(p, -1);
+    MATERIALIZE();
+    for (size_t i = 0; i < c->host_recs.size(); i++) m[i] = (int)c->host_recs[i].raynum;
+    return 1;
+}
+STCORE_API int st_sun_stats(st_context_t p, double *xmin, double *xmax, double *ymin, double *ymax, int *nsun) {
+    CTX(p, -1);
+    if (xmin) *xmin = c->model.sun.minx;
+    if (xmax) *xmax = c->model.sun.maxx;
+    if (ymin) *ymin = c->model.sun.miny;
+    if (ymax) *ymax = c->model.sun.maxy;
+    if (nsun) *nsun = (int)c->sun_ray_count;
+    return 1;
+}
+
+// ----------------------------------------------------------- simulation ---
+STCORE_API int st_sim_params(st_context_t p, int raycount, int maxcount, int dyn) {
+    CTX(p, -1);
+    c->raycount = raycount;
+    c->raymax = maxcount;
+    c->dyn_group = dyn != 0;
+    return 1;
+}
+STCORE_API int st_sim_errors(st_context_t p, int sun, int optics) {
+    CTX(p, -1);
+    c->err_sun = sun != 0;
+    c->err_opt = optics != 0;
+    return 1;
+}
+STCORE_API int st_sim_run(st_context_t p, unsigned int seed, st_callback_t cb, void *data) {
+    CTX(p, -1);
+    return c->run(seed, cb, data);
+}
+
+// ------------------------------------------------------------ utilities ---
+STCORE_API void st_calc_euler_angles(double origin[3], double aimpoint[3], double zrot, double euler[3]) {
+    euler[0] = euler[1] = euler[2] = 0;
+    double e[3];
+    if (euler_from_aim(origin, aimpoint, zrot, e))
+        for (int i = 0; i < 3; i++) euler[i] = e[i];
+}
+STCORE_API void st_transform_to_local(double posref[3], double cosref[3], double origin[3], double R[3][3],
+                                      double posloc[3], double cosloc[3]) {
+    double d[3] = {posref[0] - origin[0], posref[1] - origin[1], posref[2] - origin[2]};
+    double cc[3] = {cosref[0], cosref[1], cosref[2]};
+    for (int i = 0; i < 3; i++) {
+        posloc[i] = R[i][0] * d[0] + R[i][1] * d[1] + R[i][2] * d[2];
+        cosloc[i] = R[i][0] * cc[0] + R[i][1] * cc[1] + R[i][2] * cc[2];
+    }
+}
+STCORE_API void st_transform_to_reference(double posloc[3], double cosloc[3], double origin[3], double RT[3][3],
+                                          double posref[3], double cosref[3]) {
+    double pl[3] = {posloc[0], posloc[1], posloc[2]}, cl[3] = {cosloc[0], cosloc[1], cosloc[2]};
+    for (int i = 0; i < 3; i++) {
+        posref[i] = RT[i][0] * pl[0] + RT[i][1] * pl[1] + RT[i][2] * pl[2] + origin[i];
+        cosref[i] = RT[i][0] * cl[0] + RT[i][1] * cl[1] + RT[i][2] * cl[2];
+    }
+}
+STCORE_API void st_matrix_vector_mult(double m[3][3], double v[3], double mxv[3]) {
+    double vv[3] = {v[0], v[1], v[2]};
+    for (int i = 0; i < 3; i++) mxv[i] = m[i][0] * vv[0] + m[i][1] * vv[1] + m[i][2] * vv[2];
+}
+STCORE_API void st_calc_transform_matrices(double euler[3], double R[3][3], double RT[3][3]) {
+    double m[9];
+    euler_matrix(euler, m);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            R[i][j] = m[3 * i + j];
+            RT[j][i] = m[3 * i + j];
+        }
+}
+STCORE_API void st_matrix_transpose(double in[3][3], double out[3][3]) {
+    double t[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) t[i][j] = in[j][i];
+    memcpy(out, t, sizeof t);
+}
+
+// --------------------------------------------------- st_amd extensions ----
+STCORE_API int st_amd_set_precision(st_context_t p, int bits) {
+    CTX(p, -1);
+    if (bits != 32 && bits != 64) return -1;
+    c->precision = bits;
+    return 1;
+}
+STCORE_API int st_amd_set_cull(st_context_t p, int on) {
+    CTX(p, -1);
+    c->cull = on != 0;
+    return 1;
+}
+STCORE_API int st_amd_flux_request(st_context_t p, int stage, int elem, int nx, int ny, int absorbed_only,
+                                   double xmin, double xmax, double ymin, double ymax) {
+    CTX(p, -1);
+    if (stage < 0) {
+        c->flux = FluxSpec{};
+        return 1;
+    }
+    if ((size_t)stage >= c->model.stages.size() || elem < 0 ||
+        (size_t)elem >= c->model.stages[stage].elements.size() || nx <= 1 || ny <= 1)
+        return -1;
+    const HElement &e = c->model.stages[stage].elements[elem];
+    char sf = (char)tolower(e.surf), ap = (char)tolower(e.shape);
+    if (sf != 't' && sf != 'f' && !(ap == 'l' && sf == 's')) return -1;
+    FluxSpec f{};
+    f.active = 1;
+    f.stage = stage;
+    f.elem = elem + 1;
+    f.nx = nx;
+    f.ny = ny;
+    f.absorbed_only = absorbed_only != 0;
+    double gx = xmax - xmin, gy = ymax - ymin;
+    // cylinder and partial-cylinder unrolling (app/src/project.cpp:1204-1222)
+    if (sf == 't' || sf == 's') {
+        double Rr = 1.0 / (sf == 't' ? e.curv_rev : e.vcx);
+        f.radius = Rr;
+        if (sf == 't') {
+            f.kind = 1;
+            xmin = -M_PI * Rr;
+            gx = 2.0 * M_PI * Rr;
+        } else {
+            f.kind = 2;
+            double amin = asin(e.p[0] / Rr), amax = asin(e.p[1] / Rr);
+            xmin = Rr * amin;
+            gx = (amax - amin) * Rr;
+        }
+    }
+    if (!(gx > 0) || !(gy > 0)) return -1;
+    f.xmin = xmin;
+    f.ymin = ymin;
+    f.bx = gx / nx;
+    f.by = gy / ny;
+    double eu[3];
+    if (!euler_from_aim(e.origin, e.aim, e.zrot, eu)) return -1;
+    euler_matrix(eu, f.R);
+    for (int k = 0; k < 3; k++) f.org[k] = e.origin[k];
+    c->flux = f;
+    return 1;
+}
+STCORE_API int st_amd_flux_result(st_context_t p, unsigned int *counts, double *bin_dims) {
+    CTX(p, -1);
+    if (!c->flux.active) return -1;
+    for (size_t i = 0; i < c->flux_host.size(); i++) counts[i] = c->flux_host[i];
+    if (bin_dims) {
+        bin_dims[0] = c->flux.xmin;
+        bin_dims[1] = c->flux.ymin;
+        bin_dims[2] = c->flux.bx;
+        bin_dims[3] = c->flux.by;
+    }
+    return (int)c->flux_host.size();
+}
+STCORE_API int st_amd_ray_sun_index(st_context_t p, unsigned long long *out, int n) {
+    CTX(p, -1);
+    if (n < 0) return -1;
+    size_t m = std::min<size_t>((size_t)n, c->n_hits_done);
+    if (m == 0) return 0;
+    try {
+        HC(hipMemcpyAsync(out, c->d_hits.p, m * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+        HC(hipStreamSynchronize(c->stream));
+    } catch (const std::exception &e) {
+        c->errlog("%s", e.what());
+        return -1;
+    }
+    return (int)m;
+}
+// Timing of the last run: [0] mask ms, [1..kMaxStages] trace ms per stage,
+// [17] other ms, [18] total ms, [19] sun rays, [20..] rays per stage,
+// [36..] records per stage.
+STCORE_API int st_amd_timing(st_context_t p, double *out, int n) {
+    CTX(p, -1);
+    std::vector<double> v;
+    v.push_back(c->timing.mask_ms);
+    for (int i = 0; i < kMaxStages; i++) v.push_back(c->timing.trace_ms[i]);
+    v.push_back(c->timing.other_ms);
+    v.push_back(c->timing.total_ms);
+    v.push_back((double)c->timing.sun_rays);
+    for (int i = 0; i < kMaxStages; i++) v.push_back((double)c->timing.rays_in[i]);
+    for (int i = 0; i < kMaxStages; i++) v.push_back((double)c->timing.records[i]);
+    int m = std::min<int>(n, (int)v.size());
+    for (int i = 0; i < m; i++) out[i] = v[i];
+    return (int)v.size();
+}

@@ -1,0 +1,1028 @@
+// Device-side ray algebra of the trace loop, templated on the bulk
+// arithmetic type R (double for the FP64 path, float for the FP32 path).
+// Frame transforms between global, stage and element frames are always
+// double (positions reach kilometres and the 1e-5 m self-intersection nudge
+// of coretrace/raytrace.cpp:729-731 is below the float ulp there); the torus
+// quartic is always double; the Newton start plane is found in double.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "st_types.h"
+
+namespace st {
+
+#define STD __device__ __forceinline__
+
+static constexpr double kDeg = 0.017453292519943295;  // coretrace/types.h:64
+
+// -------------------------------------------------------------- math shims
+template <typename R> struct M;
+template <> struct M<double> {
+    static STD double sqrt_(double x) { return ::sqrt(x); }
+    static STD double sin_(double x) { return ::sin(x); }
+    static STD double cos_(double x) { return ::cos(x); }
+    static STD double atan2_(double y, double x) { return ::atan2(y, x); }
+    static STD double asin_(double x) { return ::asin(x); }
+    static STD double acos_(double x) { return ::acos(x); }
+    static STD double log_(double x) { return ::log(x); }
+    static STD double pow_(double x, double y) { return ::pow(x, y); }
+    static STD double fabs_(double x) { return ::fabs(x); }
+};
+template <> struct M<float> {
+    static STD float sqrt_(float x) { return ::sqrtf(x); }
+    static STD float sin_(float x) { return ::__sinf(x); }
+    static STD float cos_(float x) { return ::__cosf(x); }
+    static STD float atan2_(float y, float x) { return ::atan2f(y, x); }
+    static STD float asin_(float x) { return ::asinf(x); }
+    static STD float acos_(float x) { return ::acosf(x); }
+    static STD float log_(float x) { return ::__logf(x); }
+    static STD float pow_(float x, float y) { return ::powf(x, y); }
+    static STD float fabs_(float x) { return ::fabsf(x); }
+};
+
+// ------------------------------------------------------------------ RNG ---
+// Philox4x32-10 (Salmon et al., SC'11).  Uniform k of the stream
+// (seed, stage, id) comes from the block with counter (id_lo, id_hi, stage,
+// k/2) and key (seed, 0x53544F52); words 2(k%2), 2(k%2)+1 make a 64-bit w,
+// u = (w >> 11) 2^-53 in double, (w >> 40) 2^-24 in float.
+STD void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        uint32_t lo0 = 0xD2511F53u * c[0];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        uint32_t lo1 = 0xCD9E8D57u * c[2];
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+template <typename R>
+struct Rng {
+    uint32_t seed, stage, id_lo, id_hi;
+    uint32_t k;
+    uint32_t spare_lo, spare_hi;
+    bool have_saved;
+    R saved;
+
+    STD void begin(uint32_t seed_, uint32_t stage_, uint64_t id) {
+        seed = seed_; stage = stage_; id_lo = (uint32_t)id; id_hi = (uint32_t)(id >> 32);
+        k = 0; have_saved = false; saved = R(0);
+    }
+    STD uint64_t next_word() {
+        uint64_t w;
+        if (k & 1u) {
+            w = ((uint64_t)spare_hi << 32) | spare_lo;
+        } else {
+            uint32_t c[4] = {id_lo, id_hi, stage, k >> 1};
+            philox10(c, seed, 0x53544F52u);
+            w = ((uint64_t)c[1] << 32) | c[0];
+            spare_lo = c[2]; spare_hi = c[3];
+        }
+        ++k;
+        return w;
+    }
+    STD R uniform();
+    // full 53-bit draw regardless of R (sun-plane positions span kilometres)
+    STD double uniform_d() { return (double)(next_word() >> 11) * (1.0 / 9007199254740992.0); }
+    // Marsaglia polar with one cached deviate, the algorithm of libstdc++'s
+    // normal_distribution (bits/random.tcc:1802-1835): y*m first, x*m cached.
+    STD R normal(R mean, R sd) {
+        R ret;
+        if (have_saved) {
+            have_saved = false;
+            ret = saved;
+        } else {
+            R x, y, r2;
+            do {
+                x = R(2) * uniform() - R(1);
+                y = R(2) * uniform() - R(1);
+                r2 = x * x + y * y;
+            } while (r2 > R(1) || r2 == R(0));
+            R mult = M<R>::sqrt_(R(-2) * M<R>::log_(r2) / r2);
+            saved = x * mult;
+            have_saved = true;
+            ret = y * mult;
+        }
+        return mean + ret * sd;
+    }
+};
+template <> STD double Rng<double>::uniform() {
+    return (double)(next_word() >> 11) * (1.0 / 9007199254740992.0);
+}
+template <> STD float Rng<float>::uniform() {
+    return (float)(uint32_t)(next_word() >> 40) * (1.0f / 16777216.0f);
+}
+
+// ------------------------------------------------------------ transforms --
+STD void mat_vec(const double *Mx, const double v[3], double o[3]) {
+    o[0] = Mx[0] * v[0] + Mx[1] * v[1] + Mx[2] * v[2];
+    o[1] = Mx[3] * v[0] + Mx[4] * v[1] + Mx[5] * v[2];
+    o[2] = Mx[6] * v[0] + Mx[7] * v[1] + Mx[8] * v[2];
+}
+STD void mat_t_vec(const double *Mx, const double v[3], double o[3]) {
+    o[0] = Mx[0] * v[0] + Mx[3] * v[1] + Mx[6] * v[2];
+    o[1] = Mx[1] * v[0] + Mx[4] * v[1] + Mx[7] * v[2];
+    o[2] = Mx[2] * v[0] + Mx[5] * v[1] + Mx[8] * v[2];
+}
+// coretrace/mathproc.cpp:115-173 with R the reference->local rotation
+STD void to_local(const double p[3], const double c[3], const double org[3], const double *Rm, double pl[3],
+                  double cl[3]) {
+    double d[3] = {p[0] - org[0], p[1] - org[1], p[2] - org[2]};
+    double cc[3] = {c[0], c[1], c[2]};
+    mat_vec(Rm, d, pl);
+    mat_vec(Rm, cc, cl);
+}
+STD void to_ref(const double pl[3], const double cl[3], const double org[3], const double *Rm, double p[3],
+                double c[3]) {
+    double t[3], cc[3] = {cl[0], cl[1], cl[2]};
+    mat_t_vec(Rm, pl, t);
+    mat_t_vec(Rm, cc, c);
+    p[0] = t[0] + org[0];
+    p[1] = t[1] + org[1];
+    p[2] = t[2] + org[2];
+}
+template <typename R>
+STD R dot3(const R a[3], const R b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// ----------------------------------------------------- aperture tests -----
+// Final aperture test (coretrace/determineelementintersectionnew.cpp:104-623)
+// and, with plane = true, the aperture-plane variant used to choose the
+// Newton start (coretrace/intersect.cpp:272-583; its rectangle test never
+// fails, :402,408).
+template <typename R>
+STD bool in_tri(R x1, R y1, R x2, R y2, R x3, R y3, R xt, R yt) {
+    R a = (x1 - xt) * (y2 - yt) - (x2 - xt) * (y1 - yt);
+    R b = (x2 - xt) * (y3 - yt) - (x3 - xt) * (y2 - yt);
+    R c = (x3 - xt) * (y1 - yt) - (x1 - xt) * (y3 - yt);
+    bool sa = a >= 0, sb = b >= 0, sc = c >= 0;
+    return sa == sb && sb == sc;
+}
+
+template <typename R>
+STD bool aperture_contains(const DevElem &e, R x, R y, bool plane) {
+    const R A = (R)e.p[0], B = (R)e.p[1], C = (R)e.p[2];
+    R r = M<R>::sqrt_(x * x + y * y);
+    const R c30 = (R)0.86602540378443871, s30 = (R)0.49999999999999994;  // cos, sin of 30 deg
+    const R sl60 = (R)1.7320508075688767;
+    switch (e.ap) {
+    case AP_CIRCLE:
+        return !(r > A / R(2));
+    case AP_HEX: {
+        R Ro = A / R(2);
+        if (r > Ro) return false;
+        R Ri = Ro * c30;
+        if (r <= Ri) return true;
+        R XL = M<R>::sqrt_(Ro * Ro - Ri * Ri);
+        if (x <= Ro && x > XL) {
+            R Y1 = sl60 * (x - Ro);
+            return y >= Y1 && y <= -Y1;
+        }
+        if (x <= XL && x >= -XL) return y >= -Ri && y <= Ri;
+        if (x < -XL && x >= -Ro) {
+            R Y3 = sl60 * (x + Ro);
+            return y >= -Y3 && y <= Y3;
+        }
+        return false;
+    }
+    case AP_TRI: {
+        R Ro = A / R(2);
+        if (r > Ro) return false;
+        R Ri = Ro * s30;
+        if (r <= Ri) return true;
+        if (x <= Ro && x > R(0)) {
+            R Y1 = -sl60 * (x - Ri / c30);
+            return y <= Y1 && y >= -Ri;
+        }
+        if (x >= -Ro && x <= R(0)) {
+            R Y3 = sl60 * (x + Ri / c30);
+            return y >= -Ri && y <= Y3;
+        }
+        return false;
+    }
+    case AP_RECT:
+        if (plane) return true;
+        if (x > A / R(2) || x < -A / R(2)) return false;
+        if (y > B / R(2) || y < -B / R(2)) return false;
+        return true;
+    case AP_ANNULUS: {
+        bool radial = plane || !(e.p[0] == 0.0 && e.p[1] == 0.0);
+        if (radial && (r < A || r > B)) return false;
+        R half = C * (R)kDeg / R(2);
+        if (x >= R(0)) {
+            R a = M<R>::asin_(y / r);
+            return !(a > half || a < -half);
+        }
+        if (x < R(0)) {
+            if (y >= R(0) && (M<R>::acos_(y / r) + (R)(M_PI / 2.0)) > half) return false;
+            if (y < R(0) && (-M<R>::acos_(-y / r) - (R)(M_PI / 2.0)) < -half) return false;
+            return true;
+        }
+        return false;
+    }
+    case AP_LINE:
+        if (!(e.p[0] == 0.0 && e.p[1] == 0.0)) {
+            if (x < A || x > B) return false;
+        }
+        return !(y < -C / R(2) || y > C / R(2));
+    case AP_ITRI:
+        return in_tri<R>((R)e.p[0], (R)e.p[1], (R)e.p[2], (R)e.p[3], (R)e.p[4], (R)e.p[5], x, y);
+    case AP_QUAD:
+        return in_tri<R>((R)e.p[0], (R)e.p[1], (R)e.p[2], (R)e.p[3], (R)e.p[4], (R)e.p[5], x, y) ||
+               in_tri<R>((R)e.p[0], (R)e.p[1], (R)e.p[4], (R)e.p[5], (R)e.p[6], (R)e.p[7], x, y);
+    default:
+        return false;
+    }
+}
+
+// ------------------------------------------------------- closed forms -----
+template <typename R>
+STD R sqr_grad(R K, R d, R a2) { return (((((R(2) * K) * d) / a2) * R(2)) * K) * d / a2; }
+
+// Ray-quadric intersection (coretrace/quadricsurfaceclosedform.cpp:58-308).
+// Returns false on a miss (ErrorFlag 1).
+template <typename R>
+STD bool quadric(const DevElem &e, const R P[3], const R D[3], R X[3], R DF[3], R *path) {
+    R Xc = 0, Yc = 0, Zc = 0, Kx = 0, Ky = 0, Kz = 0, a2 = 1, b2 = 1, c2 = 1, A = 0, B = 0, C = 0;
+    switch (e.surf) {
+    case SURF_SPHERE: {
+        Kx = 1; Ky = (e.stype == 7) ? R(0) : R(1); Kz = 1;
+        R r = R(1) / (R)e.vcx;
+        Zc = r;
+        R xd = P[0], yd = P[1], zd = P[2] - Zc;
+        A = D[0] * D[0] * Kx / a2 + D[1] * D[1] * Ky / b2 + D[2] * D[2] * Kz / c2;
+        B = R(2) * (Kx * xd * D[0] / a2 + Ky * yd * D[1] / b2 + Kz * zd * D[2] / c2);
+        C = Kx * xd * xd / a2 + Ky * yd * yd / b2 + Kz * zd * zd / c2 - r * r;
+        break;
+    }
+    case SURF_PARABOLA: {
+        a2 = R(2) / (R)e.vcx;
+        b2 = e.vcy > 0.0 ? R(2) / (R)e.vcy : R(1e10);
+        c2 = 1;
+        Kx = 1; Ky = (e.stype == 7) ? R(0) : R(1); Kz = 0;
+        R xd = P[0], yd = P[1], zd = P[2];
+        A = D[0] * D[0] * Kx / a2 + D[1] * D[1] * Ky / b2;
+        B = R(2) * (Kx * xd * D[0] / a2 + Ky * yd * D[1] / b2) - D[2] / c2;
+        C = Kx * xd * xd / a2 + Ky * yd * yd / b2 - zd / c2;
+        break;
+    }
+    case SURF_CONIC: {
+        Kx = 1; Ky = 1; Kz = (R)e.kappa;
+        Zc = R(1) / (R)e.kappa / (R)e.vcx;
+        R xd = P[0], yd = P[1], zd = P[2] - Zc;
+        A = D[0] * D[0] * Kx / a2 + D[1] * D[1] * Ky / b2 + D[2] * D[2] * Kz / c2;
+        B = R(2) * (Kx * xd * D[0] / a2 + Ky * yd * D[1] / b2 + Kz * zd * D[2] / c2);
+        C = Kx * xd * xd / a2 + Ky * yd * yd / b2 + Kz * zd * zd / c2 - R(1) / (R)e.kappa / ((R)e.vcx * (R)e.vcx);
+        break;
+    }
+    default: {  // SURF_CYLINDER
+        Kx = 1; Ky = 0; Kz = 1;
+        R r = R(1) / (R)e.curv_rev;
+        Zc = r;
+        R xd = P[0], yd = P[1], zd = P[2] - Zc;
+        A = D[0] * D[0] * Kx / a2 + D[1] * D[1] * Ky / b2 + D[2] * D[2] * Kz / c2;
+        B = R(2) * (Kx * xd * D[0] / a2 + Ky * yd * D[1] / b2 + Kz * zd * D[2] / c2);
+        C = Kx * xd * xd / a2 + Ky * yd * yd / a2 + Kz * zd * zd / c2 - r * r;
+        break;
+    }
+    }
+    R t;
+    if (M<R>::fabs_(A) < R(1e-12)) {
+        t = -C / B;
+        if (!(t > R(0))) return false;
+    } else {
+        R disc = (B * B) - R(4) * A * C;
+        if (!((B * B) > R(4) * A * C)) return false;
+        R sq = M<R>::sqrt_(disc);
+        R t1 = (-B + sq) / (R(2) * A);
+        R t2 = (-B - sq) / (R(2) * A);
+        if (t2 > R(0)) {
+            t = t2;
+            R z2 = P[2] + t2 * D[2];
+            if (z2 > (R)e.zap) t = t1;
+            bool len_check = e.surf == SURF_CYLINDER || (e.surf == SURF_SPHERE && e.stype == 7);
+            if (len_check) {
+                R y = P[1] + t * D[1];
+                R half = (R)e.p[2] / R(2);
+                if (y < -half || y > half) t = t1;
+            }
+        } else if (t2 == R(0)) {
+            t = t1;
+        } else if (t2 < R(0) && t1 > R(0)) {
+            t = t1;
+        } else if (t1 <= R(0)) {
+            return false;
+        } else {
+            return false;  // NaN roots
+        }
+    }
+    for (int i = 0; i < 3; i++) X[i] = P[i] + t * D[i];
+    *path = t;
+    if (e.surf == SURF_PARABOLA) {
+        R sm = M<R>::sqrt_(sqr_grad(Kx, X[0] - Xc, a2) + sqr_grad(Ky, X[1] - Yc, b2) + R(1));
+        DF[0] = -(R(2) * Kx * (X[0] - Xc) / a2) / sm;
+        DF[1] = -(R(2) * Ky * (X[1] - Yc) / b2) / sm;
+        DF[2] = R(1) / sm;
+    } else {
+        R sm = M<R>::sqrt_(sqr_grad(Kx, X[0] - Xc, a2) + sqr_grad(Ky, X[1] - Yc, b2) + sqr_grad(Kz, X[2] - Zc, c2));
+        DF[0] = -(R(2) * Kx * (X[0] - Xc) / a2) / sm;
+        DF[1] = -(R(2) * Ky * (X[1] - Yc) / b2) / sm;
+        DF[2] = -(R(2) * Kz * (X[2] - Zc) / c2) / sm;
+    }
+    return true;
+}
+
+// Torus by the Ferrari quartic with a bisected resolvent cubic and the
+// reference's root bookkeeping (coretrace/torusclosedform.cpp:57-188,
+// root432.cpp:53-284, mathproc.cpp:333-348).  Always double.
+struct QuarticState {
+    double a30, a31, a32, a20, a21;
+    double r[5];
+    double im;
+    STD double f(double x) const { return x * x * x + a32 * x * x + a31 * x + a30; }
+    STD void root2() {
+        double w = a21 * a21 - 4 * a20;
+        double q1 = -a21 / 2.0;
+        if (w < 0) {
+            im = ::sqrt(-w) / 2.0;
+            r[1] = q1; r[2] = q1;
+        } else if (w == 0) {
+            r[1] = q1; r[2] = q1; im = 0.0;
+        } else if (w > 0) {
+            double q2 = ::sqrt(w) / 2.0;
+            im = 0.0;
+            r[1] = q1 + q2; r[2] = q1 - q2;
+        }
+    }
+    STD double root3_one() {
+        if (a30 == 0) return 0.0;
+        double am = ::fabs(a30);
+        double t = ::fabs(a31); if (am < t) am = t;
+        t = ::fabs(a32); if (am < t) am = t;
+        t = 0.0;  // a[3][3] is zero
+        double aa, b;
+        if (a30 > 0) { aa = -am - 1; b = 0; } else { aa = 0; b = am + 1; }
+        double xa = aa, xb = b;
+        if (f(aa) == 0) return aa;
+        if (f(b) == 0) return b;
+        double xc;
+        for (int it = 0; it < 2000; ++it) {
+            xc = (xa + xb) / 2.0;
+            double te = f(xc);
+            if (te == 0) break;
+            if (xb - xa < 0.000001) break;
+            if (f(xa) * te > 0) xa = xc; else xb = xc;
+        }
+        return xc;
+    }
+};
+
+STD bool torus(const DevElem &e, const double P[3], const double D[3], double X[3], double DF[3], double *path) {
+    const double Rs = e.cross_r, Ra = e.ann_r;
+    const double Xo = P[0], Yo = P[1], Zo = P[2];
+    const double Ep = D[0], Et = D[1], Rh = D[2];
+    double a44 = ::pow(Ep, 4) + 2.0 * Ep * Ep * (Et * Et + Rh * Rh) + ::pow(Et, 4) + 2.0 * Et * Et * Rh * Rh + ::pow(Rh, 4);
+    double a43 = 4.0 * (Ep * Ep + Et * Et + Rh * Rh) * (Ep * Xo + Et * Yo + Rh * Zo - Rh * Rs);
+    double a42 = Xo * Xo * (6.0 * Ep * Ep + 2.0 * Et * Et + 2.0 * Rh * Rh) + 8.0 * Ep * Xo * (Et * Yo + Rh * Zo - Rh * Rs) +
+                 2.0 * Yo * Yo * (Ep * Ep + 3.0 * Et * Et + Rh * Rh) + 8.0 * Et * Rh * Yo * (Zo - Rs) +
+                 (Ep * Ep + Et * Et + 3.0 * Rh * Rh) * (2.0 * Zo * Zo - 4.0 * Rs * Zo) -
+                 2.0 * Ra * Ra * (Ep * Ep + Et * Et - Rh * Rh) + 4.0 * Rh * Rh * Rs * Rs;
+    double a41 = 4.0 * (Xo * Xo * (Ep * Xo + Et * Yo + Rh * Zo - Rh * Rs) + Yo * Yo * (Ep * Xo + Et * Yo + Rh * Zo - Rh * Rs) +
+                        Zo * Zo * (Ep * Xo + Et * Yo + Rh * Zo - 3.0 * Rh * Rs) - 2.0 * Ep * Rs * Xo * Zo - Ep * Ra * Ra * Xo -
+                        2.0 * Et * Rs * Yo * Zo - Et * Ra * Ra * Yo + Rh * Ra * Ra * (Zo - Rs) + 2.0 * Rh * Rs * Rs * Zo);
+    double a40 = ::pow(Xo, 4) + 2.0 * Xo * Xo * (Yo * Yo + Zo * Zo - 2.0 * Rs * Zo - Ra * Ra) + ::pow(Yo, 4) +
+                 2.0 * Yo * Yo * (Zo * Zo - 2.0 * Rs * Zo - Ra * Ra) + ::pow(Zo, 4) - 4.0 * Rs * Zo * Zo * Zo +
+                 2.0 * Ra * Ra * Zo * Zo + 4.0 * Rs * Rs * Zo * Zo - 4.0 * Ra * Ra * Rs * Zo + ::pow(Ra, 4);
+    // normalisation (root432.cpp Root_4)
+    a40 = a40 / a44; a41 = a41 / a44; a42 = a42 / a44; a43 = a43 / a44;
+    QuarticState Q;
+    for (int i = 0; i < 5; i++) Q.r[i] = 0.0;
+    Q.im = 0.0;
+    double aa = a43, b = a42, c = a41, d = a40;
+    double q = b - (3.0 * aa * aa / 8.0);
+    double rr = c - (aa * b / 2) + (aa * aa * aa / 8.0);
+    double s = d - (aa * c / 4) + (aa * aa * b / 16.0) - (3 * aa * aa * aa * aa / 256.0);
+    Q.a32 = q / 2.0;
+    Q.a31 = (q * q - 4 * s) / 16.0;
+    Q.a30 = -(rr * rr / 64.0);
+    Q.a20 = 0; Q.a21 = 0;
+    if ((rr != 0) || (Q.a31 >= 0)) {
+        Q.r[3] = Q.root3_one();
+    } else {
+        Q.a21 = Q.a32; Q.a20 = Q.a31;
+        Q.root2();
+        Q.r[3] = Q.r[1];
+    }
+    double k = ::sqrt(Q.r[3]);
+    if (k == 0) {
+        rr = ::sqrt(q * q - 4 * s);
+    } else {
+        q = q + (4 * Q.r[3]);
+        rr = rr / (2 * k);
+    }
+    double l = (q - rr) / 2.0, m = (q + rr) / 2.0;
+    Q.a21 = 2 * k; Q.a20 = l;
+    Q.root2();
+    double rv[5];
+    rv[0] = 0.0;
+    rv[3] = Q.r[1] - (a43 / 4);
+    rv[4] = Q.r[2] - (a43 / 4.0);
+    double ii = Q.im;
+    Q.a21 = -2 * k; Q.a20 = m;
+    Q.root2();
+    rv[2] = Q.r[2] - (a43 / 4.0);
+    rv[1] = Q.r[1] - (a43 / 4.0);
+    double im1 = Q.im, im2 = ii;
+    double L = 0.0;
+    if (im1 == 0.0 && im2 == 0.0) {
+        // insertion sort of slots 0..3 where a new minimum lands in slot 1
+        for (int j = 1; j < 4; j++) {
+            double a = rv[j];
+            int i = j - 1;
+            bool stopped = false;
+            for (; i >= 0; i--) {
+                if (rv[i] <= a) { stopped = true; break; }
+                rv[i + 1] = rv[i];
+            }
+            if (!stopped) i = 0;
+            rv[i + 1] = a;
+        }
+        if (rv[4] <= 0.0) return false;
+        L = rv[4];
+        if (!(rv[3] <= 0.0)) {
+            L = rv[3];
+            if (!(rv[2] <= 0.0)) {
+                L = rv[2];
+                if (!(rv[1] <= 0.0)) L = rv[1];
+            }
+        }
+    } else if (im1 == 0.0 && im2 != 0.0) {
+        L = (rv[1] <= rv[2]) ? rv[1] : rv[2];
+    } else if (im1 != 0.0 && im2 == 0.0) {
+        L = (rv[3] <= rv[4]) ? rv[3] : rv[4];
+    }
+    *path = L;
+    if (L == 0.0) return false;
+    double x = Xo + L * Ep, y = Yo + L * Et, z = Zo + L * Rh;
+    double rxy = ::sqrt(x * x + y * y);
+    X[0] = x; X[1] = y; X[2] = z;
+    DF[0] = 2.0 * x * (Ra - rxy) / rxy;
+    DF[1] = 2.0 * y * (Ra - rxy) / rxy;
+    DF[2] = -(2.0 * (z - Rs));
+    return true;
+}
+
+// --------------------------------------------- Newton-Raphson surfaces ----
+// Zernike monomials z = sum B_ij x^j y^(i-j) and slopes (coretrace/vshot.cpp:59-139).
+template <typename R>
+STD void zernike(const DevElem &e, const double *aux, R x, R y, R *z, R *dzdx, R *dzdy, bool slopes) {
+    const int n = e.fit_order;
+    const int w = e.aux_n;  // row stride (order + 1)
+    const double *Bm = aux + e.aux;
+    R xs = x, ys = y;
+    R zz = 0, dx = 0, dy = 0;
+    R px[16], py[16];
+    px[0] = 1; py[0] = 1;
+    for (int i = 1; i <= n; i++) { px[i] = px[i - 1] * x; py[i] = py[i - 1] * y; }
+    for (int i = 0; i <= n; i++)
+        for (int j = 0; j <= i; j++) zz = zz + (R)Bm[i * w + j] * px[j] * py[i - j];
+    *z = zz;
+    if (!slopes) return;
+    if (xs == R(0)) xs = xs + R(1e-10);
+    if (ys == R(0)) ys = ys + R(1e-10);
+    px[0] = 1; py[0] = 1;
+    for (int i = 1; i <= n; i++) { px[i] = px[i - 1] * xs; py[i] = py[i - 1] * ys; }
+    for (int i = 1; i <= n; i++)
+        for (int j = 0; j <= i; j++) {
+            R b = (R)Bm[i * w + j];
+            if (j > 0) dx = dx + R(j) * b * px[j - 1] * py[i - j];
+            if (i - j > 0) dy = dy + R(i - j) * b * px[j] * py[i - j - 1];
+        }
+    *dzdx = dx;
+    *dzdy = dy;
+}
+
+// Numerical-Recipes splint (coretrace/mathproc.cpp:244-276)
+template <typename R>
+STD bool splint(const double *xa, const double *ya, const double *y2a, int n, R x, R *y, R *dydx) {
+    int klo = 0, khi = n - 1;
+    while (khi - klo > 1) {
+        int k = (khi + klo) / 2;
+        if ((R)xa[k] > x) khi = k; else klo = k;
+    }
+    R xk = (R)xa[khi], xl = (R)xa[klo];
+    R h = xk - xl;
+    if (h == R(0)) return false;
+    R a = (xk - x) / h, b = (x - xl) / h;
+    R yl = (R)ya[klo], yk = (R)ya[khi], y2l = (R)y2a[klo], y2k = (R)y2a[khi];
+    *y = a * yl + b * yk + ((a * a * a - a) * y2l + (b * b * b - b) * y2k) * (h * h) / R(6);
+    *dydx = (yk - yl) / (xk - xl) - (R(3) * a * a - R(1)) * (xk - xl) * y2l / R(6) +
+            (R(3) * b * b - R(1)) * (xk - xl) * y2k / R(6);
+    return true;
+}
+
+// Polynomial surface of revolution (coretrace/mathproc.cpp:218-241)
+template <typename R>
+STD void polyrev(const DevElem &e, const double *aux, R x, R y, R *z, R *dzdx, R *dzdy) {
+    const double *c = aux + e.aux;
+    R r = M<R>::sqrt_(x * x + y * y);
+    R zz = 0, dr = 0, rp = 1;
+    for (int i = 0; i <= e.fit_order; i++) {
+        if (i >= 1) dr = dr + R(i) * (R)c[i] * rp;
+        zz = zz + (R)c[i] * rp;
+        rp = rp * r;
+    }
+    *z = zz;
+    *dzdx = dr * (x / r);
+    *dzdy = dr * (y / r);
+}
+
+// F and grad F for Newton (coretrace/surface.cpp:57-413).  X[2] is replaced
+// by the surface z for the tabulated surfaces, as in the reference.
+template <typename R>
+STD int surface_eval(const DevElem &e, const double *aux, R X[3], R *F, R DF[3]) {
+    R x = X[0], y = X[1], z = X[2];
+    switch (e.surf) {
+    case SURF_FLAT:
+        *F = z;
+        DF[0] = 0; DF[1] = 0; DF[2] = 1;
+        return 0;
+    case SURF_GENERAL: {
+        R rho2 = (e.stype == 1) ? x * x + y * y : x * x;
+        if (e.cone_tan != 0.0) {
+            R sr = M<R>::sqrt_(rho2);
+            R ct = (R)e.cone_tan;
+            *F = z - sr / ct;
+            DF[0] = -x / (sr * ct);
+            DF[1] = -y / (sr * ct);
+            DF[2] = 1;
+            return 0;
+        }
+        R cx = (R)e.vcx, cy = (R)e.vcy, k = (R)e.kappa;
+        R arg = R(1) - k * (cx * cx * x * x + cy * cy * y * y);
+        int err = (arg < R(0)) ? 1 : 0;
+        R term = M<R>::sqrt_(arg);
+        *F = z - (cx * x * x + cy * y * y) / (R(1) + term);
+        DF[0] = -x * (cx / term);
+        DF[1] = -y * (cy / term);
+        DF[2] = 1;
+        return err;
+    }
+    case SURF_ZERNIKE: {
+        R zz, dx, dy;
+        zernike<R>(e, aux, x, y, &zz, &dx, &dy, true);
+        X[2] = zz;
+        *F = z - zz;
+        DF[0] = -dx; DF[1] = -dy; DF[2] = 1;
+        return 0;
+    }
+    case SURF_POLY: {
+        R yv = (e.ap == AP_LINE) ? R(0) : y;
+        R zz, dx, dy;
+        polyrev<R>(e, aux, x, yv, &zz, &dx, &dy);
+        X[2] = zz;
+        *F = z - zz;
+        DF[0] = -dx; DF[1] = -dy; DF[2] = 1;
+        return 0;
+    }
+    default: {  // SURF_SPLINE
+        R rho = M<R>::sqrt_(x * x + y * y);
+        R drdx = x / rho, drdy = y / rho;
+        if (e.ap == AP_LINE) { rho = x; drdx = 1; drdy = 0; }
+        const int n = e.aux_n;
+        const double *sx = aux + e.aux;
+        R zz, dz;
+        if (!splint<R>(sx, sx + n, sx + 2 * n, n, rho, &zz, &dz)) return 3;
+        X[2] = zz;
+        *F = z - zz;
+        DF[0] = -(dz * drdx); DF[1] = -(dz * drdy); DF[2] = 1;
+        return 0;
+    }
+    }
+}
+
+// z of the surface at (x, y) for the start-plane choice
+// (coretrace/surfacezatxypair.cpp:56-280), double.
+STD bool surface_z_at_xy(const DevElem &e, const double *aux, double x, double y, double *F) {
+    switch (e.surf) {
+    case SURF_GENERAL: {
+        double rho2 = (e.stype == 1) ? x * x + y * y : x * x;
+        if (e.cone_tan != 0.0) { *F = ::sqrt(rho2) / e.cone_tan; return true; }
+        double q = e.kappa * (e.vcx * e.vcx * x * x + e.vcy * e.vcy * y * y);
+        if (q > 1.0) { *F = 0.0; return true; }
+        *F = (e.vcx * x * x + e.vcy * y * y) / (1.0 + ::sqrt(1.0 - q));
+        return true;
+    }
+    case SURF_ZERNIKE: {
+        double z, dx, dy;
+        zernike<double>(e, aux, x, y, &z, &dx, &dy, false);
+        *F = z;
+        return true;
+    }
+    case SURF_POLY: {
+        double yv = (e.ap == AP_LINE) ? 0.0 : y;
+        double z, dx, dy;
+        polyrev<double>(e, aux, x, yv, &z, &dx, &dy);
+        *F = z;
+        return true;
+    }
+    default:
+        return false;
+    }
+}
+
+// coretrace/newzstartforcubicsplinesurf.cpp:57-142
+STD void spline_cyl(double CR, const double P[3], const double D[3], bool isA, double *zs, int *ef) {
+    *ef = 0;
+    if (isA) {
+        double A = D[0] * D[0] + D[1] * D[1];
+        double B = 2.0 * (P[0] * D[0] + P[1] * D[1]);
+        double C = P[0] * P[0] + P[1] * P[1] - CR * CR;
+        if (B * B > 4.0 * A * C) {
+            double t1 = (-B + ::sqrt(B * B - 4.0 * A * C)) / (2.0 * A);
+            double t2 = (-B - ::sqrt(B * B - 4.0 * A * C)) / (2.0 * A);
+            if (t2 > 0) { *zs = P[2] + t2 * D[2]; return; }
+            if (t2 == 0) { *zs = P[2] + t1 * D[2]; return; }
+            if (t2 < 0 && t1 > 0) { *zs = P[2] + t1 * D[2]; return; }
+            *ef = 1;
+        } else {
+            *ef = 2;
+        }
+    } else {
+        double t1 = (CR - P[0]) / D[0];
+        if (t1 >= 0.0) *zs = P[2] + t1 * D[2]; else *ef = 1;
+    }
+}
+
+// Newton start plane for cubic-spline surfaces (coretrace/intersect.cpp:637-946).
+STD bool spline_start(const DevElem &e, const double *aux, const double X[3], const double D[3], double *ZStart) {
+    const int n = e.aux_n;
+    const double *sx = aux + e.aux, *sy = sx + n, *sy2 = sx + 2 * n;
+    const double OuterR = sx[n - 1], InnerR = sx[0];
+    const bool isA = e.ap == AP_ANNULUS, isL = e.ap == AP_LINE;
+    const double ZA = sy[n - 1];
+    double S00 = -X[2] / (D[2] + 0.00000000001);
+    double X10 = X[0] + D[0] * S00, Y10 = X[1] + D[1] * S00;
+    double R10 = ::sqrt(X10 * X10 + Y10 * Y10);
+    double S0A = (ZA - X[2]) / (D[2] + 0.00000000001);
+    double X1A = X[0] + D[0] * S0A, Y1A = X[1] + D[1] * S0A;
+    double R1A = ::sqrt(X1A * X1A + Y1A * Y1A);
+    double zcs = 0;
+    int ef = 0;
+#define ST_CYL(CR, FROM)                                   \
+    do {                                                   \
+        spline_cyl((CR), (FROM), D, isA, &zcs, &ef);       \
+        if (ef == 0) *ZStart = zcs;                        \
+        return true;                                       \
+    } while (0)
+    const double PA[3] = {X1A, Y1A, ZA}, PZ[3] = {X10, Y10, 0.0};
+    if (X[2] >= ZA && D[2] < 0.0) {
+        if ((isA && R1A > OuterR) || (isL && X1A > OuterR)) ST_CYL(OuterR * 0.999999, PA);
+        if ((isA && R1A <= OuterR) || (isL && X1A <= OuterR)) {
+            if ((isA && R10 >= OuterR) || (isL && X10 >= OuterR)) ST_CYL(OuterR * 0.999999, PA);
+            *ZStart = 0.0;
+            return true;
+        }
+    }
+    if (X[2] <= 0.0 && D[2] > 0.0) {
+        if ((isA && R10 >= OuterR) || (isL && X10 >= OuterR)) ST_CYL(OuterR * 0.999999, PZ);
+        if ((isA && R10 < OuterR && R10 > InnerR) || (isL && X10 < OuterR && X10 > InnerR)) {
+            *ZStart = 0.0;
+            return true;
+        }
+        if ((isA && R10 <= InnerR) || (isL && X10 <= InnerR)) ST_CYL(InnerR / 0.999999, PZ);
+    }
+    if (X[2] < ZA && X[2] > 0.0 && D[2] < 0.0) {
+        double R1 = ::sqrt(X[0] * X[0] + X[1] * X[1]);
+        if ((isA && R1 >= OuterR) || (isL && X[0] >= OuterR)) ST_CYL(OuterR * 0.999999, X);
+        if ((isA && R1 < OuterR && R1 > InnerR) || (isL && X[0] < OuterR && X[0] > InnerR)) {
+            double Z1, dz;
+            if (!splint<double>(sx, sy, sy2, n, isA ? R1 : X[0], &Z1, &dz)) return false;
+            if (Z1 < X[2]) {
+                if ((isA && R10 >= OuterR) || (isL && X10 >= OuterR)) ST_CYL(OuterR * 0.999999, X);
+                *ZStart = 0.0;
+                return true;
+            }
+            if (Z1 >= X[2]) *ZStart = X[2];
+            return true;
+        }
+        if ((isA && R1 <= InnerR) || (isL && X[0] <= InnerR)) ST_CYL(InnerR / 0.999999, X);
+    }
+    if (X[2] < ZA && X[2] > 0.0 && D[2] > 0.0) {
+        double R1 = ::sqrt(X[0] * X[0] + X[1] * X[1]);
+        if ((isA && R1 >= OuterR) || (isL && X[0] >= OuterR)) ST_CYL(OuterR * 0.999999, X);
+        if ((isA && R1 < OuterR && R1 > InnerR) || (isL && X[0] < OuterR && X[0] > InnerR)) {
+            double Z1, dz;
+            if (!splint<double>(sx, sy, sy2, n, isA ? R1 : X[0], &Z1, &dz)) return false;
+            if (Z1 < X[2]) {
+                if ((isA && R1A >= OuterR) || (isL && X1A >= OuterR)) ST_CYL(OuterR * 0.999999, X);
+                return true;
+            }
+            if (Z1 >= X[2]) {
+                if ((isA && R10 < OuterR) || (isL && X10 < OuterR)) {
+                    *ZStart = 0.0;
+                    return true;
+                }
+                if ((isA && R10 >= OuterR) || (isL && X10 >= OuterR)) ST_CYL(OuterR * 0.999999, PZ);
+            }
+        }
+        if ((isA && R1 <= InnerR) || (isL && X[0] <= InnerR)) ST_CYL(OuterR * 0.999999, X);
+    }
+#undef ST_CYL
+    return true;
+}
+
+// ------------------------------------------------------ element hit -------
+template <typename R>
+struct ElemHit {
+    double path;      // path length from the nudged element-frame origin
+    R X[3];           // hit point, element frame
+    R DF[3];          // surface normal (flipped on back-side hits)
+    int back;
+};
+
+// Ray vs one element: Intersect + DetermineElementIntersectionNew
+// (coretrace/intersect.cpp:140-1011, determineelementintersectionnew.cpp:58-633).
+// P, D: the nudged ray in the element frame (double).  Returns true on an
+// intercept (before the caller's closest-hit / ZAperture rule).
+template <typename R>
+STD bool element_hit(const DevElem &e, const double *aux, const double P[3], const double D[3], ElemHit<R> &h) {
+    const int s = e.surf;
+    R X[3], DF[3];
+    double path;
+    bool ok;
+    if (s == SURF_TORUS) {
+        double Xd[3], DFd[3];
+        ok = torus(e, P, D, Xd, DFd, &path);
+        for (int i = 0; i < 3; i++) { X[i] = (R)Xd[i]; DF[i] = (R)DFd[i]; }
+    } else if (s <= SURF_CYLINDER) {
+        // closed-form quadrics; a float ray starting far away is first moved
+        // along itself to the element's bounding sphere
+        double t0 = 0.0;
+        if (sizeof(R) == 4) {
+            double d[3] = {P[0] - e.bsphere_c[0], P[1] - e.bsphere_c[1], P[2] - e.bsphere_c[2]};
+            double tc = -(d[0] * D[0] + d[1] * D[1] + d[2] * D[2]);
+            double dist2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            double rr = 2.0 * e.bsphere_r + 1.0;
+            if (dist2 > rr * rr && tc > rr) t0 = tc - rr;
+        }
+        R Pr[3] = {(R)(P[0] + t0 * D[0]), (R)(P[1] + t0 * D[1]), (R)(P[2] + t0 * D[2])};
+        R Dr[3] = {(R)D[0], (R)D[1], (R)D[2]};
+        R t;
+        ok = quadric<R>(e, Pr, Dr, X, DF, &t);
+        path = t0 + (double)t;
+        if (ok && !(path >= 0.0)) ok = false;
+    } else {
+        // Newton-Raphson: start plane in double, iterations in R
+        double S0Ap = (e.zap - P[2] == 0.0) ? 0.0 : (e.zap - P[2]) / (D[2] + 0.00000000001);
+        bool za_in = aperture_contains<double>(e, P[0] + D[0] * S0Ap, P[1] + D[1] * S0Ap, true);
+        double ZStart = 0.0;
+        if (s == SURF_SPLINE) {
+            if (!spline_start(e, aux, P, D, &ZStart)) return false;
+        } else if (s != SURF_FLAT) {
+            double FXY = 0.0;
+            surface_z_at_xy(e, aux, P[0], P[1], &FXY);
+            if (P[2] <= 0.0 && D[2] > 0.0) ZStart = 0.0;
+            else if (P[2] <= FXY && D[2] > 0.0) ZStart = 0.0;
+            else if (P[2] <= FXY && D[2] < 0.0 && P[2] > e.zap && za_in) ZStart = 0.0;
+            else if (P[2] <= FXY && D[2] < 0.0) ZStart = e.zap;
+            else if (P[2] > FXY && D[2] < 0.0) ZStart = 0.0;
+            else if (P[2] > FXY && D[2] > 0.0) ZStart = e.zap;
+        }
+        double S0 = (ZStart - P[2] == 0.0) ? 0.0 : (ZStart - P[2]) / (D[2] + 0.00000000001);
+        const R X1 = (R)(P[0] + D[0] * S0), Y1 = (R)(P[1] + D[1] * S0), Z1 = (R)ZStart;
+        const R Dr[3] = {(R)D[0], (R)D[1], (R)D[2]};
+        R lower = R(-1e10), upper = R(1e10);
+        if (s == SURF_SPLINE) {
+            const double *sx = aux + e.aux;
+            R a = ((R)sx[0] - X1) / (Dr[0] + R(0.00000000001));
+            R b = ((R)sx[e.aux_n - 1] - X1) / (Dr[0] + R(0.00000000001));
+            lower = a < b ? a : b;
+            upper = a < b ? b : a;
+        }
+        R SJ1 = 0, SJ = 0;
+        ok = false;
+        const R eps = R(0.000001);
+        for (int it = 0; it < 20; ++it) {
+            SJ = SJ1;
+            X[0] = X1 + Dr[0] * SJ;
+            X[1] = Y1 + Dr[1] * SJ;
+            X[2] = Z1 + Dr[2] * SJ;
+            R F;
+            if (surface_eval<R>(e, aux, X, &F, DF) != 0) break;
+            R dfd = dot3<R>(DF, Dr);
+            if (M<R>::fabs_(F) <= eps * M<R>::fabs_(dfd)) { ok = true; break; }
+            SJ1 = SJ - F / dfd;
+            if (s == SURF_SPLINE) {
+                if ((F < 0 && Dr[2] > 0) || (F > 0 && Dr[2] < 0)) lower = SJ > lower ? SJ : lower;
+                else if ((F < 0 && Dr[2] < 0) || (F > 0 && Dr[2] > 0)) upper = SJ < upper ? SJ : upper;
+                if (SJ1 < lower || SJ1 > upper) SJ1 = R(0.5) * (lower + upper);
+                if (upper <= lower) break;
+            }
+        }
+        path = S0 + (double)SJ;
+        if (ok && path < 0.0) ok = false;
+    }
+    if (!ok) return false;
+    if (!aperture_contains<R>(e, X[0], X[1], false)) return false;
+    R Dr[3] = {(R)D[0], (R)D[1], (R)D[2]};
+    h.back = (dot3<R>(Dr, DF) < R(0)) ? 0 : 1;
+    if (h.back) { DF[0] = -DF[0]; DF[1] = -DF[1]; DF[2] = -DF[2]; }
+    for (int i = 0; i < 3; i++) { h.X[i] = X[i]; h.DF[i] = DF[i]; }
+    h.path = path;
+    return true;
+}
+
+// ------------------------------------------------------------ optics ------
+// Rotation taking the local z axis onto direction c (coretrace/errors.cpp:98-121).
+template <typename R>
+STD void ray_frame(const R c[3], R RT[9]) {
+    R e0, e1;
+    if (c[2] == R(0)) {
+        if (c[0] == R(0)) { e0 = 0; e1 = (R)(M_PI / 2.0); }
+        else { e0 = (R)(M_PI / 2.0); e1 = M<R>::atan2_(c[1], M<R>::sqrt_(c[0] * c[0] + c[2] * c[2])); }
+    } else {
+        e0 = M<R>::atan2_(c[0], c[2]);
+        e1 = M<R>::atan2_(c[1], M<R>::sqrt_(c[0] * c[0] + c[2] * c[2]));
+    }
+    R ca = M<R>::cos_(e0), cb = M<R>::cos_(e1), sa = M<R>::sin_(e0), sb = M<R>::sin_(e1);
+    // gamma = 0: RRefToLoc rows; store its transpose (local -> reference)
+    RT[0] = ca;       RT[1] = -sa * sb; RT[2] = sa * cb;
+    RT[3] = 0;        RT[4] = cb;       RT[5] = sb;
+    RT[6] = -sa;      RT[7] = -ca * sb; RT[8] = ca * cb;
+}
+
+template <typename R>
+STD void cone_dir(R theta2, R phi, const R RT[9], R out[3]) {
+    R th = M<R>::sqrt_(theta2);
+    R st = M<R>::sin_(th);
+    R l0 = st * M<R>::cos_(phi), l1 = st * M<R>::sin_(phi), l2 = M<R>::cos_(th);
+    out[0] = RT[0] * l0 + RT[1] * l1 + RT[2] * l2;
+    out[1] = RT[3] * l0 + RT[4] * l1 + RT[5] * l2;
+    out[2] = RT[6] * l0 + RT[7] * l1 + RT[8] * l2;
+}
+
+// First index i with table[i] >= theta, capped at n-1 (the linear walk of
+// coretrace/errors.cpp:167-169 on an ascending table).
+template <typename R>
+STD int table_index(const double *ang, int n, R theta, bool ascending) {
+    if (ascending) {
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if ((R)ang[mid] < theta) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    }
+    int i = 0;
+    while (i < n - 1 && (R)ang[i] < theta) i++;
+    return i;
+}
+
+// Sunshape (source 1) or specularity (source 2) error (coretrace/errors.cpp:60-226).
+template <typename R>
+STD void errors(Rng<R> &rng, const R cin[3], int source, const DevSun &sun, const double *aux, int interaction,
+                const DevFace &f, R cout[3], const R DF[3]) {
+    R RT[9];
+    ray_frame<R>(cin, RT);
+    int dist = (source == 1) ? sun.shape : f.dist;
+    R delop = (source == 1) ? (R)(sun.sigma / 1000.0) : (R)(f.spec / 1000.0);
+    unsigned maxcall = 0;
+    R theta2 = 0;
+    for (;;) {
+        if (dist == 'g' || dist == 'G') {
+            R tx = rng.normal(R(0), delop);
+            R ty = rng.normal(R(0), delop);
+            theta2 = tx * tx + ty * ty;
+        } else if (dist == 'p' || dist == 'P') {
+            R tx, ty;
+            do {
+                tx = R(2) * delop * rng.uniform() - delop;
+                ty = R(2) * delop * rng.uniform() - delop;
+                theta2 = tx * tx + ty * ty;
+            } while (theta2 > delop * delop);
+        } else if (dist == 'd' || dist == 'D') {
+            const int n = sun.tab_n;
+            const double *ang = aux + sun.tab, *inten = ang + n;
+            const bool asc = sun.tab_ascending != 0;
+            const R ma = (R)sun.max_angle, mi = (R)sun.max_intensity;
+            for (;;) {
+                R tx = R(2) * ma * rng.uniform() - ma;
+                R ty = R(2) * ma * rng.uniform() - ma;
+                theta2 = tx * tx + ty * ty;
+                R th = M<R>::sqrt_(theta2);
+                int i = table_index<R>(ang, n, th, asc);
+                R stest;
+                if (i == 0) stest = (R)inten[0];
+                else stest = (R)inten[i - 1] + ((R)inten[i] - (R)inten[i - 1]) * (th - (R)ang[i - 1]) / ((R)ang[i] - (R)ang[i - 1]);
+                if (rng.uniform() > stest / mi) continue;
+                if (theta2 > ma * ma) continue;
+                break;
+            }
+            theta2 = theta2 / R(1000000.0);
+        } else if (dist == 'f' || dist == 'F') {
+            R a = M<R>::asin_(M<R>::sqrt_(rng.uniform()));
+            theta2 = a * a;
+        }
+        R phi = rng.uniform() * R(2) * (R)3.1415926535897932385;
+        cone_dir<R>(theta2, phi, RT, cout);
+        if (source == 2 && interaction == 2 && dot3<R>(cout, DF) < R(0) && maxcall++ < 50000u) continue;
+        break;
+    }
+}
+
+// Slope error of the surface normal (coretrace/surfacenormalerrors.cpp:58-167).
+template <typename R>
+STD void normal_errors(Rng<R> &rng, const R cin[3], const DevFace &f, R cout[3]) {
+    R RT[9];
+    ray_frame<R>(cin, RT);
+    R delop = (R)(f.slope / 1000.0);
+    R theta2 = 0;
+    if (f.dist == 'g' || f.dist == 'G') {
+        R tx = rng.normal(R(0), delop);
+        R ty = rng.normal(R(0), delop);
+        theta2 = tx * tx + ty * ty;
+    } else if (f.dist == 'p' || f.dist == 'P') {
+        R tx, ty;
+        do {
+            tx = R(2) * delop * rng.uniform() - delop;
+            ty = R(2) * delop * rng.uniform() - delop;
+            theta2 = tx * tx + ty * ty;
+        } while (theta2 > delop * delop);
+    }
+    R phi = rng.uniform() * R(2) * (R)3.1415926535897932385;
+    cone_dir<R>(theta2, phi, RT, cout);
+}
+
+// Reflection / refraction (coretrace/interaction.cpp:58-201); cout is left
+// untouched when the refraction Newton loop does not converge.
+template <typename R>
+STD void interact(Rng<R> &rng, const R c[3], const R DF[3], int type, const DevFace &f, R cout[3]) {
+    if (type == 1) {
+        R n1 = (R)f.n_in, n2 = (R)f.n_out;
+        R mu = n1 / n2;
+        R D2 = dot3<R>(DF, DF);
+        R B = (mu * mu - R(1)) / D2;
+        R A = mu * dot3<R>(c, DF) / D2;
+        if (B > A * A) {
+            R a = dot3<R>(c, DF) / dot3<R>(DF, DF);
+            for (int i = 0; i < 3; i++) cout[i] = c[i] - R(2) * a * DF[i];
+            return;
+        }
+        R nn = M<R>::sqrt_(dot3<R>(DF, DF));
+        R u[3] = {-DF[0] / nn, -DF[1] / nn, -DF[2] / nn};
+        R ia = M<R>::acos_(dot3<R>(c, u));
+        R si = M<R>::sin_(ia), ci = M<R>::cos_(ia);
+        R st = n1 * si / n2;
+        R ct = M<R>::sqrt_(R(1) - st * st);
+        R rs = (n1 * ci - n2 * ct) / (n1 * ci + n2 * ct);
+        R rp = (n1 * ct - n2 * ci) / (n1 * ct + n2 * ci);
+        R rave = (rp * rp + rs * rs) / R(2);
+        if (rave < rng.uniform()) {
+            R g = -B / (R(2) * A), g1 = 0;
+            for (int i = 1; i < 10; i++) {
+                g1 = (g * g - B) / (R(2) * (g + A));
+                if (M<R>::fabs_(g - g1) < R(0.000005)) {
+                    for (int j = 0; j < 3; j++) cout[j] = mu * c[j] + g1 * DF[j];
+                    return;
+                }
+                g = g1;
+            }
+            return;
+        }
+        R a = dot3<R>(c, DF) / dot3<R>(DF, DF);
+        for (int i = 0; i < 3; i++) cout[i] = c[i] - R(2) * a * DF[i];
+        return;
+    }
+    R a = dot3<R>(c, DF) / dot3<R>(DF, DF);
+    for (int i = 0; i < 3; i++) cout[i] = c[i] - R(2) * a * DF[i];
+}
+
+// Optical efficiency (coretrace/raytrace.cpp:885-946).
+template <typename R>
+STD R test_value(int interaction, const DevFace &f, const double *aux, const R c[3], const R DF[3]) {
+    int tab, n;
+    if (interaction == 1) {
+        if (f.trans_n <= 0) return (R)f.transmissivity;
+        tab = f.trans_tab; n = f.trans_n;
+    } else {
+        if (f.refl_n <= 0) return (R)f.reflectivity;
+        tab = f.refl_tab; n = f.refl_n;
+    }
+    const double *ang = aux + tab, *val = ang + n;
+    R nn = M<R>::sqrt_(dot3<R>(DF, DF));
+    R u[3] = {-DF[0] / nn, -DF[1] / nn, -DF[2] / nn};
+    R ia = M<R>::acos_(dot3<R>(c, u)) * R(1000);
+    if (ia >= (R)ang[n - 1]) return (R)val[n - 1];
+    int m = 0;
+    while ((R)ang[m] < ia) m++;
+    if (m == 0) return (R)val[0];
+    return ((R)val[m] + (R)val[m - 1]) / R(2);
+}
+
+}  // namespace st

@@ -1,0 +1,88 @@
+// Host-visible launch interface of the trace kernels (st_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "st_types.h"
+
+namespace st {
+
+// Device pointers of one compiled scene.
+struct SceneView {
+    const DevElem *elems;
+    const DevFace *faces;
+    const double *aux;
+    const DevStage *stages;
+    const DevBvhNode *bvh;
+    const int32_t *bvh_items;
+    const int32_t *grid_cells;   // ncell + 1 starts
+    const int32_t *grid_items;
+    DevSunGrid grid;
+    int32_t use_grid;
+    int32_t nstages;
+    DevSun sun;
+};
+
+// Arguments of one stage-trace launch.
+struct TraceArgs {
+    SceneView sv;
+    int32_t si;                  // 0-based stage
+    uint32_t seed;
+    int32_t inc_sun, inc_err;
+    int32_t n_rays;              // rays in this launch
+    int32_t slots;               // record slots per ray
+    // stage 0 input
+    const uint64_t *sun_index;   // per ray: 0-based sun-ray index
+    uint32_t raynum_base;        // ray t has ray number raynum_base + t + 1
+    // later-stage input
+    const DevRay *in_rays;
+    uint32_t last_raynum;        // LastRayNumberInPreviousStage
+    // outputs
+    DevRecord *rec_slots;        // n_rays * slots
+    uint32_t *rec_count;         // n_rays
+    DevRecord *ovf_rec;          // overflow records (seq >= slots)
+    uint32_t *ovf_meta;          // (ray, seq) pairs
+    uint32_t *ovf_count;
+    uint32_t ovf_cap;
+    DevRay *out_slots;           // n_rays
+    uint32_t *out_flag;          // n_rays
+    uint32_t *err_flags;         // bit 0: bad interaction, 1: phase mismatch, 2: overflow full
+    FluxSpec flux;
+    uint32_t *flux_bins;
+};
+
+// Arguments of the sun-ray hit-mask launch.
+struct MaskArgs {
+    SceneView sv;
+    uint32_t seed;
+    uint64_t s0;                 // first sun-ray index
+    uint32_t n;                  // sun rays in this launch
+    uint64_t *mask;              // one word per 64 sun rays
+};
+
+template <typename R> void launch_sun_mask(const MaskArgs &a, hipStream_t s);
+template <typename R> void launch_trace_stage(const TraceArgs &a, hipStream_t s);
+
+// Exclusive scan of n uint32 values into out (out may alias in); total via
+// *total (device).  tmp must hold scan_tmp_words(n) words.
+size_t scan_tmp_words(size_t n);
+void launch_exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, uint32_t *total,
+                           hipStream_t s);
+
+// Fill n words with v (a kernel, so it may sit inside a captured graph).
+void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
+// Mask words -> per-word popcounts.
+void launch_mask_popc(const uint64_t *mask, uint32_t *counts, size_t nwords, hipStream_t s);
+// Hit list: the first `limit` set bits of the mask, as sun-ray indices.
+void launch_mask_compact(const uint64_t *mask, const uint32_t *offsets, size_t nwords, uint64_t s0, uint32_t limit,
+                         uint64_t *hits, hipStream_t s);
+// Record and ray compaction after a stage launch.
+void launch_compact_records(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets, int n_rays,
+                            int nslots, DevRecord *out, hipStream_t s);
+void launch_compact_overflow(const DevRecord *ovf, const uint32_t *meta, const uint32_t *n_ovf_dev, uint32_t cap,
+                             const uint32_t *offsets, DevRecord *out, hipStream_t s);
+void launch_compact_rays(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, int n_rays, DevRay *out,
+                         hipStream_t s);
+
+}  // namespace st

@@ -1,0 +1,892 @@
+// Scene compiler of the HIP backend (see st_scene.h).
+#include "st_scene.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <strings.h>
+
+namespace st {
+
+static const double kDegRad = 0.017453292519943295;  // coretrace/types.h:64
+
+// coretrace/mathproc.cpp:175-217
+void euler_matrix(const double e[3], double R[9]) {
+    double ca = cos(e[0]), cb = cos(e[1]), cg = cos(e[2]);
+    double sa = sin(e[0]), sb = sin(e[1]), sg = sin(e[2]);
+    R[0] = ca * cg + sa * sb * sg;
+    R[1] = -cb * sg;
+    R[2] = -sa * cg + ca * sb * sg;
+    R[3] = ca * sg - sa * sb * cg;
+    R[4] = cb * cg;
+    R[5] = -sa * sg - ca * sb * cg;
+    R[6] = sa * cb;
+    R[7] = sb;
+    R[8] = ca * cb;
+}
+
+// coretrace/input.cpp:158-176
+bool euler_from_aim(const double org[3], const double aim[3], double zrot, double e[3]) {
+    double dx = aim[0] - org[0], dy = aim[1] - org[1], dz = aim[2] - org[2];
+    double d = sqrt(dx * dx + dy * dy + dz * dz);
+    if (d == 0.0) return false;
+    dx = dx / d;
+    dy = dy / d;
+    dz = dz / d;
+    e[0] = atan2(dx, dz);
+    e[1] = asin(dy);
+    e[2] = zrot * kDegRad;
+    return true;
+}
+
+static void mv(const double *M, const double v[3], double o[3]) {
+    o[0] = M[0] * v[0] + M[1] * v[1] + M[2] * v[2];
+    o[1] = M[3] * v[0] + M[4] * v[1] + M[5] * v[2];
+    o[2] = M[6] * v[0] + M[7] * v[1] + M[8] * v[2];
+}
+static void mtv(const double *M, const double v[3], double o[3]) {
+    o[0] = M[0] * v[0] + M[3] * v[1] + M[6] * v[2];
+    o[1] = M[1] * v[0] + M[4] * v[1] + M[7] * v[2];
+    o[2] = M[2] * v[0] + M[5] * v[1] + M[8] * v[2];
+}
+
+// ------------------------------------------------- parameter translation --
+bool translate_surface_params(HElement &e, const double p[8], std::string &msg) {
+    const bool line = (e.shape == 'l' || e.shape == 'L');
+    switch (e.surf) {
+    case 's': case 'S':
+        e.vcx = p[0];
+        e.vcy = line ? 0.0 : e.vcx;
+        e.kappa = 1.0;
+        e.stype = (e.vcy == 0.0) ? 7 : 1;
+        return true;
+    case 'p': case 'P':
+        e.vcx = p[0]; e.vcy = p[1]; e.kappa = 0.0;
+        e.stype = (e.vcy == 0.0) ? 7 : 1;
+        return true;
+    case 'o': case 'O':
+        e.vcx = p[0]; e.vcy = p[0]; e.kappa = p[1];
+        e.stype = (e.vcy == 0.0) ? 7 : 1;
+        return true;
+    case 'g': case 'G':
+        e.vcx = p[0]; e.vcy = p[1]; e.kappa = p[2];
+        for (int i = 0; i < 5; i++) e.alpha[i] = p[3 + i];
+        e.stype = (e.vcy == 0.0) ? 7 : 1;
+        return true;
+    case 'f': case 'F':
+        e.alpha[0] = 0; e.alpha[1] = 0; e.alpha[2] = 1; e.alpha[3] = 0; e.alpha[4] = 0;
+        e.stype = 3;
+        return true;
+    case 'c': case 'C':
+        e.cone_half = p[0];
+        e.stype = 1;
+        return true;
+    case 't': case 'T':
+        if (!line || e.p[0] != 0.0 || e.p[1] != 0.0) {
+            msg = "Bad aperture or parameters of element not consistent with cylindrical surface\n";
+            return false;
+        }
+        e.curv_rev = p[0];
+        e.vcx = e.curv_rev;
+        e.stype = 2;
+        return true;
+    case 'd': case 'D':
+        if ((e.shape != 'a' && e.shape != 'A') || e.p[0] != 0.0 || e.p[1] != 0.0) {
+            msg = "Bad aperture or parameters of element not consistent with toroidal surface\n";
+            return false;
+        }
+        e.ann_r = p[0];
+        e.cross_r = p[1];
+        e.stype = 10;
+        return true;
+    default: {
+        char b[64];
+        snprintf(b, sizeof b, "invalid surface index: %c\n", e.surf);
+        msg = b;
+        return false;
+    }
+    }
+}
+
+// Numerical-Recipes natural/clamped cubic spline second derivatives
+// (coretrace/mathproc.cpp:278-330).
+static void spline_y2(const std::vector<double> &x, const std::vector<double> &y, double yp1, double ypn,
+                      std::vector<double> &y2) {
+    const int n = (int)x.size();
+    std::vector<double> u(n);
+    y2.assign(n, 0.0);
+    if (yp1 > 0.99e30) {
+        y2[0] = 0.0;
+        u[0] = 0.0;
+    } else {
+        y2[0] = -0.5;
+        u[0] = (3.0 / (x[1] - x[0])) * ((y[1] - y[0]) / (x[1] - x[0]) - yp1);
+    }
+    for (int i = 1; i < n - 1; i++) {
+        double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+        double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        u[i] = (y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1]);
+        u[i] = (6.0 * u[i] / (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / p;
+    }
+    double qn = 0.0, un = 0.0;
+    if (!(ypn > 0.99e30)) {
+        qn = 0.5;
+        un = (3.0 / (x[n - 1] - x[n - 2])) * (ypn - (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]));
+    }
+    y2[n - 1] = (un - qn * u[n - 2]) / (qn * y2[n - 2] + 1.0);
+    for (int k = n - 2; k >= 0; k--) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
+bool read_surface_file(HElement &e, const char *file, std::string &msg) {
+    const char *dot = strrchr(file, '.');
+    char b[600];
+    if (!dot) {
+        snprintf(b, sizeof b, "Could not determine surface file type: '%s'\n", file);
+        msg = b;
+        return false;
+    }
+    std::string ext(dot + 1);
+    for (auto &c : ext) c = (char)tolower(c);
+    FILE *fp = fopen(file, "r");
+    if (!fp) {
+        snprintf(b, sizeof b, "Could not open surface file for read: %s\n", file);
+        msg = b;
+        return false;
+    }
+    char line[1024];
+    auto next = [&]() {
+        line[0] = 0;
+        if (!fgets(line, sizeof line, fp)) line[0] = 0;
+    };
+    bool ok = true;
+    if (ext == "mon") {
+        next();
+        int order = atoi(line);
+        e.fit_order = order;
+        e.zern.assign((size_t)(order + 1) * (order + 1), 0.0);
+        for (int k = 0; k <= order; k++)
+            for (int m = 0; m <= k; m++) {
+                next();
+                e.zern[(size_t)k * (order + 1) + m] = atof(line);
+            }
+        e.surf = 'm';
+        e.stype = 6;
+    } else if (ext == "ply") {
+        next();
+        int order = atoi(line);
+        e.fit_order = order;
+        e.poly.assign(order + 1, 0.0);
+        for (int k = 0; k <= order; k++) {
+            next();
+            e.poly[k] = atof(line);
+        }
+        e.surf = 'r';
+        e.stype = 8;
+    } else if (ext == "csi") {
+        next();
+        int n = atoi(line);
+        e.sx.assign(n, 0.0);
+        e.sy.assign(n, 0.0);
+        double x = 0, y = 0;
+        for (int k = 0; k < n; k++) {
+            next();
+            sscanf(line, "%lg %lg", &x, &y);
+            e.sx[k] = x;
+            e.sy[k] = y;
+        }
+        next();
+        sscanf(line, "%lg %lg", &x, &y);
+        spline_y2(e.sx, e.sy, x, y, e.sy2);
+        const bool al = (e.shape == 'a' || e.shape == 'A' || e.shape == 'l' || e.shape == 'L');
+        if (!al || e.p[0] < e.sx[0] || e.p[1] > e.sx[n - 1]) {
+            msg = "Error: Element uses cubic spline interpolation: make sure aperture is type 'a' or 'l'.\n";
+            ok = false;
+        } else {
+            e.surf = 'i';
+            e.stype = 9;
+        }
+    } else {
+        snprintf(b, sizeof b, "Surface file type extension unknown: '%s'\n", ext.c_str());
+        msg = b;
+        ok = false;
+    }
+    fclose(fp);
+    return ok;
+}
+
+// -------------------------------------------------------- aperture plane --
+// coretrace/apertureplane.cpp:57-491
+static void aperture_plane(HElement &e) {
+    auto sq = [](double v) { return v * v; };
+    const double A = e.p[0], B = e.p[1];
+    const char sh = (char)tolower(e.shape);
+    double maxr = 0.0;
+    if (sh == 'i' || sh == 'q') {
+        maxr = sqrt(sq(e.p[0]) + sq(e.p[1]));
+        maxr = std::max(maxr, sqrt(sq(e.p[2]) + sq(e.p[3])));
+        maxr = std::max(maxr, sqrt(sq(e.p[4]) + sq(e.p[5])));
+        if (sh == 'q') maxr = std::max(maxr, sqrt(sq(e.p[6]) + sq(e.p[7])));
+    }
+    const bool cht = (sh == 'c' || sh == 'h' || sh == 't');
+    const bool al = (sh == 'a' || sh == 'l');
+    const bool iq = (sh == 'i' || sh == 'q');
+    const double r2al = (fabs(B) >= fabs(A)) ? sq(B) : sq(A);
+    switch (tolower(e.surf)) {
+    case 'g': {
+        double r2 = 0, sum = 0;
+        if (cht) {
+            r2 = sq(A / 2.0);
+            double rho = 1.0;
+            for (int i = 0; i < 5; i++) { rho = rho * r2; sum += e.alpha[i] * rho; }
+            e.zap = e.vcx * r2 / (1 + sqrt(1 - e.kappa * e.vcx * e.vcx * r2)) + sum;
+        } else if (sh == 'r') {
+            r2 = sq(A / 2.0) + sq(B / 2.0);
+            for (int i = 0; i < 5; i++) sum += e.alpha[i] * pow(r2, i + 1);
+            e.zap = e.vcx * r2 / (1 + sqrt(1 - e.kappa * sq(e.vcx) * r2)) + sum;
+        } else if (al) {
+            for (int i = 0; i < 5; i++) sum = sum + e.alpha[i] * pow(r2al, i + 1);
+            e.zap = e.vcx * r2al / (1 + sqrt(1 - e.kappa * sq(e.vcx) * r2al)) + sum;
+        } else if (iq) {
+            r2 = sq(maxr);
+            e.zap = e.vcx * r2 / (1 + sqrt(1 - e.kappa * sq(e.vcx) * r2)) + 0.0;
+        }
+        break;
+    }
+    case 's': {
+        double F = 1.0 / (2.0 * e.vcx);
+        if (cht) e.zap = 2.0 * F - sqrt(sq(2.0 * F) - sq(A / 2.0));
+        else if (sh == 'r') e.zap = 2.0 * F - sqrt(sq(2.0 * F) - (sq(A / 2.0) + sq(B / 2.0)));
+        else if (al) e.zap = 2.0 * F - sqrt(sq(2.0 * F) - r2al);
+        else if (iq) e.zap = 2.0 * F - sqrt(sq(2.0 * F) - sq(maxr));
+        break;
+    }
+    case 'p': {
+        double F = 1.0 / (2.0 * e.vcx);
+        if (cht) e.zap = sq(A / 2.0) / (4.0 * F);
+        else if (sh == 'r') e.zap = (sq(A / 2.0) + sq(B / 2.0)) / (4.0 * F);
+        else if (al) e.zap = r2al / (4.0 * F);
+        else if (iq) e.zap = sq(maxr) / (4.0 * F);
+        break;
+    }
+    case 'f':
+        e.zap = 0.0;
+        break;
+    case 'o': {
+        auto z = [&](double r2) { return (1 - sqrt(1 - e.kappa * sq(e.vcx) * r2)) / (e.kappa * e.vcx); };
+        if (cht) e.zap = z(sq(A / 2.0));
+        else if (sh == 'r') e.zap = z(sq(A / 2.0) + sq(B / 2.0));
+        else if (al) e.zap = z(r2al);
+        else if (iq) e.zap = z(sq(maxr));
+        break;
+    }
+    case 'c': {
+        double t = tan(e.cone_half * kDegRad);
+        if (cht) e.zap = A / (2.0 * t);
+        else if (sh == 'r') e.zap = sqrt(sq(A / 2.0) + sq(B / 2.0)) / (2.0 * t);
+        else if (sh == 'a') e.zap = B / t;
+        else if (iq) e.zap = sqrt(sq(maxr)) / (2.0 * t);
+        break;
+    }
+    case 't':
+        e.zap = 2.0 / e.curv_rev;
+        break;
+    case 'd':
+        e.zap = 2.0 * e.cross_r;
+        break;
+    case 'm': {
+        int n = e.fit_order + 1;
+        if (!(n > 2)) return;
+        double F = (1.0 / (4.0 * e.zern[2 * n + 0]) + 1.0 / (4.0 * e.zern[2 * n + 2])) / 2.0;
+        if (cht) e.zap = sq(A / 2.0) / (4.0 * F);
+        else if (sh == 'r') e.zap = (sq(A / 2.0) + sq(B / 2.0)) / (4.0 * F);
+        else if (al) e.zap = r2al / (4.0 * F);
+        else if (iq) e.zap = sq(maxr) / (4.0 * F);
+        break;
+    }
+    case 'r': {
+        auto ev = [&](double r) {
+            double z = 0.0;
+            for (int i = 0; i <= e.fit_order; i++) z = z + e.poly[i] * pow(r, i);
+            return z;
+        };
+        if (cht) e.zap = ev(A / 2.0);
+        else if (sh == 'r') e.zap = ev(sqrt(sq(A / 2.0) + sq(B / 2.0)));
+        else if (al) e.zap = ev(sqrt(r2al));
+        else if (iq) e.zap = ev(maxr);
+        break;
+    }
+    case 'i':
+        e.zap = e.sy.empty() ? 0.0 : e.sy.back();
+        break;
+    default:
+        break;
+    }
+}
+
+static int ap_kind(char c) {
+    switch (tolower(c)) {
+    case 'c': return AP_CIRCLE;
+    case 'h': return AP_HEX;
+    case 't': return AP_TRI;
+    case 'r': return AP_RECT;
+    case 'a': return AP_ANNULUS;
+    case 'l': return AP_LINE;
+    case 'i': return AP_ITRI;
+    case 'q': return AP_QUAD;
+    default: return 0;
+    }
+}
+
+static int surf_kind(const HElement &e) {
+    switch (tolower(e.surf)) {
+    case 's': return SURF_SPHERE;
+    case 'p': return SURF_PARABOLA;
+    case 'o': return e.stype == 1 ? SURF_CONIC : SURF_GENERAL;
+    case 't': return SURF_CYLINDER;
+    case 'd': return SURF_TORUS;
+    case 'f': return SURF_FLAT;
+    case 'm': return SURF_ZERNIKE;
+    case 'i': return SURF_SPLINE;
+    case 'r': return SURF_POLY;
+    default: return SURF_GENERAL;  // 'g', 'c'
+    }
+}
+
+// Surface height used only to bound where accepted hits can lie.
+static double bound_z(const HElement &e, double x, double y) {
+    switch (tolower(e.surf)) {
+    case 's': {
+        double c = e.vcx;
+        double r2 = x * x + (e.stype == 7 ? 0.0 : y * y);
+        double q = 1.0 - c * c * r2;
+        if (q < 0) q = 0;
+        return c * r2 / (1.0 + sqrt(q));
+    }
+    case 'p':
+        return e.vcx * x * x / 2.0 + (e.stype == 7 ? 0.0 : e.vcy * y * y / 2.0);
+    case 'o': case 'g': {
+        if (e.cone_half != 0.0) return sqrt(x * x + y * y) / tan(e.cone_half * kDegRad);
+        double r2 = e.vcx * x * x + e.vcy * y * y;
+        double q = 1.0 - e.kappa * (e.vcx * e.vcx * x * x + e.vcy * e.vcy * y * y);
+        if (q < 0) q = 0;
+        return r2 / (1.0 + sqrt(q));
+    }
+    case 'c':
+        return sqrt(x * x + y * y) / tan(e.cone_half * kDegRad);
+    case 'm': {
+        int n = e.fit_order + 1;
+        double z = 0.0;
+        for (int i = 0; i <= e.fit_order; i++)
+            for (int j = 0; j <= i; j++) z += e.zern[(size_t)i * n + j] * pow(x, j) * pow(y, i - j);
+        return z;
+    }
+    case 'r': {
+        double r = (e.shape == 'l' || e.shape == 'L') ? fabs(x) : sqrt(x * x + y * y);
+        double z = 0.0;
+        for (int i = 0; i <= e.fit_order; i++) z += e.poly[i] * pow(r, i);
+        return z;
+    }
+    case 'i': {
+        double r = (e.shape == 'l' || e.shape == 'L') ? x : sqrt(x * x + y * y);
+        const int n = (int)e.sx.size();
+        if (n < 2) return 0.0;
+        if (r <= e.sx[0]) return e.sy[0];
+        if (r >= e.sx[n - 1]) return e.sy[n - 1];
+        int lo = 0, hi = n - 1;
+        while (hi - lo > 1) {
+            int k = (lo + hi) / 2;
+            if (e.sx[k] > r) hi = k; else lo = k;
+        }
+        double h = e.sx[hi] - e.sx[lo];
+        double a = (e.sx[hi] - r) / h, b = (r - e.sx[lo]) / h;
+        return a * e.sy[lo] + b * e.sy[hi] + ((a * a * a - a) * e.sy2[lo] + (b * b * b - b) * e.sy2[hi]) * (h * h) / 6.0;
+    }
+    default:
+        return 0.0;
+    }
+}
+
+// Box in the element frame holding every hit point the trace can accept.
+static void element_bounds(const HElement &e, double lo[3], double hi[3]) {
+    const double A = e.p[0], B = e.p[1], C = e.p[2];
+    double xl, xh, yl, yh;
+    const char sh = (char)tolower(e.shape);
+    const char sf = (char)tolower(e.surf);
+    switch (sh) {
+    case 'c': case 'h': case 't':
+        xl = yl = -A / 2.0; xh = yh = A / 2.0;
+        break;
+    case 'r':
+        xl = -A / 2.0; xh = A / 2.0; yl = -B / 2.0; yh = B / 2.0;
+        break;
+    case 'a':
+        if (A == 0.0 && B == 0.0) {
+            double r = (sf == 'd') ? e.ann_r + e.cross_r : 1e3;
+            xl = yl = -r; xh = yh = r;
+        } else {
+            xl = yl = -fabs(B); xh = yh = fabs(B);
+        }
+        break;
+    case 'l':
+        if (A == 0.0 && B == 0.0) {
+            double r = (sf == 't') ? 1.0 / fabs(e.curv_rev) : 1e3;
+            xl = -r; xh = r;
+        } else {
+            xl = std::min(A, B); xh = std::max(A, B);
+        }
+        yl = -C / 2.0; yh = C / 2.0;
+        break;
+    default: {  // i, q
+        int nv = (sh == 'q') ? 4 : 3;
+        xl = yl = 1e300; xh = yh = -1e300;
+        for (int k = 0; k < nv; k++) {
+            xl = std::min(xl, e.p[2 * k]); xh = std::max(xh, e.p[2 * k]);
+            yl = std::min(yl, e.p[2 * k + 1]); yh = std::max(yh, e.p[2 * k + 1]);
+        }
+        break;
+    }
+    }
+    double zl = std::min(0.0, e.zap), zh = std::max(0.0, e.zap);
+    if (sf == 't') {
+        zl = 0.0; zh = 2.0 / fabs(e.curv_rev);
+    } else if (sf == 'd') {
+        zl = 0.0; zh = 2.0 * e.cross_r;
+    } else if (sf != 'f') {
+        const int G = 64;
+        for (int i = 0; i <= G; i++)
+            for (int j = 0; j <= G; j++) {
+                double x = xl + (xh - xl) * i / G, y = yl + (yh - yl) * j / G;
+                double z = bound_z(e, x, y);
+                if (std::isfinite(z)) { zl = std::min(zl, z); zh = std::max(zh, z); }
+            }
+        double pad = 0.05 * (zh - zl) + 1e-3;
+        zl -= pad; zh += pad;
+    }
+    double pad = 1e-4 + 1e-7 * std::max({fabs(xl), fabs(xh), fabs(yl), fabs(yh)});
+    lo[0] = xl - pad; hi[0] = xh + pad;
+    lo[1] = yl - pad; hi[1] = yh + pad;
+    lo[2] = zl - pad; hi[2] = zh + pad;
+}
+
+// ------------------------------------------------------------------- BVH --
+struct BuildItem {
+    float lo[3], hi[3];
+    float c[3];
+    int32_t idx;
+};
+
+static int build_bvh(std::vector<BuildItem> &items, int begin, int end, std::vector<DevBvhNode> &nodes,
+                     std::vector<int32_t> &out_items) {
+    int me = (int)nodes.size();
+    nodes.push_back(DevBvhNode{});
+    float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+    float clo[3] = {3e38f, 3e38f, 3e38f}, chi[3] = {-3e38f, -3e38f, -3e38f};
+    for (int i = begin; i < end; i++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], items[i].lo[k]); hi[k] = std::max(hi[k], items[i].hi[k]);
+            clo[k] = std::min(clo[k], items[i].c[k]); chi[k] = std::max(chi[k], items[i].c[k]);
+        }
+    DevBvhNode nd{};
+    for (int k = 0; k < 3; k++) { nd.lo[k] = lo[k]; nd.hi[k] = hi[k]; }
+    const int n = end - begin;
+    if (n <= 4) {
+        nd.left = -((int)out_items.size() + 1);
+        nd.right = n;
+        for (int i = begin; i < end; i++) out_items.push_back(items[i].idx);
+        nodes[me] = nd;
+        return me;
+    }
+    int ax = 0;
+    float ext = chi[0] - clo[0];
+    for (int k = 1; k < 3; k++)
+        if (chi[k] - clo[k] > ext) { ext = chi[k] - clo[k]; ax = k; }
+    int mid = begin + n / 2;
+    std::nth_element(items.begin() + begin, items.begin() + mid, items.begin() + end,
+                     [ax](const BuildItem &a, const BuildItem &b) { return a.c[ax] < b.c[ax]; });
+    int l = build_bvh(items, begin, mid, nodes, out_items);
+    int r = build_bvh(items, mid, end, nodes, out_items);
+    nd.left = l;
+    nd.right = r;
+    nodes[me] = nd;
+    return me;
+}
+
+static float round_down(double v) {
+    float f = (float)v;
+    if ((double)f > v) f = nextafterf(f, -3e38f);
+    return f;
+}
+static float round_up(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = nextafterf(f, 3e38f);
+    return f;
+}
+
+// ------------------------------------------------------------- compile ----
+bool compile_scene(SceneModel &m, bool use_cull, CompiledScene &out, std::string &msg) {
+    char b[600];
+    out = CompiledScene();
+    // InitGeometries (coretrace/input.cpp:153-286)
+    for (size_t i = 0; i < m.stages.size(); i++) {
+        HStage &st = m.stages[i];
+        if (!euler_from_aim(st.origin, st.aim, st.zrot, st.euler)) {
+            snprintf(b, sizeof b, "Stage %d geometry as defined not possible.\n", (int)i);
+            msg = b;
+            return false;
+        }
+        euler_matrix(st.euler, st.R);
+        for (size_t j = 0; j < st.elements.size(); j++) {
+            HElement &e = st.elements[j];
+            if (!euler_from_aim(e.origin, e.aim, e.zrot, e.euler)) {
+                snprintf(b, sizeof b, "Stage %d element %d geometry as defined not possible.\n", (int)i, (int)j);
+                msg = b;
+                return false;
+            }
+            euler_matrix(e.euler, e.R);
+            if (ap_kind(e.shape) == 0) {
+                snprintf(b, sizeof b, "invalid shape '%c' for aperture, element %d\n", e.shape, (int)j);
+                msg = b;
+                return false;
+            }
+            if (!st.virt) {
+                const HOptic *set = nullptr;
+                for (const HOptic &o : m.optics)
+                    if (strcasecmp(o.name.c_str(), e.optic_name.c_str()) == 0) set = &o;
+                if (!set) {
+                    snprintf(b, sizeof b, "Stage %d element %d invalid optical property set name: '%s'\n", (int)i,
+                             (int)j, e.optic_name.c_str());
+                    msg = b;
+                    return false;
+                }
+            }
+            aperture_plane(e);
+        }
+    }
+    // refractive index wiring (coretrace/input.cpp:269-275), applied to
+    // every optic that any non-virtual element uses
+    for (HOptic &o : m.optics) {
+        bool used = false;
+        for (const HStage &st : m.stages)
+            if (!st.virt)
+                for (const HElement &e : st.elements)
+                    if (strcasecmp(o.name.c_str(), e.optic_name.c_str()) == 0) used = true;
+        if (!used) continue;
+        o.front.refr[2] = o.back.refr[0];
+        o.front.refr[3] = o.back.refr[1];
+        o.back.refr[2] = o.front.refr[0];
+        o.back.refr[3] = o.front.refr[1];
+    }
+    // CheckInputs (coretrace/input.cpp:87-150)
+    for (size_t i = 0; i < m.stages.size(); i++)
+        for (size_t j = 0; j < m.stages[i].elements.size(); j++) {
+            const HElement &e = m.stages[i].elements[j];
+            char s = (char)tolower(e.surf), a = (char)tolower(e.shape);
+            const char *err = nullptr;
+            if (s == 'p' && a == 'l' && e.vcy != 0.0)
+                err = "Parabolic surface combined with single axis curvature section aperture must have Cy = 0.";
+            else if (s == 's' && e.vcx == 0.0)
+                err = "Spherical surface must have a curvature greater than 0.0.";
+            else if (s == 's' && e.vcy != 0.0 && e.vcy != e.vcx)
+                err = "Spherical surface must have Cy=0 or Cy=Cx.";
+            else if (s == 'o' && (e.kappa == 1.0 || e.kappa == 0.0))
+                err = "Kappa = 0.0 or 1.0 defines either a parabola or a sphere.";
+            if (err) {
+                snprintf(b, sizeof b, "%s  Element %d, Stage %d.", err, (int)j + 1, (int)i + 1);
+                msg = b;
+                return false;
+            }
+        }
+
+    // --------------------------------------------------- device tables ----
+    auto add_table = [&](const std::vector<double> &a, const std::vector<double> &v) {
+        int off = (int)out.aux.size();
+        out.aux.insert(out.aux.end(), a.begin(), a.end());
+        out.aux.insert(out.aux.end(), v.begin(), v.end());
+        return off;
+    };
+    std::vector<int> optic_face0(m.optics.size(), -1);
+    for (size_t k = 0; k < m.optics.size(); k++) {
+        optic_face0[k] = (int)out.faces.size();
+        for (int fb = 0; fb < 2; fb++) {
+            const HFace &h = fb ? m.optics[k].back : m.optics[k].front;
+            DevFace f{};
+            f.dist = (unsigned char)h.dist;
+            f.reflectivity = h.reflectivity;
+            f.transmissivity = h.transmissivity;
+            f.slope = h.slope;
+            f.spec = h.spec;
+            f.n_in = h.refr[0];
+            f.n_out = h.refr[2];
+            f.refl_n = h.use_refl ? (int)h.refl_ang.size() : 0;
+            f.trans_n = h.use_trans ? (int)h.trans_ang.size() : 0;
+            if (f.refl_n) f.refl_tab = add_table(h.refl_ang, h.refl_val);
+            if (f.trans_n) f.trans_tab = add_table(h.trans_ang, h.trans_val);
+            out.faces.push_back(f);
+        }
+    }
+    for (size_t i = 0; i < m.stages.size(); i++) {
+        HStage &st = m.stages[i];
+        DevStage ds{};
+        for (int k = 0; k < 3; k++) ds.org[k] = st.origin[k];
+        for (int k = 0; k < 9; k++) ds.R[k] = st.R[k];
+        ds.elem0 = (int)out.elems.size();
+        ds.nelem = (int)st.elements.size();
+        ds.virt = st.virt;
+        ds.multihit = st.multihit;
+        ds.tracethrough = st.tracethrough;
+        std::vector<BuildItem> items;
+        for (size_t j = 0; j < st.elements.size(); j++) {
+            HElement &e = st.elements[j];
+            DevElem d{};
+            for (int k = 0; k < 3; k++) d.org[k] = e.origin[k];
+            for (int k = 0; k < 9; k++) d.R[k] = e.R[k];
+            for (int k = 0; k < 8; k++) d.p[k] = e.p[k];
+            d.zap = e.zap;
+            d.kappa = e.kappa;
+            d.vcx = e.vcx;
+            d.vcy = e.vcy;
+            d.ann_r = e.ann_r;
+            d.cross_r = e.cross_r;
+            d.curv_rev = e.curv_rev;
+            d.cone_tan = (e.cone_half != 0.0) ? tan(e.cone_half * kDegRad) : 0.0;
+            d.ap = ap_kind(e.shape);
+            d.surf = surf_kind(e);
+            d.stype = e.stype;
+            d.interaction = e.interaction;
+            d.enabled = e.enabled;
+            d.fit_order = e.fit_order;
+            d.zap_filter = !(tolower(e.surf) == 'm' || tolower(e.surf) == 'r');
+            d.face_front = d.face_back = -1;
+            if (!st.virt) {
+                for (size_t k = 0; k < m.optics.size(); k++)
+                    if (strcasecmp(m.optics[k].name.c_str(), e.optic_name.c_str()) == 0) {
+                        d.face_front = optic_face0[k];
+                        d.face_back = optic_face0[k] + 1;
+                    }
+            }
+            if (d.surf == SURF_ZERNIKE) {
+                d.aux = (int)out.aux.size();
+                d.aux_n = e.fit_order + 1;
+                out.aux.insert(out.aux.end(), e.zern.begin(), e.zern.end());
+            } else if (d.surf == SURF_SPLINE) {
+                d.aux = (int)out.aux.size();
+                d.aux_n = (int)e.sx.size();
+                out.aux.insert(out.aux.end(), e.sx.begin(), e.sx.end());
+                out.aux.insert(out.aux.end(), e.sy.begin(), e.sy.end());
+                out.aux.insert(out.aux.end(), e.sy2.begin(), e.sy2.end());
+            } else if (d.surf == SURF_POLY) {
+                d.aux = (int)out.aux.size();
+                d.aux_n = (int)e.poly.size();
+                out.aux.insert(out.aux.end(), e.poly.begin(), e.poly.end());
+            }
+            double lo[3], hi[3];
+            element_bounds(e, lo, hi);
+            double c[3] = {(lo[0] + hi[0]) / 2, (lo[1] + hi[1]) / 2, (lo[2] + hi[2]) / 2};
+            double r = 0.5 * sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                  (hi[2] - lo[2]) * (hi[2] - lo[2]));
+            for (int k = 0; k < 3; k++) d.bsphere_c[k] = c[k];
+            d.bsphere_r = r;
+            out.elems.push_back(d);
+            if (e.enabled) {
+                // stage-frame box from the 8 corners of the element-frame box
+                BuildItem it;
+                double slo[3] = {1e300, 1e300, 1e300}, shi[3] = {-1e300, -1e300, -1e300};
+                for (int q = 0; q < 8; q++) {
+                    double v[3] = {(q & 1) ? hi[0] : lo[0], (q & 2) ? hi[1] : lo[1], (q & 4) ? hi[2] : lo[2]};
+                    double w[3];
+                    mtv(e.R, v, w);
+                    for (int k = 0; k < 3; k++) {
+                        w[k] += e.origin[k];
+                        slo[k] = std::min(slo[k], w[k]);
+                        shi[k] = std::max(shi[k], w[k]);
+                    }
+                }
+                for (int k = 0; k < 3; k++) {
+                    double pad = 1e-2 + 1e-5 * std::max(fabs(slo[k]), fabs(shi[k]));
+                    it.lo[k] = round_down(slo[k] - pad);
+                    it.hi[k] = round_up(shi[k] + pad);
+                    it.c[k] = 0.5f * (it.lo[k] + it.hi[k]);
+                }
+                it.idx = (int32_t)j;
+                items.push_back(it);
+            }
+        }
+        if (use_cull && items.size() > 8) {
+            ds.bvh0 = (int)out.bvh.size();
+            std::vector<DevBvhNode> nodes;
+            build_bvh(items, 0, (int)items.size(), nodes, out.bvh_items);
+            ds.nbvh = (int)nodes.size();
+            // child indices are stage-local (the kernel offsets the node base)
+            out.bvh.insert(out.bvh.end(), nodes.begin(), nodes.end());
+        }
+        out.stages.push_back(ds);
+    }
+    return true;
+}
+
+}  // namespace st
+
+namespace st {
+
+// Sun frame, sun-plane box and the stage-1 sun-plane grid
+// (coretrace/suntoprimarystage.cpp:59-293).
+bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
+    HStage &st = m.stages[0];
+    HSun &sun = m.sun;
+    double dx = 0.0 - sun.origin[0], dy = 0.0 - sun.origin[1], dz = 0.0 - sun.origin[2];
+    double dtot = sqrt(dx * dx + dy * dy + dz * dz);
+    if (dtot == 0.0) {
+        msg = "error calculating sun position in primary stage, dtot = 0.0\n";
+        return false;
+    }
+    double cg[3] = {dx / dtot, dy / dtot, dz / dtot};
+    double dd[3] = {st.origin[0] - st.origin[0], st.origin[1] - st.origin[1], st.origin[2] - st.origin[2]};
+    double pss[3], cs[3];
+    mv(st.R, dd, pss);
+    mv(st.R, cg, cs);
+    double se[3] = {atan2(cs[0], cs[2]), asin(cs[1]), 0.0};
+    double Rs[9];
+    euler_matrix(se, Rs);
+    DevSun &ds = out.sun;
+    for (int k = 0; k < 3; k++) { ds.origin[k] = sun.origin[k]; ds.pos_sun_stage[k] = pss[k]; }
+    for (int k = 0; k < 9; k++) ds.R[k] = Rs[k];
+    double xs = 0.0, ys = 0.0;
+    for (const HElement &e : st.elements) {
+        if (!e.enabled) continue;
+        double pl[3];
+        mv(Rs, e.origin, pl);
+        xs = xs + pl[0];
+        ys = ys + pl[1];
+    }
+    double xcm = xs / st.elements.size(), ycm = ys / st.elements.size();
+    (void)xcm; (void)ycm;
+    double minx = 1.0e20, maxx = -1.0e20, miny = 1.0e20, maxy = -1.0e20;
+    size_t n = 0;
+    for (const HElement &e : st.elements) {
+        if (!e.enabled) continue;
+        double pl[3];
+        mv(Rs, e.origin, pl);
+        double xmn = pl[0], xmx = pl[0], ymn = pl[1], ymx = pl[1];
+        const double A = e.p[0], B = e.p[1], C = e.p[2];
+        double pad = 0.0;
+        bool padded = true;
+        switch (e.shape) {
+        case 'c': case 'C': case 'h': case 'H': pad = A / 2.0; break;
+        case 'r': case 'R': pad = sqrt(((A * A) + (B * B)) / 4.0); break;
+        case 't': case 'T': pad = A / (2.0 * cos(30.0 * (kDegRad))); break;
+        case 'a': case 'A': pad = B; break;
+        case 'l': case 'L': {
+            double xl = (fabs(B) >= fabs(A)) ? B : A;
+            pad = sqrt((xl * xl) + 0.25 * C * C);
+            break;
+        }
+        case 'i': case 'I': case 'q': case 'Q': {
+            int nv = (e.shape == 'q' || e.shape == 'Q') ? 4 : 3;
+            double rt = 0.0;
+            for (int k = 0; k < nv; k++) {
+                double r = sqrt((e.p[2 * k] * e.p[2 * k]) + (e.p[2 * k + 1] * e.p[2 * k + 1]));
+                if (k == 0 || r > rt) rt = r;
+            }
+            pad = rt;
+            break;
+        }
+        default: padded = false; break;
+        }
+        if (padded) { xmn = xmn - pad; ymn = ymn - pad; xmx = xmx + pad; ymx = ymx + pad; }
+        if (xmn < minx) minx = xmn;
+        if (xmx > maxx) maxx = xmx;
+        if (ymn < miny) miny = ymn;
+        if (ymx > maxy) maxy = ymx;
+        n++;
+    }
+    if (n == 0) {
+        msg = "error calculating sun position in primary stage because no elements were enabled";
+        return false;
+    }
+    sun.minx = ds.minx = minx;
+    sun.maxx = ds.maxx = maxx;
+    sun.miny = ds.miny = miny;
+    sun.maxy = ds.maxy = maxy;
+    ds.shape = (unsigned char)sun.shape;
+    ds.point_source = sun.point_source;
+    ds.sigma = sun.sigma;
+    ds.max_angle = sun.max_angle;
+    ds.max_intensity = sun.max_intensity;
+    ds.tab_n = (int)sun.angle.size();
+    if (ds.tab_n > 0) {
+        ds.tab = (int)out.aux.size();
+        out.aux.insert(out.aux.end(), sun.angle.begin(), sun.angle.end());
+        out.aux.insert(out.aux.end(), sun.intensity.begin(), sun.intensity.end());
+        ds.tab_ascending = 1;
+        for (int k = 1; k < ds.tab_n; k++)
+            if (!(sun.angle[k] > sun.angle[k - 1])) ds.tab_ascending = 0;
+    }
+
+    // sun-plane grid of stage-1 element footprints (bounding spheres)
+    const DevStage &s0 = out.stages[0];
+    std::vector<double> fx, fy, fr;
+    std::vector<int32_t> fj;
+    double med = 0.0;
+    for (int j = 0; j < s0.nelem; j++) {
+        const DevElem &d = out.elems[s0.elem0 + j];
+        if (!d.enabled) continue;
+        double w[3], c[3];
+        mtv(d.R, d.bsphere_c, w);
+        for (int k = 0; k < 3; k++) c[k] = w[k] + d.org[k] - pss[k];
+        double q[3];
+        mv(Rs, c, q);
+        fx.push_back(q[0]);
+        fy.push_back(q[1]);
+        fr.push_back(d.bsphere_r * (1.0 + 1e-9) + 1e-6 + 1e-12 * (fabs(q[0]) + fabs(q[1])));
+        fj.push_back(j);
+    }
+    if (!fr.empty()) {
+        std::vector<double> tmp(fr);
+        std::nth_element(tmp.begin(), tmp.begin() + tmp.size() / 2, tmp.end());
+        med = tmp[tmp.size() / 2];
+    }
+    double w = maxx - minx, h = maxy - miny;
+    double cell = std::max(2.0 * med, std::max(w, h) / 4096.0);
+    if (!(cell > 0)) cell = std::max(w, h) + 1.0;
+    int nx = std::max(1, std::min(4096, (int)ceil(w / cell)));
+    int ny = std::max(1, std::min(4096, (int)ceil(h / cell)));
+    DevSunGrid &g = out.grid;
+    g.x0 = minx;
+    g.y0 = miny;
+    g.nx = nx;
+    g.ny = ny;
+    g.inv_dx = nx / (w > 0 ? w : 1.0);
+    g.inv_dy = ny / (h > 0 ? h : 1.0);
+    std::vector<int32_t> cnt((size_t)nx * ny + 1, 0);
+    auto span = [&](size_t k, int &ax, int &bx, int &ay, int &by) {
+        ax = (int)floor((fx[k] - fr[k] - g.x0) * g.inv_dx);
+        bx = (int)floor((fx[k] + fr[k] - g.x0) * g.inv_dx);
+        ay = (int)floor((fy[k] - fr[k] - g.y0) * g.inv_dy);
+        by = (int)floor((fy[k] + fr[k] - g.y0) * g.inv_dy);
+        ax = std::max(0, std::min(nx - 1, ax)); bx = std::max(0, std::min(nx - 1, bx));
+        ay = std::max(0, std::min(ny - 1, ay)); by = std::max(0, std::min(ny - 1, by));
+    };
+    for (size_t k = 0; k < fx.size(); k++) {
+        int ax, bx, ay, by;
+        span(k, ax, bx, ay, by);
+        for (int y = ay; y <= by; y++)
+            for (int x = ax; x <= bx; x++) cnt[(size_t)y * nx + x + 1]++;
+    }
+    for (size_t c = 1; c < cnt.size(); c++) cnt[c] += cnt[c - 1];
+    out.grid_items.assign(cnt.back(), 0);
+    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+    for (size_t k = 0; k < fx.size(); k++) {
+        int ax, bx, ay, by;
+        span(k, ax, bx, ay, by);
+        for (int y = ay; y <= by; y++)
+            for (int x = ax; x <= bx; x++) out.grid_items[fill[(size_t)y * nx + x]++] = fj[k];
+    }
+    out.grid_cells.swap(cnt);
+    out.use_grid = !sun.point_source;
+    return true;
+}
+
+}  // namespace st

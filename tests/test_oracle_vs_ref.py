@@ -1,0 +1,145 @@
+"""Pins the oracle to the reference: in RNG mode 0 (the reference's own
+sequential libstdc++ stream) the oracle's full record stream must equal the
+reference build's bit for bit -- positions, cosines, elements, stages, ray
+numbers, sun box and sun-ray count.  Scenes cover every in-scope surface,
+aperture and optics branch the trace loop has."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from soltrace_amd import workloads
+from soltrace_amd.scene import Element, Optic, OpticFace, Scene, Stage
+from harness import run_scene
+
+
+def _bitexact(ra, rb):
+    assert len(ra) == len(rb)
+    for f in ("pos", "cos"):
+        a, b = getattr(ra, f), getattr(rb, f)
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), f
+    for f in ("element", "stage", "raynum"):
+        assert np.array_equal(getattr(ra, f), getattr(rb, f)), f
+    assert ra.nsunrays == rb.nsunrays
+    assert ra.sun_box == rb.sun_box
+
+
+def _compare(ref_lib, oracle_lib, sc, seed):
+    n_r, r_r, m_r, _ = run_scene(ref_lib, sc, seed)
+    n_o, r_o, m_o, _ = run_scene(oracle_lib, sc, seed)
+    assert n_r == n_o, (n_r, n_o, m_r, m_o)
+    if n_r < 0:
+        return n_r
+    _bitexact(r_o, r_r)
+    return n_r
+
+
+def test_trough_bitexact(ref_lib, oracle_lib):
+    # > 8192 records per stage, so TRayData::Merge's block reordering matters
+    n = _compare(ref_lib, oracle_lib, workloads.trough(raycount=12000), 123)
+    assert n > 2 * 8192
+
+
+def test_tower_buie_bitexact(ref_lib, oracle_lib):
+    sc = workloads.tower(n_helio=40, raycount=3000, r_max=300.0)
+    assert _compare(ref_lib, oracle_lib, sc, 77) > 3000
+
+
+def test_tower_dynamic_group_cull_is_exact(ref_lib, oracle_lib):
+    # the reference's sun-plane hash and polar receiver hash (dynamic
+    # grouping) give the same records as the oracle's full scan here
+    sc = workloads.tower(n_helio=40, raycount=3000, r_max=300.0, dyn_group=True)
+    assert _compare(ref_lib, oracle_lib, sc, 78) > 3000
+
+
+def _mini(surface, sp, ap, app, sun="p", dist="g", interaction=2, extra_stage=True, surf_file=None,
+          refl=0.9, refr=(1.0, 1.5), stage_flags=(False, True, False), slope=1.0, spec=0.5, n=1500):
+    s = Scene(sun_shape=sun, sun_sigma=4.65 if sun != "g" else 2.0, sun_xyz=(0.2, 0.1, 10.0), raycount=n,
+              maxcount=400000)
+    front = OpticFace(dist=dist, refl=refl, trans=0.9, slope=slope, spec=spec, rreal=refr[0])
+    back = OpticFace(dist=dist, refl=refl, trans=0.9, slope=slope, spec=spec, rreal=refr[1])
+    s.optics = [Optic("m", front, back), Optic("abs", OpticFace(refl=0.2), OpticFace(refl=0.2))]
+    el = Element(xyz=(0, 0, 0), aim=(0, 0, 1), zrot=10.0, aperture=ap, ap_params=app, surface=surface,
+                 surf_params=sp, surf_file=surf_file, interaction=interaction, optic="m")
+    virt, mh, tt = stage_flags
+    s.stages = [Stage(elements=[el], virtual=False, multihit=mh)]
+    if surface == "d":
+        # a torus with the 'a' aperture pads the sun box by 0 (coretrace/
+        # suntoprimarystage.cpp:230-237): a wide flat element below it opens
+        # the box
+        s.stages[0].elements.append(Element(xyz=(0, 0, -0.5), aim=(0, 0, 1), aperture="r", ap_params=(3.0, 3.0),
+                                            surface="f", optic="m"))
+    if extra_stage:
+        tgt = Element(xyz=(0, 0, 1.0), aim=(0, 0.1, -5), aperture="r", ap_params=(3.0, 3.0), surface="f",
+                      optic="abs", interaction=2)
+        s.stages.append(Stage(elements=[tgt], virtual=virt, tracethrough=tt))
+    return s
+
+
+CASES = {
+    "flat_rect": dict(surface="f", sp=(), ap="r", app=(2.0, 1.5)),
+    "flat_circle": dict(surface="f", sp=(), ap="c", app=(2.0,)),
+    "flat_hex": dict(surface="f", sp=(), ap="h", app=(2.0,)),
+    "flat_tri": dict(surface="f", sp=(), ap="t", app=(2.0,)),
+    "flat_itri": dict(surface="f", sp=(), ap="i", app=(-1.0, -0.8, 1.1, -0.5, 0.2, 1.0)),
+    "flat_quad": dict(surface="f", sp=(), ap="q", app=(-1.0, -0.8, 1.1, -0.7, 0.9, 1.0, -0.8, 0.9)),
+    "flat_annulus": dict(surface="f", sp=(), ap="a", app=(0.3, 1.2, 270.0)),
+    "sphere_circle": dict(surface="s", sp=(0.3,), ap="c", app=(2.0,)),
+    "sphere_line": dict(surface="s", sp=(0.3,), ap="l", app=(-1.0, 1.0, 2.0)),
+    "parabola_rect": dict(surface="p", sp=(0.4, 0.4), ap="r", app=(2.0, 2.0)),
+    "hyperboloid": dict(surface="o", sp=(0.3, -0.5), ap="c", app=(2.0,)),
+    "ellipsoid": dict(surface="o", sp=(0.3, 0.5), ap="h", app=(2.0,)),
+    "torus": dict(surface="d", sp=(0.8, 0.3), ap="a", app=(0.0, 0.0, 120.0)),
+    "cylinder": dict(surface="t", sp=(1.0 / 0.4,), ap="l", app=(0.0, 0.0, 2.0)),
+    "cone_general": dict(surface="c", sp=(70.0,), ap="c", app=(2.0,)),
+    "general_conic": dict(surface="g", sp=(0.3, 0.2, 0.0, 0, 0, 0, 0, 0), ap="r", app=(2.0, 2.0)),
+    "pillbox_optics": dict(surface="f", sp=(), ap="r", app=(2.0, 2.0), dist="p"),
+    "diffuse_optics": dict(surface="f", sp=(), ap="r", app=(2.0, 2.0), dist="f"),
+    "refraction": dict(surface="s", sp=(0.2,), ap="c", app=(2.0,), interaction=1),
+    "gauss_sun": dict(surface="p", sp=(0.4, 0.4), ap="c", app=(2.0,), sun="g"),
+    "virtual_target": dict(surface="f", sp=(), ap="r", app=(2.0, 2.0), stage_flags=(True, True, False)),
+    "tracethrough": dict(surface="f", sp=(), ap="r", app=(2.0, 2.0), stage_flags=(False, True, True)),
+    "single_hit": dict(surface="s", sp=(0.6,), ap="c", app=(2.0,), stage_flags=(False, False, False)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_feature_scenes_bitexact(ref_lib, oracle_lib, name):
+    sc = _mini(**CASES[name])
+    n = _compare(ref_lib, oracle_lib, sc, 31)
+    assert n > 0
+
+
+def _write(tmpdir, name, text):
+    p = os.path.join(tmpdir, name)
+    with open(p, "w") as f:
+        f.write(text)
+    return p
+
+
+def test_zernike_and_spline_files_bitexact(ref_lib, oracle_lib):
+    with tempfile.TemporaryDirectory() as d:
+        # z = B20 y^2 + B22 x^2 + small higher terms (order 3)
+        mon = _write(d, "z.mon", "3\n0\n0\n0\n0.2\n0.001\n0.22\n0.0005\n0\n0.0003\n0.0001\n")
+        sc = _mini(surface="m", sp=(), ap="c", app=(2.0,), surf_file=mon)
+        assert _compare(ref_lib, oracle_lib, sc, 3) > 0
+        rows = "".join(f"{x:.3f} {0.25 * x * x:.6f}\n" for x in np.linspace(0.0, 1.2, 13))
+        csi = _write(d, "s.csi", f"13\n{rows}0 0.6\n")
+        sc = _mini(surface="i", sp=(), ap="a", app=(0.0, 1.2, 360.0), surf_file=csi)
+        assert _compare(ref_lib, oracle_lib, sc, 4) > 0
+        ply = _write(d, "p.ply", "2\n0\n0\n0.2\n")
+        sc = _mini(surface="r", sp=(), ap="c", app=(2.0,), surf_file=ply)
+        assert _compare(ref_lib, oracle_lib, sc, 5) > 0
+
+
+def test_error_paths_match(ref_lib, oracle_lib):
+    sc = workloads.trough(raycount=1000)
+    sc.maxcount = 50  # sun-ray cap reached (coretrace/raytrace.cpp:600-604)
+    n_r, _, m_r, _ = run_scene(ref_lib, sc, 1)
+    n_o, _, m_o, _ = run_scene(oracle_lib, sc, 1)
+    assert n_r == n_o == -1
+    assert "maximum count" in m_r[-1] and "maximum count" in m_o[-1]
+    sc = workloads.trough(raycount=1000)
+    sc.stages[0].elements[0].optic = "nope"  # unknown optic name
+    assert run_scene(ref_lib, sc, 1)[0] == run_scene(oracle_lib, sc, 1)[0] == -1
