@@ -45,8 +45,19 @@ int st_amd_ray_sun_index(st_context_t pcxt, unsigned long long *out, int n);
 /* Device timing of the last run (HIP events): [0] sun-mask ms,
  * [1..16] trace ms per stage, [17] compaction ms, [18] wall ms of st_sim_run,
  * [19] sun rays drawn, [20..35] rays entering each stage, [36..51] records
- * per stage.  Writes min(n, 52) values; returns 52. */
+ * per stage, [52] ms of the cooperative tabulated-sunshape draws.  Writes
+ * min(n, 53) values; returns 53. */
 int st_amd_timing(st_context_t pcxt, double *out, int n);
+
+/* Count kernel events during the next runs (off by default: it adds
+ * atomics).  Returns 1. */
+int st_amd_count_events(st_context_t pcxt, int on);
+
+/* Launch counts and event counters of the last run: [0] sun-mask launches,
+ * [1..16] trace launches per stage, then five counters for the sun mask and
+ * five per stage (rays, element tests, BVH nodes, uniforms drawn, records).
+ * Writes min(n, 102) values; returns 102. */
+int st_amd_events(st_context_t pcxt, double *out, int n);
 
 #ifdef __cplusplus
 }

@@ -176,7 +176,7 @@ static bool trace(System &sys, uint32_t seed, uint64_t N, uint64_t maxrays, bool
                     rs.pg[k] = g.pos[k];
                     rs.cg[k] = g.cos[k];
                 }
-                rng.begin_ray((uint32_t)si, ray_number);
+                rng.begin_ray((uint32_t)si, sun_index);
             }
             if (rng.mode() == 1) rs.cos_out_el[0] = rs.cos_out_el[1] = rs.cos_out_el[2] = 0.0;
             to_local(rs.pg, rs.cg, st.origin, st.R, rs.ps, rs.cs);

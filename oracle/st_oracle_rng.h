@@ -21,12 +21,12 @@
 //
 //  * CounterStream -- the counter-based per-ray stream the GPU kernels use:
 //    Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
-//    easy as 1, 2, 3", SC'11).  Uniform k of stream (stage, id) comes from the
-//    Philox block with counter (id_lo, id_hi, stage, k/2) and key
-//    (seed, 0x53544F52): words 2(k%2), 2(k%2)+1 form a 64-bit value w and
-//    u = (w >> 11) * 2^-53.  Normals use the same polar method as above with
-//    the same caching, so both streams feed identical algorithms.
-//    Checked against the Random123 known-answer vectors.
+//    easy as 1, 2, 3", SC'11).  Uniform k of stream (stage, id) is word k%4
+//    of the Philox block with counter (id_lo, id_hi, stage, k/4) and key
+//    (seed, 0x53544F52), scaled by 2^-32.  The id of every stage's stream
+//    is the ray's 0-based sun-ray index.  Normals use the same polar method
+//    as above with the same caching, so both streams feed identical
+//    algorithms.  Checked against the Random123 known-answer vectors.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -60,14 +60,12 @@ inline Philox4 philox4x32_10(const uint32_t ctr[4], const uint32_t key[2]) {
 
 static const uint32_t kCounterKeyHi = 0x53544F52u;  // second key word
 
-// Uniform k of the per-ray stream (seed, stage, id).
+// Uniform k of the per-ray stream (seed, stage, id): word k%4 of block k/4.
 inline double counter_uniform(uint32_t seed, uint32_t stage, uint64_t id, uint64_t k) {
-    uint32_t ctr[4] = {(uint32_t)id, (uint32_t)(id >> 32), stage, (uint32_t)(k >> 1)};
+    uint32_t ctr[4] = {(uint32_t)id, (uint32_t)(id >> 32), stage, (uint32_t)(k >> 2)};
     uint32_t key[2] = {seed, kCounterKeyHi};
     Philox4 b = philox4x32_10(ctr, key);
-    int o = (int)(k & 1) * 2;
-    uint64_t w = ((uint64_t)b.v[o + 1] << 32) | (uint64_t)b.v[o];
-    return (double)(w >> 11) * (1.0 / 9007199254740992.0);
+    return (double)b.v[k & 3] * 2.3283064365386963e-10;  // 2^-32
 }
 
 // ------------------------------------------------------------ RefStream ---

@@ -18,7 +18,10 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("SOLTRACE_AMD_ARCH", "gfx950")
 
-SOURCES = ["st_kernels.hip", "st_scene.cpp", "st_context.cpp"]
+SOURCES = ["st_trace_f64.hip", "st_trace_f32.hip", "st_compact.hip", "st_scene.cpp", "st_context.cpp"]
+# The FP64 trace keeps separate multiply/add roundings (no FMA contraction),
+# the arithmetic the reference's CPU build performs.
+EXTRA = {"st_trace_f64.hip": ["-ffp-contract=off"], "st_trace_f32.hip": ["-ffp-contract=fast"]}
 
 # -packed-fp32-ops off: packed FP32 forms are excluded from every kernel.
 HIP_FLAGS = [
@@ -52,9 +55,8 @@ def build(verbose: bool = False) -> str:
     objs = []
     for src in SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-        cmd = [cc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
-        if src.endswith(".cpp"):
-            cmd = [cc, "-x", "hip", *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        lang = ["-x", "hip"] if src.endswith(".cpp") else []
+        cmd = [cc, *lang, *HIP_FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         _run(cmd)

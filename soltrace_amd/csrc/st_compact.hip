@@ -1,0 +1,176 @@
+// Scans and compaction kernels shared by both precisions.
+#include <hip/hip_runtime.h>
+
+#include "st_kernels.h"
+
+namespace st {
+
+// ------------------------------------------------------------- scans ------
+static constexpr int kScanItems = 16;
+static constexpr int kScanBlock = 256;
+static constexpr int kScanTile = kScanBlock * kScanItems;  // 4096
+
+// Block-wide exclusive scan of one value per thread (LDS, 64-wide waves).
+__device__ uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
+    __shared__ uint32_t wsum[kScanBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (int w = 0; w < kScanBlock / 64; ++w) {
+        if (w < wid) off += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + x - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_tiles(const uint32_t *in, uint32_t *out, size_t n,
+                                                         uint32_t *tile_sums) {
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0u;
+        s += v[i];
+    }
+    uint32_t tot;
+    uint32_t off = block_excl_scan(s, &tot);
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < n) out[base + i] = off;
+        off += v[i];
+    }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_sums(uint32_t *sums, size_t ntiles, uint32_t *total) {
+    // single block; ntiles <= kScanTile
+    const size_t base = (size_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        v[i] = (base + i < ntiles) ? sums[base + i] : 0u;
+        s += v[i];
+    }
+    uint32_t tot;
+    uint32_t off = block_excl_scan(s, &tot);
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < ntiles) sums[base + i] = off;
+        off += v[i];
+    }
+    if (threadIdx.x == 0) *total = tot;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_add(uint32_t *out, size_t n, const uint32_t *tile_off) {
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    const uint32_t add = tile_off[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i)
+        if (base + i < n) out[base + i] += add;
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+
+size_t scan_tmp_words(size_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+void launch_exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, uint32_t *total,
+                           hipStream_t s) {
+    const size_t ntiles = (n + kScanTile - 1) / kScanTile;
+    if (ntiles == 0) {
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, total, (size_t)1, 0u);
+        return;
+    }
+    hipLaunchKernelGGL(scan_tiles, dim3((unsigned)ntiles), dim3(kScanBlock), 0, s, in, out, n, tmp);
+    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanBlock), 0, s, tmp, ntiles, total);
+    hipLaunchKernelGGL(scan_add, dim3((unsigned)ntiles), dim3(kScanBlock), 0, s, out, n, tmp);
+}
+
+// -------------------------------------------------------- compaction ------
+__global__ void mask_popc_kernel(const uint64_t *mask, uint32_t *counts, size_t nwords) {
+    size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nwords) counts[w] = (uint32_t)__popcll(mask[w]);
+}
+void launch_mask_popc(const uint64_t *mask, uint32_t *counts, size_t nwords, hipStream_t s) {
+    hipLaunchKernelGGL(mask_popc_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s, mask, counts,
+                       nwords);
+}
+
+__global__ void mask_compact_kernel(const uint64_t *mask, const uint32_t *offsets, size_t nwords, uint64_t s0,
+                                    uint32_t limit, uint64_t *hits) {
+    size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwords) return;
+    uint64_t m = mask[w];
+    uint32_t o = offsets[w];
+    while (m != 0 && o < limit) {
+        int b = __ffsll((unsigned long long)m) - 1;
+        hits[o++] = s0 + w * 64 + (uint64_t)b;
+        m &= m - 1;
+    }
+}
+void launch_mask_compact(const uint64_t *mask, const uint32_t *offsets, size_t nwords, uint64_t s0, uint32_t limit,
+                         uint64_t *hits, hipStream_t s) {
+    hipLaunchKernelGGL(mask_compact_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s, mask, offsets,
+                       nwords, s0, limit, hits);
+}
+
+__global__ void compact_records_kernel(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets,
+                                       int n_rays, int nslots, DevRecord *out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_rays) return;
+    uint32_t c = count[t];
+    uint32_t m = c < (uint32_t)nslots ? c : (uint32_t)nslots;
+    uint32_t o = offsets[t];
+    for (uint32_t j = 0; j < m; ++j) out[o + j] = slots[(size_t)t * nslots + j];
+}
+void launch_compact_records(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets, int n_rays,
+                            int nslots, DevRecord *out, hipStream_t s) {
+    hipLaunchKernelGGL(compact_records_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, s, slots, count, offsets,
+                       n_rays, nslots, out);
+}
+
+__global__ void compact_overflow_kernel(const DevRecord *ovf, const uint32_t *meta, const uint32_t *n_ovf_dev,
+                                        uint32_t cap, const uint32_t *offsets, DevRecord *out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = *n_ovf_dev;
+    if (n > cap) n = cap;
+    if (i >= n) return;
+    uint32_t t = meta[2 * i], seq = meta[2 * i + 1];
+    out[offsets[t] + seq] = ovf[i];
+}
+void launch_compact_overflow(const DevRecord *ovf, const uint32_t *meta, const uint32_t *n_ovf_dev, uint32_t cap,
+                             const uint32_t *offsets, DevRecord *out, hipStream_t s) {
+    hipLaunchKernelGGL(compact_overflow_kernel, dim3((cap + 255) / 256), dim3(256), 0, s, ovf, meta, n_ovf_dev, cap,
+                       offsets, out);
+}
+
+__global__ void compact_rays_kernel(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, int n_rays,
+                                    DevRay *out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_rays) return;
+    if (flag[t]) out[offsets[t]] = slots[t];
+}
+void launch_compact_rays(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, int n_rays, DevRay *out,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(compact_rays_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, s, slots, flag, offsets, n_rays,
+                       out);
+}
+
+}  // namespace st

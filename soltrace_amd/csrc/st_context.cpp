@@ -75,8 +75,11 @@ struct StageRecords {
 };
 
 struct Timing {
-    double mask_ms = 0, trace_ms[kMaxStages] = {0}, other_ms = 0, total_ms = 0;
+    double mask_ms = 0, trace_ms[kMaxStages] = {0}, other_ms = 0, total_ms = 0, sunpre_ms = 0;
     uint64_t sun_rays = 0, rays_in[kMaxStages] = {0}, records[kMaxStages] = {0};
+    uint64_t mask_launches = 0, trace_launches[kMaxStages] = {0};
+    // event counters per kernel: [0] sun mask, [1 + stage] trace
+    unsigned long long ev[1 + kMaxStages][5] = {{0}};
 };
 
 struct Context {
@@ -85,16 +88,21 @@ struct Context {
     bool dyn_group = false, err_sun = true, err_opt = true;
     int precision = 64;
     bool cull = true;
+    bool count_events = false;
+    DevBuf d_counters;
     std::vector<std::string> messages;
     hipStream_t stream = nullptr;
     int device = 0;
 
     CompiledScene cs;
+    bool dirty = true;            // a setter ran since the last compile
+    bool compiled_cull = true;
+    SceneView scene_view{};
     DevBuf d_elems, d_faces, d_aux, d_stages, d_bvh, d_bvh_items, d_grid_cells, d_grid_items;
     // run scratch
     DevBuf d_mask, d_popc, d_offs, d_scan_tmp, d_total, d_hits, d_rec_slots, d_rec_count, d_rec_offs, d_ovf_rec,
         d_ovf_meta, d_ovf_count, d_out_slots, d_out_flag, d_out_offs, d_err, d_flux;
-    DevBuf d_rays_a, d_rays_b;
+    DevBuf d_rays_a, d_rays_b, d_sunpre;
 
     // results
     std::vector<StageRecords> recs;
@@ -192,7 +200,10 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     clear_results();
     timing = Timing();
     std::string msg;
-    if (!compile_scene(model, cull, cs, msg)) {
+    // The scene is compiled (geometry set-up, cull structures, device tables)
+    // only when a setter changed it since the last successful compile.
+    const bool recompile = dirty || compiled_cull != cull;
+    if (recompile && !compile_scene(model, cull, cs, msg)) {
         errlog("%s", msg.c_str());
         return -1;
     }
@@ -210,24 +221,31 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         errlog("at most %d stages are supported", (int)kMaxStages);
         return -1;
     }
-    if (!compile_sun(model, cs, msg)) {
-        errlog("%s", msg.c_str());
-        return -1;
-    }
     hipStream_t s = stream;
-    SceneView sv{};
-    sv.elems = upload(d_elems, cs.elems, s);
-    sv.faces = upload(d_faces, cs.faces, s);
-    sv.aux = upload(d_aux, cs.aux, s);
-    sv.stages = upload(d_stages, cs.stages, s);
-    sv.bvh = upload(d_bvh, cs.bvh, s);
-    sv.bvh_items = upload(d_bvh_items, cs.bvh_items, s);
-    sv.grid_cells = upload(d_grid_cells, cs.grid_cells, s);
-    sv.grid_items = upload(d_grid_items, cs.grid_items, s);
-    sv.grid = cs.grid;
-    sv.use_grid = cs.use_grid ? 1 : 0;
-    sv.nstages = (int)cs.stages.size();
-    sv.sun = cs.sun;
+    if (recompile) {
+        if (!compile_sun(model, cs, msg)) {
+            errlog("%s", msg.c_str());
+            return -1;
+        }
+        SceneView &v = scene_view;
+        v = SceneView{};
+        v.elems = upload(d_elems, cs.elems, s);
+        v.faces = upload(d_faces, cs.faces, s);
+        v.aux = upload(d_aux, cs.aux, s);
+        v.stages = upload(d_stages, cs.stages, s);
+        v.bvh = upload(d_bvh, cs.bvh, s);
+        v.bvh_items = upload(d_bvh_items, cs.bvh_items, s);
+        v.grid_cells = upload(d_grid_cells, cs.grid_cells, s);
+        v.grid_items = upload(d_grid_items, cs.grid_items, s);
+        v.grid = cs.grid;
+        v.use_grid = cs.use_grid ? 1 : 0;
+        v.nstages = (int)cs.stages.size();
+        v.sun = cs.sun;
+        HC(hipStreamSynchronize(s));
+        dirty = false;
+        compiled_cull = cull;
+    }
+    const SceneView sv = scene_view;
 
     uint32_t *err = (uint32_t *)d_err.ensure(sizeof(uint32_t));
     launch_fill_u32(err, 1, 0u, s);
@@ -244,6 +262,11 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     d_out_offs.ensure((size_t)kMaxChunk * sizeof(uint32_t));
     d_scan_tmp.ensure(scan_tmp_words((size_t)1 << 26) * sizeof(uint32_t));
 
+    unsigned long long *ctrs = nullptr;
+    if (count_events) {
+        ctrs = (unsigned long long *)d_counters.ensure((1 + kMaxStages) * 5 * sizeof(unsigned long long));
+        launch_fill_u32((uint32_t *)ctrs, (1 + kMaxStages) * 10, 0u, s);
+    }
     FluxSpec fs = flux;
     if (fs.active) {
         d_flux.ensure((size_t)fs.nx * fs.ny * sizeof(uint32_t));
@@ -282,6 +305,10 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             errlog("internal error: record overflow buffer full");
             return false;
         }
+        if (e & 8u) {
+            errlog("sunshape table sampling found no accepted angle in 2^26 tries");
+            return false;
+        }
         return true;
     };
 
@@ -317,6 +344,23 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         a.err_flags = err;
         a.flux = fs;
         a.flux_bins = fs.active ? d_flux.as<uint32_t>() : nullptr;
+        a.counters = ctrs ? ctrs + 5 * (1 + si) : nullptr;
+        a.sun_pre = nullptr;
+        if (si == 0 && err_sun && (sv.sun.shape == 'd' || sv.sun.shape == 'D') && !sv.sun.point_source) {
+            // tabulated sunshape: draw the polar angles cooperatively first
+            SunPreArgs pa{};
+            pa.sun = sv.sun;
+            pa.aux = sv.aux;
+            pa.seed = seed;
+            pa.k0 = 3;  // two sun-plane draws and the reflectivity draw come first
+            pa.sun_index = sun_idx;
+            pa.n = n;
+            pa.out = (SunPre *)d_sunpre.ensure((size_t)kMaxChunk * sizeof(SunPre));
+            pa.err_flags = err;
+            timed(timing.sunpre_ms, [&]() { launch_sunshape_d<R>(pa, s); });
+            a.sun_pre = pa.out;
+        }
+        timing.trace_launches[si]++;
         launch_fill_u32(ovf_count, 1, 0u, s);
         timed(timing.trace_ms[si], [&]() { launch_trace_stage<R>(a, s); });
         timing.rays_in[si] += (uint64_t)n;
@@ -374,6 +418,8 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         ma.s0 = s0;
         ma.n = (uint32_t)batch;
         ma.mask = mask;
+        ma.counters = ctrs;
+        timing.mask_launches++;
         timed(timing.mask_ms, [&]() { launch_sun_mask<R>(ma, s); });
         timing.sun_rays += batch;
         launch_mask_popc(mask, popc, nwords, s);
@@ -455,6 +501,10 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
                           s));
         HC(hipStreamSynchronize(s));
     }
+    if (ctrs) {
+        HC(hipMemcpyAsync(&timing.ev[0][0], ctrs, sizeof(timing.ev), hipMemcpyDeviceToHost, s));
+        HC(hipStreamSynchronize(s));
+    }
     timing.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return (int)total_records();
 }
@@ -466,6 +516,10 @@ using namespace st;
 #define CTX(p, r)                       \
     Context *c = ctx_of(p);             \
     if (!c) return r;
+// setters also mark the compiled scene stale
+#define CTXW(p, r)                      \
+    CTX(p, r)                           \
+    c->dirty = true;
 #define STAGE(i)                                        \
     if ((i) >= c->model.stages.size()) return -1;      \
     HStage &stg = c->model.stages[i];
@@ -506,19 +560,19 @@ STCORE_API int st_num_optics(st_context_t p) {
     return (int)c->model.optics.size();
 }
 STCORE_API int st_add_optic(st_context_t p, const char *name) {
-    CTX(p, -1);
+    CTXW(p, -1);
     c->model.optics.emplace_back();
     c->model.optics.back().name = name ? name : "";
     return (int)c->model.optics.size() - 1;
 }
 STCORE_API int st_delete_optic(st_context_t p, st_uint_t idx) {
-    CTX(p, -1);
+    CTXW(p, -1);
     if (idx >= c->model.optics.size()) return -1;
     c->model.optics.erase(c->model.optics.begin() + idx);
     return 1;
 }
 STCORE_API int st_clear_optics(st_context_t p) {
-    CTX(p, -1);
+    CTXW(p, -1);
     c->model.optics.clear();
     return 1;
 }
@@ -526,7 +580,7 @@ STCORE_API int st_optic(st_context_t p, st_uint_t idx, int fb, char dist, int op
                         double rreal, double rimag, double ref, double tra, double gratingab12[3], double rmsslope,
                         double rmsspec, int userefltable, int refl_npoints, double *refl_angles, double *refls,
                         int usetranstable, int trans_npoints, double *trans_angles, double *transs) {
-    CTX(p, -1);
+    CTXW(p, -1);
     if (idx >= c->model.optics.size()) return -1;
     HFace &f = (fb == 2) ? c->model.optics[idx].back : c->model.optics[idx].front;
     f.dist = dist;
@@ -560,46 +614,46 @@ STCORE_API int st_num_stages(st_context_t p) {
     return (int)c->model.stages.size();
 }
 STCORE_API int st_add_stage(st_context_t p) {
-    CTX(p, -1);
+    CTXW(p, -1);
     c->model.stages.emplace_back();
     return (int)c->model.stages.size() - 1;
 }
 STCORE_API int st_add_stages(st_context_t p, st_uint_t num) {
-    CTX(p, -1);
+    CTXW(p, -1);
     for (st_uint_t i = 0; i < num; i++) c->model.stages.emplace_back();
     return (int)num;
 }
 STCORE_API int st_delete_stage(st_context_t p, st_uint_t idx) {
-    CTX(p, -1);
+    CTXW(p, -1);
     if (idx >= c->model.stages.size()) return -1;
     c->model.stages.erase(c->model.stages.begin() + idx);
     return 1;
 }
 STCORE_API int st_clear_stages(st_context_t p) {
-    CTX(p, -1);
+    CTXW(p, -1);
     c->model.stages.clear();
     return 1;
 }
 STCORE_API int st_stage_xyz(st_context_t p, st_uint_t i, double x, double y, double z) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     stg.origin[0] = x; stg.origin[1] = y; stg.origin[2] = z;
     return 1;
 }
 STCORE_API int st_stage_aim(st_context_t p, st_uint_t i, double x, double y, double z) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     stg.aim[0] = x; stg.aim[1] = y; stg.aim[2] = z;
     return 1;
 }
 STCORE_API int st_stage_zrot(st_context_t p, st_uint_t i, double z) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     stg.zrot = z;
     return 1;
 }
 STCORE_API int st_stage_flags(st_context_t p, st_uint_t i, int virt, int multihit, int tracethrough) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     stg.virt = virt != 0;
     stg.multihit = multihit != 0;
@@ -614,82 +668,82 @@ STCORE_API int st_num_elements(st_context_t p, st_uint_t i) {
     return (int)stg.elements.size();
 }
 STCORE_API int st_add_element(st_context_t p, st_uint_t i) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     stg.elements.emplace_back();
     return (int)stg.elements.size() - 1;
 }
 STCORE_API int st_add_elements(st_context_t p, st_uint_t i, st_uint_t num) {
-    CTX(p, -1);
+    CTXW(p, -1);
     if (num < 1) return -1;
     STAGE(i);
     stg.elements.resize(stg.elements.size() + num);
     return (int)num;
 }
 STCORE_API int st_delete_element(st_context_t p, st_uint_t i, st_uint_t idx) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     if (idx >= stg.elements.size()) return -1;
     stg.elements.erase(stg.elements.begin() + idx);
     return 1;
 }
 STCORE_API int st_clear_elements(st_context_t p, st_uint_t i) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     stg.elements.clear();
     return 1;
 }
 STCORE_API int st_element_enabled(st_context_t p, st_uint_t i, st_uint_t j, int en) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     ELEM(j);
     el.enabled = en != 0;
     return 1;
 }
 STCORE_API int st_element_xyz(st_context_t p, st_uint_t i, st_uint_t j, double x, double y, double z) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     ELEM(j);
     el.origin[0] = x; el.origin[1] = y; el.origin[2] = z;
     return 1;
 }
 STCORE_API int st_element_aim(st_context_t p, st_uint_t i, st_uint_t j, double x, double y, double z) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     ELEM(j);
     el.aim[0] = x; el.aim[1] = y; el.aim[2] = z;
     return 1;
 }
 STCORE_API int st_element_zrot(st_context_t p, st_uint_t i, st_uint_t j, double z) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     ELEM(j);
     el.zrot = z;
     return 1;
 }
 STCORE_API int st_element_aperture(st_context_t p, st_uint_t i, st_uint_t j, char ap) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     ELEM(j);
     el.shape = ap;
     return 1;
 }
 STCORE_API int st_element_aperture_params(st_context_t p, st_uint_t i, st_uint_t j, double params[8]) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     ELEM(j);
     for (int k = 0; k < 8; k++) el.p[k] = params[k];
     return 1;
 }
 STCORE_API int st_element_surface(st_context_t p, st_uint_t i, st_uint_t j, char surf) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     ELEM(j);
     el.surf = surf;
     return 1;
 }
 STCORE_API int st_element_surface_params(st_context_t p, st_uint_t i, st_uint_t j, double params[8]) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     ELEM(j);
     std::string msg;
@@ -700,7 +754,7 @@ STCORE_API int st_element_surface_params(st_context_t p, st_uint_t i, st_uint_t 
     return 1;
 }
 STCORE_API int st_element_surface_file(st_context_t p, st_uint_t i, st_uint_t j, const char *file) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     ELEM(j);
     std::string msg;
@@ -711,14 +765,14 @@ STCORE_API int st_element_surface_file(st_context_t p, st_uint_t i, st_uint_t j,
     return 1;
 }
 STCORE_API int st_element_interaction(st_context_t p, st_uint_t i, st_uint_t j, int type) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     ELEM(j);
     el.interaction = type;
     return 1;
 }
 STCORE_API int st_element_optic(st_context_t p, st_uint_t i, st_uint_t j, const char *name) {
-    CTX(p, -1);
+    CTXW(p, -1);
     STAGE(i);
     ELEM(j);
     el.optic_name = name ? name : "";
@@ -727,14 +781,14 @@ STCORE_API int st_element_optic(st_context_t p, st_uint_t i, st_uint_t j, const 
 
 // ------------------------------------------------------------------ sun ---
 STCORE_API int st_sun(st_context_t p, int point_source, char shape, double sigma) {
-    CTX(p, -1);
+    CTXW(p, -1);
     c->model.sun.point_source = point_source != 0;
     c->model.sun.shape = shape;
     c->model.sun.sigma = sigma;
     return 1;
 }
 STCORE_API int st_sun_xyz(st_context_t p, double x, double y, double z) {
-    CTX(p, -1);
+    CTXW(p, -1);
     c->model.sun.origin[0] = x; c->model.sun.origin[1] = y; c->model.sun.origin[2] = z;
     return 1;
 }
@@ -758,7 +812,7 @@ STCORE_API int st_sun_position(st_context_t p, double lat, double day, double ho
 }
 // coretrace/stapi.cpp:529-560 (table mirrored about zero)
 STCORE_API int st_sun_userdata(st_context_t p, st_uint_t n, double angle[], double intensity[]) {
-    CTX(p, -1);
+    CTXW(p, -1);
     if (n <= 2) return -1;
     HSun &s = c->model.sun;
     s.max_angle = 0;
@@ -919,7 +973,7 @@ STCORE_API int st_amd_set_precision(st_context_t p, int bits) {
     return 1;
 }
 STCORE_API int st_amd_set_cull(st_context_t p, int on) {
-    CTX(p, -1);
+    CTXW(p, -1);
     c->cull = on != 0;
     return 1;
 }
@@ -997,6 +1051,25 @@ STCORE_API int st_amd_ray_sun_index(st_context_t p, unsigned long long *out, int
     }
     return (int)m;
 }
+STCORE_API int st_amd_count_events(st_context_t p, int on) {
+    CTX(p, -1);
+    c->count_events = on != 0;
+    return 1;
+}
+// Launch counts and event counters of the last run: [0] sun-mask launches,
+// [1..16] trace launches per stage, then 5 counters for the sun mask and 5
+// per stage (rays, element tests, BVH nodes, uniforms drawn, records).
+STCORE_API int st_amd_events(st_context_t p, double *out, int n) {
+    CTX(p, -1);
+    std::vector<double> v;
+    v.push_back((double)c->timing.mask_launches);
+    for (int i = 0; i < kMaxStages; i++) v.push_back((double)c->timing.trace_launches[i]);
+    for (int k = 0; k < 1 + kMaxStages; k++)
+        for (int j = 0; j < 5; j++) v.push_back((double)c->timing.ev[k][j]);
+    int m = std::min<int>(n, (int)v.size());
+    for (int i = 0; i < m; i++) out[i] = v[i];
+    return (int)v.size();
+}
 // Timing of the last run: [0] mask ms, [1..kMaxStages] trace ms per stage,
 // [17] other ms, [18] total ms, [19] sun rays, [20..] rays per stage,
 // [36..] records per stage.
@@ -1010,6 +1083,7 @@ STCORE_API int st_amd_timing(st_context_t p, double *out, int n) {
     v.push_back((double)c->timing.sun_rays);
     for (int i = 0; i < kMaxStages; i++) v.push_back((double)c->timing.rays_in[i]);
     for (int i = 0; i < kMaxStages; i++) v.push_back((double)c->timing.records[i]);
+    v.push_back(c->timing.sunpre_ms);
     int m = std::min<int>(n, (int)v.size());
     for (int i = 0; i < m; i++) out[i] = v[i];
     return (int)v.size();

@@ -32,8 +32,8 @@ template <> struct M<double> {
 };
 template <> struct M<float> {
     static STD float sqrt_(float x) { return ::sqrtf(x); }
-    static STD float sin_(float x) { return ::__sinf(x); }
-    static STD float cos_(float x) { return ::__cosf(x); }
+    static STD float sin_(float x) { return ::sinf(x); }
+    static STD float cos_(float x) { return ::cosf(x); }
     static STD float atan2_(float y, float x) { return ::atan2f(y, x); }
     static STD float asin_(float x) { return ::asinf(x); }
     static STD float acos_(float x) { return ::acosf(x); }
@@ -43,10 +43,10 @@ template <> struct M<float> {
 };
 
 // ------------------------------------------------------------------ RNG ---
-// Philox4x32-10 (Salmon et al., SC'11).  Uniform k of the stream
-// (seed, stage, id) comes from the block with counter (id_lo, id_hi, stage,
-// k/2) and key (seed, 0x53544F52); words 2(k%2), 2(k%2)+1 make a 64-bit w,
-// u = (w >> 11) 2^-53 in double, (w >> 40) 2^-24 in float.
+// Philox4x32-10 (Salmon et al., SC'11).  Draw k of the stream (seed, stage,
+// id) is word k%4 of the block with counter (id_lo, id_hi, stage, k/4) and
+// key (seed, 0x53544F52): u = w 2^-32 in double, (w >> 8) 2^-24 in float
+// (the same number at the two resolutions).
 STD void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
@@ -62,11 +62,15 @@ STD void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
     }
 }
 
+STD uint32_t pick4(const uint32_t b[4], uint32_t i) {
+    return i == 0 ? b[0] : (i == 1 ? b[1] : (i == 2 ? b[2] : b[3]));
+}
+
 template <typename R>
 struct Rng {
     uint32_t seed, stage, id_lo, id_hi;
     uint32_t k;
-    uint32_t spare_lo, spare_hi;
+    uint32_t blk[4];
     bool have_saved;
     R saved;
 
@@ -74,22 +78,24 @@ struct Rng {
         seed = seed_; stage = stage_; id_lo = (uint32_t)id; id_hi = (uint32_t)(id >> 32);
         k = 0; have_saved = false; saved = R(0);
     }
-    STD uint64_t next_word() {
-        uint64_t w;
-        if (k & 1u) {
-            w = ((uint64_t)spare_hi << 32) | spare_lo;
-        } else {
-            uint32_t c[4] = {id_lo, id_hi, stage, k >> 1};
-            philox10(c, seed, 0x53544F52u);
-            w = ((uint64_t)c[1] << 32) | c[0];
-            spare_lo = c[2]; spare_hi = c[3];
-        }
+    STD void fill(uint32_t block) {
+        blk[0] = id_lo; blk[1] = id_hi; blk[2] = stage; blk[3] = block;
+        philox10(blk, seed, 0x53544F52u);
+    }
+    STD uint32_t next_word() {
+        if ((k & 3u) == 0u) fill(k >> 2);
+        uint32_t w = pick4(blk, k & 3u);
         ++k;
         return w;
     }
+    // Continue the stream at draw index kk (the draws before it were consumed elsewhere).
+    STD void seek(uint32_t kk) {
+        k = kk;
+        if (k & 3u) fill(k >> 2);
+    }
     STD R uniform();
-    // full 53-bit draw regardless of R (sun-plane positions span kilometres)
-    STD double uniform_d() { return (double)(next_word() >> 11) * (1.0 / 9007199254740992.0); }
+    // double draw regardless of R (sun-plane positions span kilometres)
+    STD double uniform_d() { return (double)next_word() * 2.3283064365386963e-10; }
     // Marsaglia polar with one cached deviate, the algorithm of libstdc++'s
     // normal_distribution (bits/random.tcc:1802-1835): y*m first, x*m cached.
     STD R normal(R mean, R sd) {
@@ -112,12 +118,12 @@ struct Rng {
         return mean + ret * sd;
     }
 };
-template <> STD double Rng<double>::uniform() {
-    return (double)(next_word() >> 11) * (1.0 / 9007199254740992.0);
-}
-template <> STD float Rng<float>::uniform() {
-    return (float)(uint32_t)(next_word() >> 40) * (1.0f / 16777216.0f);
-}
+template <> STD double Rng<double>::uniform() { return (double)next_word() * 2.3283064365386963e-10; }
+template <> STD float Rng<float>::uniform() { return (float)(next_word() >> 8) * (1.0f / 16777216.0f); }
+
+template <typename R> STD R word_to_uniform(uint32_t w);
+template <> STD double word_to_uniform<double>(uint32_t w) { return (double)w * 2.3283064365386963e-10; }
+template <> STD float word_to_uniform<float>(uint32_t w) { return (float)(w >> 8) * (1.0f / 16777216.0f); }
 
 // ------------------------------------------------------------ transforms --
 STD void mat_vec(const double *Mx, const double v[3], double o[3]) {
@@ -291,6 +297,17 @@ STD bool quadric(const DevElem &e, const R P[3], const R D[3], R X[3], R DF[3], 
         break;
     }
     }
+    if constexpr (sizeof(R) == 4) {
+        // FP32: (z - zc)^2 - zc^2 cancels catastrophically when the centre
+        // is far from the vertex (heliostat spheres have R ~ 1 km); use the
+        // expanded forms, which are algebraically equal.
+        if (e.surf == SURF_SPHERE || e.surf == SURF_CYLINDER) {
+            R r2x = Zc + Zc;
+            C = Kx * P[0] * P[0] + Ky * P[1] * P[1] + P[2] * (P[2] - r2x);
+        } else if (e.surf == SURF_CONIC) {
+            C = P[0] * P[0] + P[1] * P[1] + Kz * P[2] * P[2] - R(2) * P[2] / (R)e.vcx;
+        }
+    }
     R t;
     if (M<R>::fabs_(A) < R(1e-12)) {
         t = -C / B;
@@ -299,8 +316,16 @@ STD bool quadric(const DevElem &e, const R P[3], const R D[3], R X[3], R DF[3], 
         R disc = (B * B) - R(4) * A * C;
         if (!((B * B) > R(4) * A * C)) return false;
         R sq = M<R>::sqrt_(disc);
-        R t1 = (-B + sq) / (R(2) * A);
-        R t2 = (-B - sq) / (R(2) * A);
+        R t1, t2;
+        if constexpr (sizeof(R) == 4) {
+            // roots without cancellation: t1 t2 = C / A
+            R q = R(-0.5) * (B + (B >= R(0) ? sq : -sq));
+            if (B >= R(0)) { t2 = q / A; t1 = C / q; }
+            else { t1 = q / A; t2 = C / q; }
+        } else {
+            t1 = (-B + sq) / (R(2) * A);
+            t2 = (-B - sq) / (R(2) * A);
+        }
         if (t2 > R(0)) {
             t = t2;
             R z2 = P[2] + t2 * D[2];
@@ -531,10 +556,11 @@ template <typename R>
 STD void polyrev(const DevElem &e, const double *aux, R x, R y, R *z, R *dzdx, R *dzdy) {
     const double *c = aux + e.aux;
     R r = M<R>::sqrt_(x * x + y * y);
-    R zz = 0, dr = 0, rp = 1;
+    R zz = 0, dr = 0, rp = 1, rpm1 = 0;  // rp = r^i, rpm1 = r^(i-1)
     for (int i = 0; i <= e.fit_order; i++) {
-        if (i >= 1) dr = dr + R(i) * (R)c[i] * rp;
+        if (i >= 1) dr = dr + R(i) * (R)c[i] * rpm1;
         zz = zz + (R)c[i] * rp;
+        rpm1 = rp;
         rp = rp * r;
     }
     *z = zz;
@@ -757,8 +783,23 @@ STD bool element_hit(const DevElem &e, const double *aux, const double P[3], con
     double path;
     bool ok;
     if (s == SURF_TORUS) {
-        double Xd[3], DFd[3];
-        ok = torus(e, P, D, Xd, DFd, &path);
+        // The quartic in the ray parameter is ill-conditioned for an origin
+        // kilometres away (a sun ray starts 10 km back); start it at the
+        // bounding sphere instead.  All real intersections lie inside the
+        // sphere, so their order and signs -- and the root choice -- are
+        // unchanged.
+        double t0 = 0.0;
+        {
+            double d[3] = {P[0] - e.bsphere_c[0], P[1] - e.bsphere_c[1], P[2] - e.bsphere_c[2]};
+            double tc = -(d[0] * D[0] + d[1] * D[1] + d[2] * D[2]);
+            double dist2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            double rr = 2.0 * e.bsphere_r + 1.0;
+            if (dist2 > rr * rr && tc > rr) t0 = tc - rr;
+        }
+        const double Ps[3] = {P[0] + t0 * D[0], P[1] + t0 * D[1], P[2] + t0 * D[2]};
+        double Xd[3], DFd[3], L;
+        ok = torus(e, t0 != 0.0 ? Ps : P, D, Xd, DFd, &L);
+        path = t0 + L;
         for (int i = 0; i < 3; i++) { X[i] = (R)Xd[i]; DF[i] = (R)DFd[i]; }
     } else if (s <= SURF_CYLINDER) {
         // closed-form quadrics; a float ray starting far away is first moved
@@ -907,22 +948,9 @@ STD void errors(Rng<R> &rng, const R cin[3], int source, const DevSun &sun, cons
                 theta2 = tx * tx + ty * ty;
             } while (theta2 > delop * delop);
         } else if (dist == 'd' || dist == 'D') {
-            const int n = sun.tab_n;
-            const double *ang = aux + sun.tab, *inten = ang + n;
-            const bool asc = sun.tab_ascending != 0;
-            const R ma = (R)sun.max_angle, mi = (R)sun.max_intensity;
             for (;;) {
-                R tx = R(2) * ma * rng.uniform() - ma;
-                R ty = R(2) * ma * rng.uniform() - ma;
-                theta2 = tx * tx + ty * ty;
-                R th = M<R>::sqrt_(theta2);
-                int i = table_index<R>(ang, n, th, asc);
-                R stest;
-                if (i == 0) stest = (R)inten[0];
-                else stest = (R)inten[i - 1] + ((R)inten[i] - (R)inten[i - 1]) * (th - (R)ang[i - 1]) / ((R)ang[i] - (R)ang[i - 1]);
-                if (rng.uniform() > stest / mi) continue;
-                if (theta2 > ma * ma) continue;
-                break;
+                R u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+                if (sunshape_d_try<R>(u0, u1, u2, sun, aux, &theta2)) break;
             }
             theta2 = theta2 / R(1000000.0);
         } else if (dist == 'f' || dist == 'F') {
@@ -934,6 +962,74 @@ STD void errors(Rng<R> &rng, const R cin[3], int source, const DevSun &sun, cons
         if (source == 2 && interaction == 2 && dot3<R>(cout, DF) < R(0) && maxcall++ < 50000u) continue;
         break;
     }
+}
+
+// Sunshape error when the polar angle was drawn ahead of time (the 'd'
+// rejection loop run cooperatively, see sunshape_d_kernel): frame, phi draw
+// and rotation exactly as errors() does after its loop.
+template <typename R>
+STD void sun_error_pre(Rng<R> &rng, const R cin[3], R theta2, R cout[3]) {
+    R RT[9];
+    ray_frame<R>(cin, RT);
+    R phi = rng.uniform() * R(2) * (R)3.1415926535897932385;
+    cone_dir<R>(theta2, phi, RT, cout);
+}
+
+// One try of the 'd' sunshape rejection loop of coretrace/errors.cpp:160-185
+// from its three uniforms.  A precomputed upper envelope of the profile
+// (per polar-angle bin, padded for rounding) rejects most tries without the
+// table walk; it only ever rejects what the exact test rejects.
+template <typename R>
+STD bool sunshape_d_try(R u0, R u1, R u2, const DevSun &sun, const double *aux, R *t2out) {
+    const R ma = (R)sun.max_angle, mi = (R)sun.max_intensity;
+    R tx = R(2) * ma * u0 - ma;
+    R ty = R(2) * ma * u1 - ma;
+    R t2 = tx * tx + ty * ty;
+    *t2out = t2;
+    R th = M<R>::sqrt_(t2);
+    if (sun.env_n > 0 && !(t2 > ma * ma)) {
+        int b = (int)(th * (R)sun.env_scale);
+        b = b < 0 ? 0 : (b >= sun.env_n ? sun.env_n - 1 : b);
+        if (u2 > (R)aux[sun.env + b]) return false;
+    }
+    const int n = sun.tab_n;
+    const double *ang = aux + sun.tab, *inten = ang + n;
+    int i = table_index<R>(ang, n, th, sun.tab_ascending != 0);
+    R stest;
+    if (i == 0) stest = (R)inten[0];
+    else stest = (R)inten[i - 1] + ((R)inten[i] - (R)inten[i - 1]) * (th - (R)ang[i - 1]) / ((R)ang[i] - (R)ang[i - 1]);
+    if (u2 > stest / mi) return false;
+    if (t2 > ma * ma) return false;
+    return true;
+}
+
+// Tries 4q .. 4q+3 of the 'd' loop for the stream (seed, stage 0, sun_index)
+// whose loop starts at draw k0 (try j uses draws k0+3j .. k0+3j+2).  Returns
+// the first accepted try or -1; *theta2 is its (mrad^2) sample.
+template <typename R>
+STD int sunshape_d_group(uint32_t seed, uint64_t sun_index, uint32_t k0, uint32_t q, const DevSun &sun,
+                         const double *aux, R *theta2) {
+    const uint32_t kfirst = k0 + 12u * q;
+    uint32_t cur = 0xffffffffu;
+    uint32_t b[4];
+    auto word = [&](uint32_t k) {
+        if ((k >> 2) != cur) {
+            cur = k >> 2;
+            b[0] = (uint32_t)sun_index; b[1] = (uint32_t)(sun_index >> 32); b[2] = 0u; b[3] = cur;
+            philox10(b, seed, 0x53544F52u);
+        }
+        return pick4(b, k & 3u);
+    };
+    for (int t = 0; t < 4; ++t) {
+        const uint32_t k = kfirst + 3u * (uint32_t)t;
+        R u0 = word_to_uniform<R>(word(k)), u1 = word_to_uniform<R>(word(k + 1)), u2 = word_to_uniform<R>(word(k + 2));
+        R t2;
+        if (sunshape_d_try<R>(u0, u1, u2, sun, aux, &t2)) {
+            *theta2 = t2;
+            return t;
+        }
+    }
+    return -1;
 }
 
 // Slope error of the surface normal (coretrace/surfacenormalerrors.cpp:58-167).

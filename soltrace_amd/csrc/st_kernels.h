@@ -24,8 +24,27 @@ struct SceneView {
     DevSun sun;
 };
 
+// Sunshape polar angle of one stage-1 ray drawn ahead of the trace.
+struct SunPre {
+    double theta2;     // accepted sample scaled to rad^2
+    uint32_t kend;     // stream index of the phi draw that follows
+    uint32_t ok;
+};
+
+struct SunPreArgs {
+    DevSun sun;
+    const double *aux;
+    uint32_t seed;
+    uint32_t k0;                 // first draw of the rejection loop
+    const uint64_t *sun_index;
+    int32_t n;
+    SunPre *out;
+    uint32_t *err_flags;
+};
+
 // Arguments of one stage-trace launch.
 struct TraceArgs {
+    const SunPre *sun_pre;       // per ray, or nullptr
     SceneView sv;
     int32_t si;                  // 0-based stage
     uint32_t seed;
@@ -50,6 +69,9 @@ struct TraceArgs {
     uint32_t *err_flags;         // bit 0: bad interaction, 1: phase mismatch, 2: overflow full
     FluxSpec flux;
     uint32_t *flux_bins;
+    // event counters (nullptr: off): rays, element tests, BVH nodes,
+    // uniforms drawn, records
+    unsigned long long *counters;
 };
 
 // Arguments of the sun-ray hit-mask launch.
@@ -59,9 +81,11 @@ struct MaskArgs {
     uint64_t s0;                 // first sun-ray index
     uint32_t n;                  // sun rays in this launch
     uint64_t *mask;              // one word per 64 sun rays
+    unsigned long long *counters;  // as TraceArgs::counters (nullptr: off)
 };
 
 template <typename R> void launch_sun_mask(const MaskArgs &a, hipStream_t s);
+template <typename R> void launch_sunshape_d(const SunPreArgs &a, hipStream_t s);
 template <typename R> void launch_trace_stage(const TraceArgs &a, hipStream_t s);
 
 // Exclusive scan of n uint32 values into out (out may alias in); total via

@@ -493,9 +493,9 @@ static int build_bvh(std::vector<BuildItem> &items, int begin, int end, std::vec
     for (int k = 0; k < 3; k++) { nd.lo[k] = lo[k]; nd.hi[k] = hi[k]; }
     const int n = end - begin;
     if (n <= 4) {
-        nd.left = -((int)out_items.size() + 1);
-        nd.right = n;
+        nd.leaf = ((int)out_items.size() << 3) | n;
         for (int i = begin; i < end; i++) out_items.push_back(items[i].idx);
+        nd.skip = me + 1;
         nodes[me] = nd;
         return me;
     }
@@ -506,10 +506,10 @@ static int build_bvh(std::vector<BuildItem> &items, int begin, int end, std::vec
     int mid = begin + n / 2;
     std::nth_element(items.begin() + begin, items.begin() + mid, items.begin() + end,
                      [ax](const BuildItem &a, const BuildItem &b) { return a.c[ax] < b.c[ax]; });
-    int l = build_bvh(items, begin, mid, nodes, out_items);
-    int r = build_bvh(items, mid, end, nodes, out_items);
-    nd.left = l;
-    nd.right = r;
+    build_bvh(items, begin, mid, nodes, out_items);   // left child = me + 1
+    build_bvh(items, mid, end, nodes, out_items);
+    nd.leaf = 0;
+    nd.skip = (int)nodes.size();                      // first node after this subtree
     nodes[me] = nd;
     return me;
 }
@@ -823,6 +823,33 @@ bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
         ds.tab_ascending = 1;
         for (int k = 1; k < ds.tab_n; k++)
             if (!(sun.angle[k] > sun.angle[k - 1])) ds.tab_ascending = 0;
+        // Upper envelope of the interpolated profile / max intensity per
+        // polar-angle bin over [0, max angle], padded for rounding, so a
+        // uniform above it rejects without the table walk.
+        if (ds.tab_ascending && sun.max_angle > 0 && sun.max_intensity > 0) {
+            const int NB = 256;
+            const double ma = sun.max_angle, dth = ma / NB;
+            auto prof = [&](double th) {  // the interpolation of coretrace/errors.cpp:167-178
+                int i = 0;
+                while (i < ds.tab_n - 1 && sun.angle[i] < th) i++;
+                if (i == 0) return sun.intensity[0];
+                return sun.intensity[i - 1] + (sun.intensity[i] - sun.intensity[i - 1]) * (th - sun.angle[i - 1]) /
+                                                  (sun.angle[i] - sun.angle[i - 1]);
+            };
+            ds.env = (int)out.aux.size();
+            ds.env_n = NB;
+            ds.env_scale = NB / ma;
+            for (int b = 0; b < NB; b++) {
+                double lo = std::max(0.0, b * dth - 1e-4 * dth), hi = (b + 1) * dth + 1e-4 * dth;
+                double m = std::max(prof(lo), prof(hi));
+                for (int k = 0; k < ds.tab_n; k++)
+                    if (sun.angle[k] > lo && sun.angle[k] < hi) m = std::max(m, sun.intensity[k]);
+                double v = m / sun.max_intensity;
+                out.aux.push_back(v * (1.0 + 1e-5) + 1e-7);
+            }
+            // bins past the last one (theta up to sqrt(2) * max angle) fall
+            // to the exact test
+        }
     }
 
     // sun-plane grid of stage-1 element footprints (bounding spheres)

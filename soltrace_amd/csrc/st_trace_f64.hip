@@ -1,0 +1,9 @@
+// Compiled with -ffp-contract=off so the FP64 path keeps the reference's
+// separate multiply and add roundings (coretrace is built without FMA).
+#include "st_trace_impl.h"
+
+namespace st {
+template void launch_sun_mask<double>(const MaskArgs &, hipStream_t);
+template void launch_sunshape_d<double>(const SunPreArgs &, hipStream_t);
+template void launch_trace_stage<double>(const TraceArgs &, hipStream_t);
+}  // namespace st

@@ -1,4 +1,5 @@
-// Trace kernels for CDNA4 (gfx950).
+// Trace kernels for CDNA4 (gfx950): templates, instantiated for FP64 in
+// st_trace_f64.hip and for FP32 in st_trace_f32.hip.
 //
 // Stage 0 runs in two launches per batch of sun rays:
 //   1. sun_mask   -- one lane per sun-ray index s: draw the two sun-plane
@@ -11,6 +12,7 @@
 // in ray-number order.  Records go to per-lane slots and are compacted by an
 // exclusive scan, so the record stream is deterministic and in the
 // reference's per-stage order (ray number, then hit order).
+#pragma once
 #include <hip/hip_runtime.h>
 
 #include "st_device.h"
@@ -52,6 +54,8 @@ struct Best {
     int j;               // element index within the stage
     ElemHit<R> h;
     double ce[3];        // element-frame incoming direction
+    uint32_t tests;      // element tests (event counter)
+    uint32_t nodes;      // BVH nodes visited (event counter)
 };
 
 // Test element j; keep it when it is the closest accepted hit so far
@@ -62,6 +66,7 @@ STD void consider(const SceneView &sv, const DevStage &st, int j, const double p
                   Best<R> &b) {
     const DevElem &e = sv.elems[st.elem0 + j];
     if (!e.enabled) return;
+    b.tests++;
     double pe[3], ce[3];
     to_local(ps, cs, e.org, e.R, pe, ce);
     pe[0] = pe[0] + 1.0e-5 * ce[0];
@@ -107,27 +112,31 @@ STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3]
             float d = (float)cs[i];
             inv[i] = (d == 0.0f) ? 3.0e38f : 1.0f / d;
         }
-        int stack[48];
-        int sp = 0;
-        stack[sp++] = 0;
-        while (sp > 0) {
-            const DevBvhNode nd = nodes[stack[--sp]];
+        // stackless depth-first walk: descend on a box hit, else jump to the
+        // node's skip index (no per-lane stack, so no scratch memory)
+        int i = 0;
+        while (i < st.nbvh) {
+            const DevBvhNode nd = nodes[i];
+            b.nodes++;
             float t0 = 0.0f, t1 = 3.0e38f;
-            for (int i = 0; i < 3; i++) {
-                float ta = (nd.lo[i] - o[i]) * inv[i];
-                float tb = (nd.hi[i] - o[i]) * inv[i];
-                float lo = fminf(ta, tb), hi = fmaxf(ta, tb);
-                t0 = fmaxf(t0, lo);
-                t1 = fminf(t1, hi);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                float ta = (nd.lo[k] - o[k]) * inv[k];
+                float tb = (nd.hi[k] - o[k]) * inv[k];
+                t0 = fmaxf(t0, fminf(ta, tb));
+                t1 = fminf(t1, fmaxf(ta, tb));
             }
-            if (t0 > t1) continue;
-            if ((double)t0 > b.path * 1.000001 + 1.0e-3) continue;
-            if (nd.left < 0) {
-                int first = -nd.left - 1;
-                for (int k = 0; k < nd.right; ++k) consider<R>(sv, st, sv.bvh_items[first + k], ps, cs, b);
-            } else if (sp < 46) {
-                stack[sp++] = nd.right;
-                stack[sp++] = nd.left;
+            const bool enter = (t0 <= t1) && !((double)t0 > b.path * 1.000001 + 1.0e-3);
+            if (!enter) {
+                i = nd.skip;
+                continue;
+            }
+            if (nd.leaf) {
+                const int first = nd.leaf >> 3, cnt = nd.leaf & 7;
+                for (int k = 0; k < cnt; ++k) consider<R>(sv, st, sv.bvh_items[first + k], ps, cs, b);
+                i = nd.skip;
+            } else {
+                i = i + 1;
             }
         }
         return;
@@ -138,9 +147,12 @@ STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3]
 // Any accepted hit at all (the first-hit test of a sun ray).
 template <typename R>
 STD bool any_hit(const SceneView &sv, const DevStage &st, const double ps[3], const double cs[3],
-                 const double *psun) {
+                 const double *psun, uint32_t *tests) {
     Best<R> b;
+    b.tests = 0;
+    b.nodes = 0;
     closest_hit<R>(sv, st, ps, cs, psun, b);
+    *tests += b.tests;
     return b.path < 1e99;
 }
 
@@ -156,13 +168,95 @@ __global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
         double pg[3], cg[3], psun[2], ps[3], cs[3];
         gen_sun_ray<R>(rng, a.sv.sun, st, pg, cg, psun);
         to_local(pg, cg, st.org, st.R, ps, cs);
-        hit = any_hit<R>(a.sv, st, ps, cs, (a.sv.use_grid && !a.sv.sun.point_source) ? psun : nullptr);
+        uint32_t tests = 0;
+        hit = any_hit<R>(a.sv, st, ps, cs, (a.sv.use_grid && !a.sv.sun.point_source) ? psun : nullptr, &tests);
+        if (a.counters) {
+            atomicAdd(&a.counters[0], 1ull);
+            atomicAdd(&a.counters[1], (unsigned long long)tests);
+            atomicAdd(&a.counters[3], (unsigned long long)rng.k);
+        }
     }
     const uint64_t word = __ballot(hit);
     if ((threadIdx.x & 63) == 0) {
         const uint32_t w = t >> 6;
         if ((uint64_t)w * 64 < a.n) a.mask[w] = word;
     }
+}
+
+// ------------------------------------------- cooperative sunshape draw -----
+// The 'd' sunshape accepts about one rejection iteration in a hundred for a
+// Buie aureole, and a lane-per-ray loop runs a whole wave until its slowest
+// lane is done.  Here the 64 lanes of a wave share the wave's unresolved
+// rays: each round, lane l takes ray (l mod m) of the m unresolved ones and
+// iteration next_j + l/m of that ray's loop; each ray keeps the smallest
+// accepted iteration.  The result is exactly what the serial loop of
+// coretrace/errors.cpp:160-185 returns on the same stream.
+STD int nth_set_bit(uint64_t v, int n) {
+    for (int i = 0; i < n; ++i) v &= v - 1;
+    return __ffsll((unsigned long long)v) - 1;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int myray = blockIdx.x * kBlock + (threadIdx.x & ~63) + lane;
+    const bool valid = myray < a.n;
+    const uint64_t sid = valid ? a.sun_index[myray] : 0;
+    const uint32_t sid_lo = (uint32_t)sid, sid_hi = (uint32_t)(sid >> 32);
+    uint64_t U = __ballot(valid);
+    uint32_t next_g = 0;  // next group of four tries of this lane's ray
+    for (int round = 0; U != 0; ++round) {
+        const int m = __popcll(U);
+        const int rank = lane % m, rep = lane / m;
+        const int rl = nth_set_bit(U, rank);
+        const uint32_t g = (uint32_t)__shfl((int)next_g, rl) + (uint32_t)rep;
+        const uint64_t s = ((uint64_t)(uint32_t)__shfl((int)sid_hi, rl) << 32) | (uint32_t)__shfl((int)sid_lo, rl);
+        R t2 = R(0);
+        const int off = sunshape_d_group<R>(a.seed, s, a.k0, g, a.sun, a.aux, &t2);
+        const bool acc = off >= 0;
+        const uint64_t accm = __ballot(acc);
+        // lanes that served this lane's ray: rank r_own, r_own + m, ...
+        const bool owner = (U >> lane) & 1ull;
+        const int r_own = __popcll(U & ((1ull << lane) - 1ull));
+        uint64_t mine = 0;
+        if (owner)
+            for (int q = r_own; q < 64; q += m) mine |= 1ull << q;
+        const uint64_t hit = accm & mine;
+        const int first = hit ? (__ffsll((unsigned long long)hit) - 1) : 0;
+        const double t2_first = (double)__shfl((double)t2, first);
+        const int off_first = __shfl(off, first);
+        bool resolved = false;
+        if (owner) {
+            if (hit) {
+                const uint32_t gg = next_g + (uint32_t)((first - r_own) / m);
+                const uint32_t jj = 4u * gg + (uint32_t)off_first;
+                SunPre p;
+                p.theta2 = (double)((R)t2_first / R(1000000.0));
+                p.kend = a.k0 + 3u * (jj + 1u);
+                p.ok = 1u;
+                a.out[myray] = p;
+                resolved = true;
+            } else {
+                next_g += (uint32_t)__popcll(mine);
+                if (next_g > (1u << 24)) {  // no acceptance in 2^26 tries: give up on this ray
+                    SunPre p;
+                    p.theta2 = 0.0;
+                    p.kend = 0;
+                    p.ok = 0u;
+                    a.out[myray] = p;
+                    atomicOr(a.err_flags, 8u);
+                    resolved = true;
+                }
+            }
+        }
+        U &= ~__ballot(resolved);
+    }
+}
+
+template <typename R>
+void launch_sunshape_d(const SunPreArgs &a, hipStream_t s) {
+    const int blocks = (a.n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(sunshape_d_kernel<R>, dim3(blocks), dim3(kBlock), 0, s, a);
 }
 
 // ------------------------------------------------------- stage trace ------
@@ -201,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void trace_stage_kernel(TraceArgs a) {
         raynum = r.raynum;
         sun_index = r.sun_index;
         for (int i = 0; i < 3; i++) { pg[i] = r.pos[i]; cg[i] = r.cos[i]; }
-        rng.begin(a.seed, (uint32_t)a.si, raynum);
+        rng.begin(a.seed, (uint32_t)a.si, sun_index);
     }
     double ps[3], cs[3];
     to_local(pg, cg, st.org, st.R, ps, cs);
@@ -235,10 +329,15 @@ __global__ __launch_bounds__(kBlock) void trace_stage_kernel(TraceArgs a) {
         has_pend = false;
     };
 
+    uint32_t ev_tests = 0, ev_nodes = 0;
     for (;;) {
         Best<R> b;
+        b.tests = 0;
+        b.nodes = 0;
         const bool first_sun = sun_stage && mhc == 0 && sv.use_grid && !sv.sun.point_source;
         closest_hit<R>(sv, st, ps, cs, first_sun ? psun : nullptr, b);
+        ev_tests += b.tests;
+        ev_nodes += b.nodes;
         if (!(b.path < 1e99)) {
             if (sun_stage) {
                 if (mhc == 0) atomicOr(a.err_flags, 2u);
@@ -291,7 +390,12 @@ __global__ __launch_bounds__(kBlock) void trace_stage_kernel(TraceArgs a) {
             R df[3] = {b.h.DF[0], b.h.DF[1], b.h.DF[2]};
             if (a.inc_sun && sun_stage && mhc == 1) {
                 R co[3];
-                errors<R>(rng, cin, 1, sv.sun, sv.aux, e.interaction, f, co, df);
+                if (a.sun_pre != nullptr && rng.k == 3u && a.sun_pre[t].ok) {
+                    rng.seek(a.sun_pre[t].kend);
+                    sun_error_pre<R>(rng, cin, (R)a.sun_pre[t].theta2, co);
+                } else {
+                    errors<R>(rng, cin, 1, sv.sun, sv.aux, e.interaction, f, co, df);
+                }
                 cin[0] = co[0]; cin[1] = co[1]; cin[2] = co[2];
             }
             if (a.inc_err) {
@@ -322,6 +426,13 @@ __global__ __launch_bounds__(kBlock) void trace_stage_kernel(TraceArgs a) {
     flush();
     a.rec_count[t] = nrec;
     a.out_flag[t] = passed ? 1u : 0u;
+    if (a.counters) {
+        atomicAdd(&a.counters[0], 1ull);
+        atomicAdd(&a.counters[1], (unsigned long long)ev_tests);
+        atomicAdd(&a.counters[2], (unsigned long long)ev_nodes);
+        atomicAdd(&a.counters[3], (unsigned long long)rng.k);
+        atomicAdd(&a.counters[4], (unsigned long long)nrec);
+    }
     if (passed) {
         DevRay r;
         for (int i = 0; i < 3; i++) { r.pos[i] = pg[i]; r.cos[i] = cg[i]; }
@@ -363,178 +474,6 @@ template <typename R>
 void launch_trace_stage(const TraceArgs &a, hipStream_t s) {
     const int blocks = (a.n_rays + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(trace_stage_kernel<R>, dim3(blocks), dim3(kBlock), 0, s, a);
-}
-template void launch_sun_mask<double>(const MaskArgs &, hipStream_t);
-template void launch_sun_mask<float>(const MaskArgs &, hipStream_t);
-template void launch_trace_stage<double>(const TraceArgs &, hipStream_t);
-template void launch_trace_stage<float>(const TraceArgs &, hipStream_t);
-
-// ------------------------------------------------------------- scans ------
-static constexpr int kScanItems = 16;
-static constexpr int kScanBlock = 256;
-static constexpr int kScanTile = kScanBlock * kScanItems;  // 4096
-
-// Block-wide exclusive scan of one value per thread (LDS, 64-wide waves).
-__device__ uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
-    __shared__ uint32_t wsum[kScanBlock / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) wsum[wid] = x;
-    __syncthreads();
-    uint32_t off = 0, tot = 0;
-    for (int w = 0; w < kScanBlock / 64; ++w) {
-        if (w < wid) off += wsum[w];
-        tot += wsum[w];
-    }
-    __syncthreads();
-    *total = tot;
-    return off + x - v;
-}
-
-__global__ __launch_bounds__(kScanBlock) void scan_tiles(const uint32_t *in, uint32_t *out, size_t n,
-                                                         uint32_t *tile_sums) {
-    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
-    uint32_t s = 0;
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        v[i] = (base + i < n) ? in[base + i] : 0u;
-        s += v[i];
-    }
-    uint32_t tot;
-    uint32_t off = block_excl_scan(s, &tot);
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        if (base + i < n) out[base + i] = off;
-        off += v[i];
-    }
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(kScanBlock) void scan_sums(uint32_t *sums, size_t ntiles, uint32_t *total) {
-    // single block; ntiles <= kScanTile
-    const size_t base = (size_t)threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
-    uint32_t s = 0;
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        v[i] = (base + i < ntiles) ? sums[base + i] : 0u;
-        s += v[i];
-    }
-    uint32_t tot;
-    uint32_t off = block_excl_scan(s, &tot);
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        if (base + i < ntiles) sums[base + i] = off;
-        off += v[i];
-    }
-    if (threadIdx.x == 0) *total = tot;
-}
-
-__global__ __launch_bounds__(kScanBlock) void scan_add(uint32_t *out, size_t n, const uint32_t *tile_off) {
-    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
-    const uint32_t add = tile_off[blockIdx.x];
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i)
-        if (base + i < n) out[base + i] += add;
-}
-
-__global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
-}
-
-size_t scan_tmp_words(size_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
-
-void launch_exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, uint32_t *total,
-                           hipStream_t s) {
-    const size_t ntiles = (n + kScanTile - 1) / kScanTile;
-    if (ntiles == 0) {
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, total, (size_t)1, 0u);
-        return;
-    }
-    hipLaunchKernelGGL(scan_tiles, dim3((unsigned)ntiles), dim3(kScanBlock), 0, s, in, out, n, tmp);
-    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanBlock), 0, s, tmp, ntiles, total);
-    hipLaunchKernelGGL(scan_add, dim3((unsigned)ntiles), dim3(kScanBlock), 0, s, out, n, tmp);
-}
-
-// -------------------------------------------------------- compaction ------
-__global__ void mask_popc_kernel(const uint64_t *mask, uint32_t *counts, size_t nwords) {
-    size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w < nwords) counts[w] = (uint32_t)__popcll(mask[w]);
-}
-void launch_mask_popc(const uint64_t *mask, uint32_t *counts, size_t nwords, hipStream_t s) {
-    hipLaunchKernelGGL(mask_popc_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s, mask, counts,
-                       nwords);
-}
-
-__global__ void mask_compact_kernel(const uint64_t *mask, const uint32_t *offsets, size_t nwords, uint64_t s0,
-                                    uint32_t limit, uint64_t *hits) {
-    size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= nwords) return;
-    uint64_t m = mask[w];
-    uint32_t o = offsets[w];
-    while (m != 0 && o < limit) {
-        int b = __ffsll((unsigned long long)m) - 1;
-        hits[o++] = s0 + w * 64 + (uint64_t)b;
-        m &= m - 1;
-    }
-}
-void launch_mask_compact(const uint64_t *mask, const uint32_t *offsets, size_t nwords, uint64_t s0, uint32_t limit,
-                         uint64_t *hits, hipStream_t s) {
-    hipLaunchKernelGGL(mask_compact_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s, mask, offsets,
-                       nwords, s0, limit, hits);
-}
-
-__global__ void compact_records_kernel(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets,
-                                       int n_rays, int nslots, DevRecord *out) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_rays) return;
-    uint32_t c = count[t];
-    uint32_t m = c < (uint32_t)nslots ? c : (uint32_t)nslots;
-    uint32_t o = offsets[t];
-    for (uint32_t j = 0; j < m; ++j) out[o + j] = slots[(size_t)t * nslots + j];
-}
-void launch_compact_records(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets, int n_rays,
-                            int nslots, DevRecord *out, hipStream_t s) {
-    hipLaunchKernelGGL(compact_records_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, s, slots, count, offsets,
-                       n_rays, nslots, out);
-}
-
-__global__ void compact_overflow_kernel(const DevRecord *ovf, const uint32_t *meta, const uint32_t *n_ovf_dev,
-                                        uint32_t cap, const uint32_t *offsets, DevRecord *out) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n = *n_ovf_dev;
-    if (n > cap) n = cap;
-    if (i >= n) return;
-    uint32_t t = meta[2 * i], seq = meta[2 * i + 1];
-    out[offsets[t] + seq] = ovf[i];
-}
-void launch_compact_overflow(const DevRecord *ovf, const uint32_t *meta, const uint32_t *n_ovf_dev, uint32_t cap,
-                             const uint32_t *offsets, DevRecord *out, hipStream_t s) {
-    hipLaunchKernelGGL(compact_overflow_kernel, dim3((cap + 255) / 256), dim3(256), 0, s, ovf, meta, n_ovf_dev, cap,
-                       offsets, out);
-}
-
-__global__ void compact_rays_kernel(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, int n_rays,
-                                    DevRay *out) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_rays) return;
-    if (flag[t]) out[offsets[t]] = slots[t];
-}
-void launch_compact_rays(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, int n_rays, DevRay *out,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(compact_rays_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, s, slots, flag, offsets, n_rays,
-                       out);
 }
 
 }  // namespace st

@@ -76,7 +76,9 @@ struct DevSun {
     int32_t shape, point_source;
     int32_t tab, tab_n;     // mirrored table (angles then intensities) in aux
     int32_t tab_ascending;  // 1: angles strictly ascending (binary search is exact)
+    int32_t env, env_n;     // upper envelope of intensity/max per polar-angle bin (aux)
     int32_t pad;
+    double env_scale;       // bins per mrad
 };
 
 // Uniform grid over the sun-plane box listing, per cell, the stage-1
@@ -87,12 +89,14 @@ struct DevSunGrid {
     int32_t cell0, list0;   // offsets into the cell-start and list buffers
 };
 
-// BVH node: float box padded outward; children or leaf range.
+// BVH node in depth-first order (an inner node's left child is the next
+// node): float box padded outward, the index to continue at when the box is
+// missed or the subtree is done, and the leaf item range.
 struct DevBvhNode {
     float lo[3];
-    int32_t left;           // >= 0 inner: left child index; < 0 leaf: -(first+1)
+    int32_t skip;           // next node after this subtree (node count = end)
     float hi[3];
-    int32_t right;          // inner: right child; leaf: count
+    int32_t leaf;           // 0: inner node; else (first item << 3) | count (1..7)
 };
 
 // Output record in stage frame (coretrace/types.h:242-249 fields).
