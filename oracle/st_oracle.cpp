@@ -12,7 +12,9 @@
 //                                   stream, 1 = the per-ray Philox streams
 //                                   the GPU kernels draw from;
 //   st_oracle_ray_sun_index(ctx, out)  0-based sun-ray index of each ray;
-//   st_oracle_flux(...)            the GUI flux map of one element.
+//   st_oracle_flux(...)            the GUI flux map of one element;
+//   st_oracle_set_torus_start(ctx, on)  start torus solves where the HIP
+//                                   library starts them for far origins.
 // In mode 0 its records are checked bit for bit against oracle/_ref.
 //
 // Parity pins: RNG streams vs libstdc++ and Random123 vectors; full record
@@ -635,6 +637,11 @@ int st_sim_run(st_context_t p, unsigned int seed, progress_cb cb, void *data) {
     sys->all.clear();
     for (Stage *s : sys->stages) s->recs.clear();
     if (!init_geometries(*sys)) return -1;
+    for (Stage *s : sys->stages)
+        for (Element *e : s->elements) {
+            e->torus_start_r = 0.0;
+            if (sys->torus_start && e->stype == 10) torus_start_sphere(*e);
+        }
     bool cancelled = false;
     if (!trace(*sys, seed, (uint64_t)(int64_t)sys->raycount, (uint64_t)(int64_t)sys->raymax, sys->err_sunshape,
                sys->err_optical, cb, data, &cancelled))
@@ -666,6 +673,7 @@ void st_matrix_transpose(double in[3][3], double out[3][3]) {
 
 // ---- oracle-only entry points ----
 int st_oracle_set_rng(st_context_t p, int mode) { SYS(p, -1); sys->rng_mode = mode; return 1; }
+int st_oracle_set_torus_start(st_context_t p, int on) { SYS(p, -1); sys->torus_start = on != 0; return 1; }
 int st_oracle_ray_sun_index(st_context_t p, uint64_t *out) {
     SYS(p, -1);
     for (size_t i = 0; i < sys->ray_sun_index.size(); i++) out[i] = sys->ray_sun_index[i];

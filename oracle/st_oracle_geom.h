@@ -4,6 +4,9 @@
 // surfacezatxypair.cpp and newzstartforcubicsplinesurf.cpp, keeping their
 // floating-point evaluation order.
 #pragma once
+#include <algorithm>
+#include <cctype>
+
 #include "st_oracle_model.h"
 
 namespace oracle {
@@ -678,6 +681,57 @@ inline bool spline_start_plane(const Element &e, const double X[3], const double
     return true;
 }
 
+// Optional torus start point (off unless st_oracle_set_torus_start is
+// called).  From an origin kilometres away the reference quartic is
+// ill-conditioned: a 1-ulp change of the origin flips the outcome of a
+// sizeable fraction of rays, so no other build can reproduce those answers.
+// The HIP library therefore starts the solve on the ray where it enters
+// the sphere of radius 2r+1 about the element's hit-bounding sphere (c, r)
+// when the origin lies beyond it.  This restates that sphere for a torus
+// (x, y from the aperture as the library bounds them, z in [0, 2 Rs], padded
+// by 1e-4 + 1e-7 max|x,y|) so the oracle can run the same reference solve
+// from the same start point.
+inline void torus_start_sphere(Element &e) {
+    const double A = e.p[0], B = e.p[1], C = e.p[2];
+    double xl, xh, yl, yh;
+    switch (tolower(e.shape)) {
+    case 'c': case 'h': case 't':
+        xl = yl = -A / 2.0; xh = yh = A / 2.0;
+        break;
+    case 'r':
+        xl = -A / 2.0; xh = A / 2.0; yl = -B / 2.0; yh = B / 2.0;
+        break;
+    case 'a':
+        if (A == 0.0 && B == 0.0) {
+            double r = e.annular_r + e.cross_r;
+            xl = yl = -r; xh = yh = r;
+        } else {
+            xl = yl = -fabs(B); xh = yh = fabs(B);
+        }
+        break;
+    case 'l':
+        if (A == 0.0 && B == 0.0) { xl = -1e3; xh = 1e3; }
+        else { xl = std::min(A, B); xh = std::max(A, B); }
+        yl = -C / 2.0; yh = C / 2.0;
+        break;
+    default: {
+        int nv = (tolower(e.shape) == 'q') ? 4 : 3;
+        xl = yl = 1e300; xh = yh = -1e300;
+        for (int k = 0; k < nv; k++) {
+            xl = std::min(xl, e.p[2 * k]); xh = std::max(xh, e.p[2 * k]);
+            yl = std::min(yl, e.p[2 * k + 1]); yh = std::max(yh, e.p[2 * k + 1]);
+        }
+        break;
+    }
+    }
+    const double zl = 0.0, zh = 2.0 * e.cross_r;
+    const double pad = 1e-4 + 1e-7 * std::max(std::max(fabs(xl), fabs(xh)), std::max(fabs(yl), fabs(yh)));
+    const double lo[3] = {xl - pad, yl - pad, zl - pad}, hi[3] = {xh + pad, yh + pad, zh + pad};
+    for (int k = 0; k < 3; k++) e.torus_start_c[k] = (lo[k] + hi[k]) / 2;
+    e.torus_start_r = 0.5 * sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                 (hi[2] - lo[2]) * (hi[2] - lo[2]));
+}
+
 // coretrace/intersect.cpp:140-1011
 inline void intersect(const double P[3], const double D[3], const Element &e, double X[3], double Cout[3], double DF[3],
                       double *path, int *err) {
@@ -704,7 +758,21 @@ inline void intersect(const double P[3], const double D[3], const Element &e, do
         return;
     }
     if (e.stype == 10) {
-        torus_closed_form(e, P, D, X, DF, path, err);
+        double t0 = 0.0;
+        if (e.torus_start_r > 0.0) {
+            const double d[3] = {P[0] - e.torus_start_c[0], P[1] - e.torus_start_c[1], P[2] - e.torus_start_c[2]};
+            const double tc = -(d[0] * D[0] + d[1] * D[1] + d[2] * D[2]);
+            const double dist2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const double rr = 2.0 * e.torus_start_r + 1.0;
+            if (dist2 > rr * rr && tc > rr) t0 = tc - rr;
+        }
+        if (t0 == 0.0) {
+            torus_closed_form(e, P, D, X, DF, path, err);
+        } else {
+            const double Ps[3] = {P[0] + t0 * D[0], P[1] + t0 * D[1], P[2] + t0 * D[2]};
+            torus_closed_form(e, Ps, D, X, DF, path, err);
+            *path = t0 + *path;
+        }
         return;
     }
 

@@ -63,6 +63,8 @@ struct Element {
     int interaction = 0;
     OpticSet *optics = nullptr;
     std::string optic_name;
+    // torus start sphere (0 = off): see torus_start_sphere in st_oracle_geom.h
+    double torus_start_c[3] = {0, 0, 0}, torus_start_r = 0;
     double B(int i, int j) const { return bc[(size_t)i * bc_n + j]; }
 };
 
@@ -112,6 +114,7 @@ struct System {
     uint64_t sunraycount = 0;
     std::vector<std::string> messages;
     int rng_mode = 0;                  // 0 reference stream, 1 counter streams
+    bool torus_start = false;          // start torus solves at the bounding sphere
     std::vector<uint64_t> ray_sun_index;  // raynum-1 -> 0-based sun ray index
     uint64_t draws_total = 0;
 

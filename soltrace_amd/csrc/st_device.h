@@ -45,8 +45,9 @@ template <> struct M<float> {
 // ------------------------------------------------------------------ RNG ---
 // Philox4x32-10 (Salmon et al., SC'11).  Draw k of the stream (seed, stage,
 // id) is word k%4 of the block with counter (id_lo, id_hi, stage, k/4) and
-// key (seed, 0x53544F52): u = w 2^-32 in double, (w >> 8) 2^-24 in float
-// (the same number at the two resolutions).
+// key (seed, 0x53544F52), u = w 2^-32.  Both precisions draw the same double
+// uniforms and take every accept/reject decision in double, so the FP32 path
+// samples exactly the errors the FP64 path samples.
 STD void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
@@ -66,17 +67,18 @@ STD uint32_t pick4(const uint32_t b[4], uint32_t i) {
     return i == 0 ? b[0] : (i == 1 ? b[1] : (i == 2 ? b[2] : b[3]));
 }
 
-template <typename R>
+STD double word_uniform(uint32_t w) { return (double)w * 2.3283064365386963e-10; }
+
 struct Rng {
     uint32_t seed, stage, id_lo, id_hi;
     uint32_t k;
     uint32_t blk[4];
     bool have_saved;
-    R saved;
+    double saved;
 
     STD void begin(uint32_t seed_, uint32_t stage_, uint64_t id) {
         seed = seed_; stage = stage_; id_lo = (uint32_t)id; id_hi = (uint32_t)(id >> 32);
-        k = 0; have_saved = false; saved = R(0);
+        k = 0; have_saved = false; saved = 0.0;
     }
     STD void fill(uint32_t block) {
         blk[0] = id_lo; blk[1] = id_hi; blk[2] = stage; blk[3] = block;
@@ -93,24 +95,33 @@ struct Rng {
         k = kk;
         if (k & 3u) fill(k >> 2);
     }
-    STD R uniform();
-    // double draw regardless of R (sun-plane positions span kilometres)
-    STD double uniform_d() { return (double)next_word() * 2.3283064365386963e-10; }
+    STD double uniform() { return word_uniform(next_word()); }
     // Marsaglia polar with one cached deviate, the algorithm of libstdc++'s
     // normal_distribution (bits/random.tcc:1802-1835): y*m first, x*m cached.
-    STD R normal(R mean, R sd) {
-        R ret;
+    // The rejection test is double in both modes; FAST takes the radial
+    // factor from float log1p/sqrt (relative error ~1e-7 of a deviate).
+    template <bool FAST>
+    STD double normal(double mean, double sd) {
+        double ret;
         if (have_saved) {
             have_saved = false;
             ret = saved;
         } else {
-            R x, y, r2;
+            double x, y, r2;
             do {
-                x = R(2) * uniform() - R(1);
-                y = R(2) * uniform() - R(1);
+                x = 2.0 * uniform() - 1.0;
+                y = 2.0 * uniform() - 1.0;
                 r2 = x * x + y * y;
-            } while (r2 > R(1) || r2 == R(0));
-            R mult = M<R>::sqrt_(R(-2) * M<R>::log_(r2) / r2);
+            } while (r2 > 1.0 || r2 == 0.0);
+            double mult;
+            if constexpr (FAST) {
+                // log(r2) to float relative accuracy over (0, 1]: log1p of
+                // r2 - 1 (exact in double) near 1, log of r2 below 1/2
+                const float lg = (r2 < 0.5) ? ::logf((float)r2) : ::log1pf((float)(r2 - 1.0));
+                mult = (double)::sqrtf(-2.0f * lg / (float)r2);
+            } else {
+                mult = ::sqrt(-2 * ::log(r2) / r2);
+            }
             saved = x * mult;
             have_saved = true;
             ret = y * mult;
@@ -118,12 +129,6 @@ struct Rng {
         return mean + ret * sd;
     }
 };
-template <> STD double Rng<double>::uniform() { return (double)next_word() * 2.3283064365386963e-10; }
-template <> STD float Rng<float>::uniform() { return (float)(next_word() >> 8) * (1.0f / 16777216.0f); }
-
-template <typename R> STD R word_to_uniform(uint32_t w);
-template <> STD double word_to_uniform<double>(uint32_t w) { return (double)w * 2.3283064365386963e-10; }
-template <> STD float word_to_uniform<float>(uint32_t w) { return (float)(w >> 8) * (1.0f / 16777216.0f); }
 
 // ------------------------------------------------------------ transforms --
 STD void mat_vec(const double *Mx, const double v[3], double o[3]) {
@@ -250,6 +255,32 @@ STD bool aperture_contains(const DevElem &e, R x, R y, bool plane) {
 template <typename R>
 STD R sqr_grad(R K, R d, R a2) { return (((((R(2) * K) * d) / a2) * R(2)) * K) * d / a2; }
 
+// Unit normal of a closed-form quadric at X
+// (coretrace/quadricsurfaceclosedform.cpp:294-306; same constants as below).
+template <typename R>
+STD void quadric_normal(const DevElem &e, const R X[3], R DF[3]) {
+    R Zc = 0, Kx = 1, Ky = 1, Kz = 1, a2 = 1, b2 = 1, c2 = 1;
+    switch (e.surf) {
+    case SURF_SPHERE: Ky = (e.stype == 7) ? R(0) : R(1); Zc = R(1) / (R)e.vcx; break;
+    case SURF_PARABOLA: {
+        a2 = R(2) / (R)e.vcx;
+        b2 = e.vcy > 0.0 ? R(2) / (R)e.vcy : R(1e10);
+        Ky = (e.stype == 7) ? R(0) : R(1);
+        R sm = M<R>::sqrt_(sqr_grad(Kx, X[0], a2) + sqr_grad(Ky, X[1], b2) + R(1));
+        DF[0] = -(R(2) * Kx * X[0] / a2) / sm;
+        DF[1] = -(R(2) * Ky * X[1] / b2) / sm;
+        DF[2] = R(1) / sm;
+        return;
+    }
+    case SURF_CONIC: Kz = (R)e.kappa; Zc = R(1) / (R)e.kappa / (R)e.vcx; break;
+    default: Ky = 0; Zc = R(1) / (R)e.curv_rev; break;  // SURF_CYLINDER
+    }
+    R sm = M<R>::sqrt_(sqr_grad(Kx, X[0], a2) + sqr_grad(Ky, X[1], b2) + sqr_grad(Kz, X[2] - Zc, c2));
+    DF[0] = -(R(2) * Kx * X[0] / a2) / sm;
+    DF[1] = -(R(2) * Ky * X[1] / b2) / sm;
+    DF[2] = -(R(2) * Kz * (X[2] - Zc) / c2) / sm;
+}
+
 // Ray-quadric intersection (coretrace/quadricsurfaceclosedform.cpp:58-308).
 // Returns false on a miss (ErrorFlag 1).
 template <typename R>
@@ -348,18 +379,17 @@ STD bool quadric(const DevElem &e, const R P[3], const R D[3], R X[3], R DF[3], 
     }
     for (int i = 0; i < 3; i++) X[i] = P[i] + t * D[i];
     *path = t;
-    if (e.surf == SURF_PARABOLA) {
-        R sm = M<R>::sqrt_(sqr_grad(Kx, X[0] - Xc, a2) + sqr_grad(Ky, X[1] - Yc, b2) + R(1));
-        DF[0] = -(R(2) * Kx * (X[0] - Xc) / a2) / sm;
-        DF[1] = -(R(2) * Ky * (X[1] - Yc) / b2) / sm;
-        DF[2] = R(1) / sm;
-    } else {
-        R sm = M<R>::sqrt_(sqr_grad(Kx, X[0] - Xc, a2) + sqr_grad(Ky, X[1] - Yc, b2) + sqr_grad(Kz, X[2] - Zc, c2));
-        DF[0] = -(R(2) * Kx * (X[0] - Xc) / a2) / sm;
-        DF[1] = -(R(2) * Ky * (X[1] - Yc) / b2) / sm;
-        DF[2] = -(R(2) * Kz * (X[2] - Zc) / c2) / sm;
-    }
+    quadric_normal<R>(e, X, DF);
     return true;
+}
+
+// Torus normal at a hit point (coretrace/torusclosedform.cpp:176-187).
+STD void torus_normal(const DevElem &e, const double X[3], double DF[3]) {
+    const double Rs = e.cross_r, Ra = e.ann_r;
+    const double rxy = ::sqrt(X[0] * X[0] + X[1] * X[1]);
+    DF[0] = 2.0 * X[0] * (Ra - rxy) / rxy;
+    DF[1] = 2.0 * X[1] * (Ra - rxy) / rxy;
+    DF[2] = -(2.0 * (X[2] - Rs));
 }
 
 // Torus by the Ferrari quartic with a bisected resolvent cubic and the
@@ -783,11 +813,13 @@ STD bool element_hit(const DevElem &e, const double *aux, const double P[3], con
     double path;
     bool ok;
     if (s == SURF_TORUS) {
-        // The quartic in the ray parameter is ill-conditioned for an origin
-        // kilometres away (a sun ray starts 10 km back); start it at the
-        // bounding sphere instead.  All real intersections lie inside the
-        // sphere, so their order and signs -- and the root choice -- are
-        // unchanged.
+        // The reference quartic solve (root432) is ill-conditioned for an
+        // origin kilometres away (a sun ray starts 10 km back): there a
+        // 1-ulp change of the origin flips the outcome of a sizeable share
+        // of rays.  For such origins the solve starts where the ray enters
+        // the sphere of radius 2r+1 about the element's hit-bounding sphere;
+        // every real intersection lies inside, so no hit is skipped.  Nearer
+        // origins run the reference arithmetic unchanged.
         double t0 = 0.0;
         {
             double d[3] = {P[0] - e.bsphere_c[0], P[1] - e.bsphere_c[1], P[2] - e.bsphere_c[2]};
@@ -800,6 +832,7 @@ STD bool element_hit(const DevElem &e, const double *aux, const double P[3], con
         double Xd[3], DFd[3], L;
         ok = torus(e, t0 != 0.0 ? Ps : P, D, Xd, DFd, &L);
         path = t0 + L;
+        if (ok && path < 0.0) ok = false;  // coretrace/determineelementintersectionnew.cpp:86-100
         for (int i = 0; i < 3; i++) { X[i] = (R)Xd[i]; DF[i] = (R)DFd[i]; }
     } else if (s <= SURF_CYLINDER) {
         // closed-form quadrics; a float ray starting far away is first moved
@@ -879,30 +912,132 @@ STD bool element_hit(const DevElem &e, const double *aux, const double P[3], con
     return true;
 }
 
-// ------------------------------------------------------------ optics ------
-// Rotation taking the local z axis onto direction c (coretrace/errors.cpp:98-121).
-template <typename R>
-STD void ray_frame(const R c[3], R RT[9]) {
-    R e0, e1;
-    if (c[2] == R(0)) {
-        if (c[0] == R(0)) { e0 = 0; e1 = (R)(M_PI / 2.0); }
-        else { e0 = (R)(M_PI / 2.0); e1 = M<R>::atan2_(c[1], M<R>::sqrt_(c[0] * c[0] + c[2] * c[2])); }
+// Double-precision finish of a hit the FP32 search found: the point is
+// moved onto the surface along the ray (path from the float solve, then one
+// Newton step on the surface function, which squares the float error away)
+// and the normal is evaluated there, flipped towards the incoming ray as
+// element_hit flips it.  A float hit point is off by ~1e-6 m, which tilts a
+// heliostat normal (curvature radius ~1 km) by ~1e-9 rad; reflected towards
+// a receiver hundreds of metres away at grazing incidence that grows to
+// 1e-4 m.  P, D: the nudged ray in the element frame.
+STD void hit_finish_d(const DevElem &e, const double *aux, const double P[3], const double D[3], double path,
+                      double X[3], double DF[3], int *back) {
+    const int s = e.surf;
+    if (s == SURF_FLAT) {
+        // the reference's start plane z = 0, taken as the hit
+        const double S0 = (0.0 - P[2] == 0.0) ? 0.0 : (0.0 - P[2]) / (D[2] + 0.00000000001);
+        X[0] = P[0] + D[0] * S0; X[1] = P[1] + D[1] * S0; X[2] = 0.0;
+        DF[0] = 0.0; DF[1] = 0.0; DF[2] = 1.0;
+    } else if (s == SURF_TORUS) {
+        for (int i = 0; i < 3; i++) X[i] = P[i] + path * D[i];
+        torus_normal(e, X, DF);
+    } else if (s <= SURF_CYLINDER) {
+        for (int i = 0; i < 3; i++) X[i] = P[i] + path * D[i];
+        // F and grad F of the implicit quadric (cancellation-free forms)
+        double F, G[3];
+        const double x = X[0], y = X[1], z = X[2];
+        if (s == SURF_PARABOLA) {
+            const double a2 = 2.0 / e.vcx, b2 = e.vcy > 0.0 ? 2.0 / e.vcy : 1e10;
+            const double ky = (e.stype == 7) ? 0.0 : 1.0;
+            F = x * x / a2 + ky * y * y / b2 - z;
+            G[0] = 2.0 * x / a2; G[1] = 2.0 * ky * y / b2; G[2] = -1.0;
+        } else if (s == SURF_CONIC) {
+            F = x * x + y * y + e.kappa * z * z - 2.0 * z / e.vcx;
+            G[0] = 2.0 * x; G[1] = 2.0 * y; G[2] = 2.0 * e.kappa * z - 2.0 / e.vcx;
+        } else {
+            const double zc = (s == SURF_SPHERE) ? 1.0 / e.vcx : 1.0 / e.curv_rev;
+            const double ky = (s == SURF_CYLINDER || e.stype == 7) ? 0.0 : 1.0;
+            F = x * x + ky * y * y + z * (z - 2.0 * zc);
+            G[0] = 2.0 * x; G[1] = 2.0 * ky * y; G[2] = 2.0 * (z - zc);
+        }
+        const double gd = G[0] * D[0] + G[1] * D[1] + G[2] * D[2];
+        if (gd != 0.0) {
+            const double t = path - F / gd;
+            for (int i = 0; i < 3; i++) X[i] = P[i] + t * D[i];
+        }
+        quadric_normal<double>(e, X, DF);
     } else {
-        e0 = M<R>::atan2_(c[0], c[2]);
-        e1 = M<R>::atan2_(c[1], M<R>::sqrt_(c[0] * c[0] + c[2] * c[2]));
+        // Newton surfaces: one more iteration in double from the float
+        // iterate, then the surface z and slopes at the final point
+        double Xc[3], F;
+        for (int i = 0; i < 3; i++) Xc[i] = P[i] + path * D[i];
+        surface_eval<double>(e, aux, Xc, &F, DF);
+        const double dfd = DF[0] * D[0] + DF[1] * D[1] + DF[2] * D[2];
+        const double t = (dfd != 0.0) ? path - F / dfd : path;
+        for (int i = 0; i < 3; i++) X[i] = P[i] + t * D[i];
+        surface_eval<double>(e, aux, X, &F, DF);
     }
-    R ca = M<R>::cos_(e0), cb = M<R>::cos_(e1), sa = M<R>::sin_(e0), sb = M<R>::sin_(e1);
-    // gamma = 0: RRefToLoc rows; store its transpose (local -> reference)
+    *back = (dot3<double>(D, DF) < 0.0) ? 0 : 1;
+    if (*back) { DF[0] = -DF[0]; DF[1] = -DF[1]; DF[2] = -DF[2]; }
+}
+
+// ------------------------------------------------------------ optics ------
+// Everything after the closest hit runs on double directions in both
+// precisions: a direction error of 1e-7 rad already moves a receiver hit by
+// 0.1 mm over a kilometre of flight, and on CDNA4 double add/multiply/FMA
+// issue at the non-packed float rate.  FAST (the FP32 path) differs only in
+// how it gets there: ray frames are built algebraically instead of through
+// atan2/sin/cos, and the sines and cosines of the small perturbation angles,
+// whose relative error is what matters, are float.
+
+// Rotation taking the local z axis onto direction c, stored local ->
+// reference (coretrace/errors.cpp:98-121, surfacenormalerrors.cpp:87-106:
+// Euler angles alpha = atan2(cx, cz), beta = atan2(cy, sqrt(cx^2 + cz^2)),
+// gamma = 0).  The algebraic form uses cos(alpha) = cz/h, sin(alpha) = cx/h,
+// cos(beta) = h/n, sin(beta) = cy/n (h = sqrt(cx^2 + cz^2), n = |c|), and
+// keeps the reference's special cases for cz == 0 (alpha fixed at pi/2, or
+// alpha = 0 and beta = pi/2 when cx == 0 too).
+template <bool FAST>
+STD void ray_frame(const double c[3], double RT[9]) {
+    double ca, cb, sa, sb;
+    if constexpr (FAST) {
+        if (c[2] == 0.0) {
+            if (c[0] == 0.0) { ca = 1.0; sa = 0.0; cb = 0.0; sb = 1.0; }
+            else {
+                const double h = ::fabs(c[0]), in = 1.0 / ::sqrt(h * h + c[1] * c[1]);
+                ca = 0.0; sa = 1.0; cb = h * in; sb = c[1] * in;
+            }
+        } else {
+            const double h2 = c[0] * c[0] + c[2] * c[2];
+            const double h = ::sqrt(h2), ih = 1.0 / h, in = 1.0 / ::sqrt(h2 + c[1] * c[1]);
+            ca = c[2] * ih; sa = c[0] * ih; cb = h * in; sb = c[1] * in;
+        }
+    } else {
+        double e0, e1;
+        if (c[2] == 0.0) {
+            if (c[0] == 0.0) { e0 = 0.0; e1 = M_PI / 2.0; }
+            else { e0 = M_PI / 2.0; e1 = ::atan2(c[1], ::sqrt(c[0] * c[0] + c[2] * c[2])); }
+        } else {
+            e0 = ::atan2(c[0], c[2]);
+            e1 = ::atan2(c[1], ::sqrt(c[0] * c[0] + c[2] * c[2]));
+        }
+        ca = ::cos(e0); cb = ::cos(e1); sa = ::sin(e0); sb = ::sin(e1);
+    }
     RT[0] = ca;       RT[1] = -sa * sb; RT[2] = sa * cb;
     RT[3] = 0;        RT[4] = cb;       RT[5] = sb;
     RT[6] = -sa;      RT[7] = -ca * sb; RT[8] = ca * cb;
 }
 
-template <typename R>
-STD void cone_dir(R theta2, R phi, const R RT[9], R out[3]) {
-    R th = M<R>::sqrt_(theta2);
-    R st = M<R>::sin_(th);
-    R l0 = st * M<R>::cos_(phi), l1 = st * M<R>::sin_(phi), l2 = M<R>::cos_(th);
+// Direction at polar angle sqrt(theta2) and azimuth 2 pi u about the frame's
+// z axis.  FAST: float sin of the polar angle and sinpi/cospi of 2u, with
+// cos(theta) = 1 - 2 sin^2(theta/2) formed in double (no rounding near 1).
+template <bool FAST>
+STD void cone_dir(double theta2, double u, const double RT[9], double out[3]) {
+    double l0, l1, l2;
+    if constexpr (FAST) {
+        const float th = ::sqrtf((float)theta2);
+        const float st = ::sinf(th), sh = ::sinf(0.5f * th);
+        float sp, cp;
+        ::sincospif((float)(2.0 * u), &sp, &cp);
+        l0 = (double)st * (double)cp;
+        l1 = (double)st * (double)sp;
+        l2 = 1.0 - 2.0 * ((double)sh * (double)sh);
+    } else {
+        const double phi = u * 2.0 * 3.1415926535897932385;
+        const double th = ::sqrt(theta2);
+        const double st = ::sin(th);
+        l0 = st * ::cos(phi); l1 = st * ::sin(phi); l2 = ::cos(th);
+    }
     out[0] = RT[0] * l0 + RT[1] * l1 + RT[2] * l2;
     out[1] = RT[3] * l0 + RT[4] * l1 + RT[5] * l2;
     out[2] = RT[6] * l0 + RT[7] * l1 + RT[8] * l2;
@@ -910,94 +1045,45 @@ STD void cone_dir(R theta2, R phi, const R RT[9], R out[3]) {
 
 // First index i with table[i] >= theta, capped at n-1 (the linear walk of
 // coretrace/errors.cpp:167-169 on an ascending table).
-template <typename R>
-STD int table_index(const double *ang, int n, R theta, bool ascending) {
+STD int table_index(const double *ang, int n, double theta, bool ascending) {
     if (ascending) {
         int lo = 0, hi = n - 1;
         while (lo < hi) {
             int mid = (lo + hi) >> 1;
-            if ((R)ang[mid] < theta) lo = mid + 1; else hi = mid;
+            if (ang[mid] < theta) lo = mid + 1; else hi = mid;
         }
         return lo;
     }
     int i = 0;
-    while (i < n - 1 && (R)ang[i] < theta) i++;
+    while (i < n - 1 && ang[i] < theta) i++;
     return i;
 }
 
-// Sunshape (source 1) or specularity (source 2) error (coretrace/errors.cpp:60-226).
-template <typename R>
-STD void errors(Rng<R> &rng, const R cin[3], int source, const DevSun &sun, const double *aux, int interaction,
-                const DevFace &f, R cout[3], const R DF[3]) {
-    R RT[9];
-    ray_frame<R>(cin, RT);
-    int dist = (source == 1) ? sun.shape : f.dist;
-    R delop = (source == 1) ? (R)(sun.sigma / 1000.0) : (R)(f.spec / 1000.0);
-    unsigned maxcall = 0;
-    R theta2 = 0;
-    for (;;) {
-        if (dist == 'g' || dist == 'G') {
-            R tx = rng.normal(R(0), delop);
-            R ty = rng.normal(R(0), delop);
-            theta2 = tx * tx + ty * ty;
-        } else if (dist == 'p' || dist == 'P') {
-            R tx, ty;
-            do {
-                tx = R(2) * delop * rng.uniform() - delop;
-                ty = R(2) * delop * rng.uniform() - delop;
-                theta2 = tx * tx + ty * ty;
-            } while (theta2 > delop * delop);
-        } else if (dist == 'd' || dist == 'D') {
-            for (;;) {
-                R u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
-                if (sunshape_d_try<R>(u0, u1, u2, sun, aux, &theta2)) break;
-            }
-            theta2 = theta2 / R(1000000.0);
-        } else if (dist == 'f' || dist == 'F') {
-            R a = M<R>::asin_(M<R>::sqrt_(rng.uniform()));
-            theta2 = a * a;
-        }
-        R phi = rng.uniform() * R(2) * (R)3.1415926535897932385;
-        cone_dir<R>(theta2, phi, RT, cout);
-        if (source == 2 && interaction == 2 && dot3<R>(cout, DF) < R(0) && maxcall++ < 50000u) continue;
-        break;
-    }
-}
-
-// Sunshape error when the polar angle was drawn ahead of time (the 'd'
-// rejection loop run cooperatively, see sunshape_d_kernel): frame, phi draw
-// and rotation exactly as errors() does after its loop.
-template <typename R>
-STD void sun_error_pre(Rng<R> &rng, const R cin[3], R theta2, R cout[3]) {
-    R RT[9];
-    ray_frame<R>(cin, RT);
-    R phi = rng.uniform() * R(2) * (R)3.1415926535897932385;
-    cone_dir<R>(theta2, phi, RT, cout);
-}
-
 // One try of the 'd' sunshape rejection loop of coretrace/errors.cpp:160-185
-// from its three uniforms.  A precomputed upper envelope of the profile
-// (per polar-angle bin, padded for rounding) rejects most tries without the
-// table walk; it only ever rejects what the exact test rejects.
-template <typename R>
-STD bool sunshape_d_try(R u0, R u1, R u2, const DevSun &sun, const double *aux, R *t2out) {
-    const R ma = (R)sun.max_angle, mi = (R)sun.max_intensity;
-    R tx = R(2) * ma * u0 - ma;
-    R ty = R(2) * ma * u1 - ma;
-    R t2 = tx * tx + ty * ty;
+// from its three uniform words; *t2out gets theta^2 in mrad^2.  A precomputed
+// upper envelope of the profile per polar-angle bin (padded by more than the
+// float error of the bin index) rejects most tries before the table walk; it
+// only ever rejects what the exact double test rejects.
+STD bool sunshape_d_try(uint32_t w0, uint32_t w1, uint32_t w2, const DevSun &sun, const double *aux,
+                        double *t2out) {
+    const double ma = sun.max_angle, mi = sun.max_intensity;
+    const double tx = 2.0 * ma * word_uniform(w0) - ma;
+    const double ty = 2.0 * ma * word_uniform(w1) - ma;
+    const double t2 = tx * tx + ty * ty;
+    const double u2 = word_uniform(w2);
     *t2out = t2;
-    R th = M<R>::sqrt_(t2);
     if (sun.env_n > 0 && !(t2 > ma * ma)) {
-        int b = (int)(th * (R)sun.env_scale);
+        int b = (int)(::sqrtf((float)t2) * (float)sun.env_scale);
         b = b < 0 ? 0 : (b >= sun.env_n ? sun.env_n - 1 : b);
-        if (u2 > (R)aux[sun.env + b]) return false;
+        if (u2 > aux[sun.env + b]) return false;
     }
     const int n = sun.tab_n;
     const double *ang = aux + sun.tab, *inten = ang + n;
-    int i = table_index<R>(ang, n, th, sun.tab_ascending != 0);
-    R stest;
-    if (i == 0) stest = (R)inten[0];
-    else stest = (R)inten[i - 1] + ((R)inten[i] - (R)inten[i - 1]) * (th - (R)ang[i - 1]) / ((R)ang[i] - (R)ang[i - 1]);
+    const double th = ::sqrt(t2);
+    const int i = table_index(ang, n, th, sun.tab_ascending != 0);
+    double stest;
+    if (i == 0) stest = inten[0];
+    else stest = inten[i - 1] + (inten[i] - inten[i - 1]) * (th - ang[i - 1]) / (ang[i] - ang[i - 1]);
     if (u2 > stest / mi) return false;
     if (t2 > ma * ma) return false;
     return true;
@@ -1006,9 +1092,8 @@ STD bool sunshape_d_try(R u0, R u1, R u2, const DevSun &sun, const double *aux, 
 // Tries 4q .. 4q+3 of the 'd' loop for the stream (seed, stage 0, sun_index)
 // whose loop starts at draw k0 (try j uses draws k0+3j .. k0+3j+2).  Returns
 // the first accepted try or -1; *theta2 is its (mrad^2) sample.
-template <typename R>
 STD int sunshape_d_group(uint32_t seed, uint64_t sun_index, uint32_t k0, uint32_t q, const DevSun &sun,
-                         const double *aux, R *theta2) {
+                         const double *aux, double *theta2) {
     const uint32_t kfirst = k0 + 12u * q;
     uint32_t cur = 0xffffffffu;
     uint32_t b[4];
@@ -1022,9 +1107,9 @@ STD int sunshape_d_group(uint32_t seed, uint64_t sun_index, uint32_t k0, uint32_
     };
     for (int t = 0; t < 4; ++t) {
         const uint32_t k = kfirst + 3u * (uint32_t)t;
-        R u0 = word_to_uniform<R>(word(k)), u1 = word_to_uniform<R>(word(k + 1)), u2 = word_to_uniform<R>(word(k + 2));
-        R t2;
-        if (sunshape_d_try<R>(u0, u1, u2, sun, aux, &t2)) {
+        const uint32_t w0 = word(k), w1 = word(k + 1), w2 = word(k + 2);
+        double t2;
+        if (sunshape_d_try(w0, w1, w2, sun, aux, &t2)) {
             *theta2 = t2;
             return t;
         }
@@ -1032,58 +1117,107 @@ STD int sunshape_d_group(uint32_t seed, uint64_t sun_index, uint32_t k0, uint32_
     return -1;
 }
 
+// Sunshape (source 1) or specularity (source 2) error (coretrace/errors.cpp:60-226).
+template <bool FAST>
+STD void errors(Rng &rng, const double cin[3], int source, const DevSun &sun, const double *aux, int interaction,
+                const DevFace &f, double cout[3], const double DF[3]) {
+    double RT[9];
+    ray_frame<FAST>(cin, RT);
+    const int dist = (source == 1) ? sun.shape : f.dist;
+    const double delop = (source == 1) ? sun.sigma / 1000.0 : f.spec / 1000.0;
+    unsigned maxcall = 0;
+    double theta2 = 0;
+    for (;;) {
+        if (dist == 'g' || dist == 'G') {
+            const double tx = rng.normal<FAST>(0.0, delop);
+            const double ty = rng.normal<FAST>(0.0, delop);
+            theta2 = tx * tx + ty * ty;
+        } else if (dist == 'p' || dist == 'P') {
+            double tx, ty;
+            do {
+                tx = 2.0 * delop * rng.uniform() - delop;
+                ty = 2.0 * delop * rng.uniform() - delop;
+                theta2 = tx * tx + ty * ty;
+            } while (theta2 > delop * delop);
+        } else if (dist == 'd' || dist == 'D') {
+            for (;;) {
+                const uint32_t w0 = rng.next_word(), w1 = rng.next_word(), w2 = rng.next_word();
+                if (sunshape_d_try(w0, w1, w2, sun, aux, &theta2)) break;
+            }
+            theta2 = theta2 / 1000000.0;
+        } else if (dist == 'f' || dist == 'F') {
+            const double u = rng.uniform();
+            double a;
+            if constexpr (FAST) a = (double)::asinf(::sqrtf((float)u));
+            else a = ::asin(::sqrt(u));
+            theta2 = a * a;
+        }
+        cone_dir<FAST>(theta2, rng.uniform(), RT, cout);
+        if (source == 2 && interaction == 2 && dot3<double>(cout, DF) < 0.0 && maxcall++ < 50000u) continue;
+        break;
+    }
+}
+
+// Sunshape error when the polar angle was drawn ahead of time (the 'd'
+// rejection loop run cooperatively, see sunshape_d_kernel): frame, phi draw
+// and rotation exactly as errors() does after its loop.
+template <bool FAST>
+STD void sun_error_pre(Rng &rng, const double cin[3], double theta2, double cout[3]) {
+    double RT[9];
+    ray_frame<FAST>(cin, RT);
+    cone_dir<FAST>(theta2, rng.uniform(), RT, cout);
+}
+
 // Slope error of the surface normal (coretrace/surfacenormalerrors.cpp:58-167).
-template <typename R>
-STD void normal_errors(Rng<R> &rng, const R cin[3], const DevFace &f, R cout[3]) {
-    R RT[9];
-    ray_frame<R>(cin, RT);
-    R delop = (R)(f.slope / 1000.0);
-    R theta2 = 0;
+template <bool FAST>
+STD void normal_errors(Rng &rng, const double cin[3], const DevFace &f, double cout[3]) {
+    double RT[9];
+    ray_frame<FAST>(cin, RT);
+    const double delop = f.slope / 1000.0;
+    double theta2 = 0;
     if (f.dist == 'g' || f.dist == 'G') {
-        R tx = rng.normal(R(0), delop);
-        R ty = rng.normal(R(0), delop);
+        const double tx = rng.normal<FAST>(0.0, delop);
+        const double ty = rng.normal<FAST>(0.0, delop);
         theta2 = tx * tx + ty * ty;
     } else if (f.dist == 'p' || f.dist == 'P') {
-        R tx, ty;
+        double tx, ty;
         do {
-            tx = R(2) * delop * rng.uniform() - delop;
-            ty = R(2) * delop * rng.uniform() - delop;
+            tx = 2.0 * delop * rng.uniform() - delop;
+            ty = 2.0 * delop * rng.uniform() - delop;
             theta2 = tx * tx + ty * ty;
         } while (theta2 > delop * delop);
     }
-    R phi = rng.uniform() * R(2) * (R)3.1415926535897932385;
-    cone_dir<R>(theta2, phi, RT, cout);
+    cone_dir<FAST>(theta2, rng.uniform(), RT, cout);
 }
 
 // Reflection / refraction (coretrace/interaction.cpp:58-201); cout is left
 // untouched when the refraction Newton loop does not converge.
-template <typename R>
-STD void interact(Rng<R> &rng, const R c[3], const R DF[3], int type, const DevFace &f, R cout[3]) {
+STD void interact(Rng &rng, const double c[3], const double DF[3], int type, const DevFace &f, double cout[3]) {
     if (type == 1) {
-        R n1 = (R)f.n_in, n2 = (R)f.n_out;
-        R mu = n1 / n2;
-        R D2 = dot3<R>(DF, DF);
-        R B = (mu * mu - R(1)) / D2;
-        R A = mu * dot3<R>(c, DF) / D2;
+        const double n1 = f.n_in, n2 = f.n_out;
+        const double mu = n1 / n2;
+        const double D2 = dot3<double>(DF, DF);
+        const double B = (mu * mu - 1.0) / D2;
+        const double A = mu * dot3<double>(c, DF) / D2;
         if (B > A * A) {
-            R a = dot3<R>(c, DF) / dot3<R>(DF, DF);
-            for (int i = 0; i < 3; i++) cout[i] = c[i] - R(2) * a * DF[i];
+            const double a = dot3<double>(c, DF) / dot3<double>(DF, DF);
+            for (int i = 0; i < 3; i++) cout[i] = c[i] - 2.0 * a * DF[i];
             return;
         }
-        R nn = M<R>::sqrt_(dot3<R>(DF, DF));
-        R u[3] = {-DF[0] / nn, -DF[1] / nn, -DF[2] / nn};
-        R ia = M<R>::acos_(dot3<R>(c, u));
-        R si = M<R>::sin_(ia), ci = M<R>::cos_(ia);
-        R st = n1 * si / n2;
-        R ct = M<R>::sqrt_(R(1) - st * st);
-        R rs = (n1 * ci - n2 * ct) / (n1 * ci + n2 * ct);
-        R rp = (n1 * ct - n2 * ci) / (n1 * ct + n2 * ci);
-        R rave = (rp * rp + rs * rs) / R(2);
+        const double nn = ::sqrt(dot3<double>(DF, DF));
+        const double u[3] = {-DF[0] / nn, -DF[1] / nn, -DF[2] / nn};
+        const double ia = ::acos(dot3<double>(c, u));
+        const double si = ::sin(ia), ci = ::cos(ia);
+        const double st = n1 * si / n2;
+        const double ct = ::sqrt(1.0 - st * st);
+        const double rs = (n1 * ci - n2 * ct) / (n1 * ci + n2 * ct);
+        const double rp = (n1 * ct - n2 * ci) / (n1 * ct + n2 * ci);
+        const double rave = (rp * rp + rs * rs) / 2.0;
         if (rave < rng.uniform()) {
-            R g = -B / (R(2) * A), g1 = 0;
+            double g = -B / (2.0 * A), g1 = 0;
             for (int i = 1; i < 10; i++) {
-                g1 = (g * g - B) / (R(2) * (g + A));
-                if (M<R>::fabs_(g - g1) < R(0.000005)) {
+                g1 = (g * g - B) / (2.0 * (g + A));
+                if (::fabs(g - g1) < 0.000005) {
                     for (int j = 0; j < 3; j++) cout[j] = mu * c[j] + g1 * DF[j];
                     return;
                 }
@@ -1091,34 +1225,33 @@ STD void interact(Rng<R> &rng, const R c[3], const R DF[3], int type, const DevF
             }
             return;
         }
-        R a = dot3<R>(c, DF) / dot3<R>(DF, DF);
-        for (int i = 0; i < 3; i++) cout[i] = c[i] - R(2) * a * DF[i];
+        const double a = dot3<double>(c, DF) / dot3<double>(DF, DF);
+        for (int i = 0; i < 3; i++) cout[i] = c[i] - 2.0 * a * DF[i];
         return;
     }
-    R a = dot3<R>(c, DF) / dot3<R>(DF, DF);
-    for (int i = 0; i < 3; i++) cout[i] = c[i] - R(2) * a * DF[i];
+    const double a = dot3<double>(c, DF) / dot3<double>(DF, DF);
+    for (int i = 0; i < 3; i++) cout[i] = c[i] - 2.0 * a * DF[i];
 }
 
 // Optical efficiency (coretrace/raytrace.cpp:885-946).
-template <typename R>
-STD R test_value(int interaction, const DevFace &f, const double *aux, const R c[3], const R DF[3]) {
+STD double test_value(int interaction, const DevFace &f, const double *aux, const double c[3], const double DF[3]) {
     int tab, n;
     if (interaction == 1) {
-        if (f.trans_n <= 0) return (R)f.transmissivity;
+        if (f.trans_n <= 0) return f.transmissivity;
         tab = f.trans_tab; n = f.trans_n;
     } else {
-        if (f.refl_n <= 0) return (R)f.reflectivity;
+        if (f.refl_n <= 0) return f.reflectivity;
         tab = f.refl_tab; n = f.refl_n;
     }
     const double *ang = aux + tab, *val = ang + n;
-    R nn = M<R>::sqrt_(dot3<R>(DF, DF));
-    R u[3] = {-DF[0] / nn, -DF[1] / nn, -DF[2] / nn};
-    R ia = M<R>::acos_(dot3<R>(c, u)) * R(1000);
-    if (ia >= (R)ang[n - 1]) return (R)val[n - 1];
+    const double nn = ::sqrt(dot3<double>(DF, DF));
+    const double u[3] = {-DF[0] / nn, -DF[1] / nn, -DF[2] / nn};
+    const double ia = ::acos(dot3<double>(c, u)) * 1000.0;
+    if (ia >= ang[n - 1]) return val[n - 1];
     int m = 0;
-    while ((R)ang[m] < ia) m++;
-    if (m == 0) return (R)val[0];
-    return ((R)val[m] + (R)val[m - 1]) / R(2);
+    while (ang[m] < ia) m++;
+    if (m == 0) return val[0];
+    return (val[m] + val[m - 1]) / 2.0;
 }
 
 }  // namespace st

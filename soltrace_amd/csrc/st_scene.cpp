@@ -824,8 +824,10 @@ bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
         for (int k = 1; k < ds.tab_n; k++)
             if (!(sun.angle[k] > sun.angle[k - 1])) ds.tab_ascending = 0;
         // Upper envelope of the interpolated profile / max intensity per
-        // polar-angle bin over [0, max angle], padded for rounding, so a
-        // uniform above it rejects without the table walk.
+        // polar-angle bin over [0, max angle], so a uniform above it rejects
+        // without the table walk.  Each bin is widened by 1e-3 of a bin on
+        // both sides: the kernels find the bin from a float square root,
+        // whose error is below 1e-4 of a bin.
         if (ds.tab_ascending && sun.max_angle > 0 && sun.max_intensity > 0) {
             const int NB = 256;
             const double ma = sun.max_angle, dth = ma / NB;
@@ -840,7 +842,7 @@ bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
             ds.env_n = NB;
             ds.env_scale = NB / ma;
             for (int b = 0; b < NB; b++) {
-                double lo = std::max(0.0, b * dth - 1e-4 * dth), hi = (b + 1) * dth + 1e-4 * dth;
+                double lo = std::max(0.0, b * dth - 1e-3 * dth), hi = (b + 1) * dth + 1e-3 * dth;
                 double m = std::max(prof(lo), prof(hi));
                 for (int k = 0; k < ds.tab_n; k++)
                     if (sun.angle[k] > lo && sun.angle[k] < hi) m = std::max(m, sun.intensity[k]);

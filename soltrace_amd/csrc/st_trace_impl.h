@@ -24,22 +24,21 @@ static constexpr int kBlock = 256;
 
 // ------------------------------------------------------ sun-ray source ----
 // coretrace/generateray.cpp:58-161 (the sun-plane draws are full 53-bit)
-template <typename R>
-STD void gen_sun_ray(Rng<R> &rng, const DevSun &sun, const DevStage &st, double pg[3], double cg[3],
+STD void gen_sun_ray(Rng &rng, const DevSun &sun, const DevStage &st, double pg[3], double cg[3],
                      double psun[2]) {
     if (sun.point_source) {
         pg[0] = sun.origin[0]; pg[1] = sun.origin[1]; pg[2] = sun.origin[2];
         for (int i = 0; i < 3; i++) {
-            double sgn = (rng.uniform_d() <= 0.5) ? -1.0 : 1.0;
-            cg[i] = sgn * rng.uniform_d();
+            double sgn = (rng.uniform() <= 0.5) ? -1.0 : 1.0;
+            cg[i] = sgn * rng.uniform();
         }
         double m = ::sqrt(cg[0] * cg[0] + cg[1] * cg[1] + cg[2] * cg[2]);
         cg[0] = cg[0] / m; cg[1] = cg[1] / m; cg[2] = cg[2] / m;
         psun[0] = psun[1] = 0.0;
         return;
     }
-    double x = sun.minx + (sun.maxx - sun.minx) * rng.uniform_d();
-    double y = sun.miny + (sun.maxy - sun.miny) * rng.uniform_d();
+    double x = sun.minx + (sun.maxx - sun.minx) * rng.uniform();
+    double y = sun.miny + (sun.maxy - sun.miny) * rng.uniform();
     psun[0] = x; psun[1] = y;
     const double ps[3] = {x, y, -10000.0}, cs[3] = {0.0, 0.0, 1.0};
     double pst[3], cst[3];
@@ -163,10 +162,10 @@ __global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
     bool hit = false;
     if (t < a.n) {
         const DevStage st = a.sv.stages[0];
-        Rng<R> rng;
+        Rng rng;
         rng.begin(a.seed, 0, a.s0 + t);
         double pg[3], cg[3], psun[2], ps[3], cs[3];
-        gen_sun_ray<R>(rng, a.sv.sun, st, pg, cg, psun);
+        gen_sun_ray(rng, a.sv.sun, st, pg, cg, psun);
         to_local(pg, cg, st.org, st.R, ps, cs);
         uint32_t tests = 0;
         hit = any_hit<R>(a.sv, st, ps, cs, (a.sv.use_grid && !a.sv.sun.point_source) ? psun : nullptr, &tests);
@@ -211,8 +210,8 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
         const int rl = nth_set_bit(U, rank);
         const uint32_t g = (uint32_t)__shfl((int)next_g, rl) + (uint32_t)rep;
         const uint64_t s = ((uint64_t)(uint32_t)__shfl((int)sid_hi, rl) << 32) | (uint32_t)__shfl((int)sid_lo, rl);
-        R t2 = R(0);
-        const int off = sunshape_d_group<R>(a.seed, s, a.k0, g, a.sun, a.aux, &t2);
+        double t2 = 0.0;
+        const int off = sunshape_d_group(a.seed, s, a.k0, g, a.sun, a.aux, &t2);
         const bool acc = off >= 0;
         const uint64_t accm = __ballot(acc);
         // lanes that served this lane's ray: rank r_own, r_own + m, ...
@@ -223,7 +222,7 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
             for (int q = r_own; q < 64; q += m) mine |= 1ull << q;
         const uint64_t hit = accm & mine;
         const int first = hit ? (__ffsll((unsigned long long)hit) - 1) : 0;
-        const double t2_first = (double)__shfl((double)t2, first);
+        const double t2_first = __shfl(t2, first);
         const int off_first = __shfl(off, first);
         bool resolved = false;
         if (owner) {
@@ -231,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
                 const uint32_t gg = next_g + (uint32_t)((first - r_own) / m);
                 const uint32_t jj = 4u * gg + (uint32_t)off_first;
                 SunPre p;
-                p.theta2 = (double)((R)t2_first / R(1000000.0));
+                p.theta2 = t2_first / 1000000.0;
                 p.kend = a.k0 + 3u * (jj + 1u);
                 p.ok = 1u;
                 a.out[myray] = p;
@@ -284,12 +283,12 @@ __global__ __launch_bounds__(kBlock) void trace_stage_kernel(TraceArgs a) {
     double pg[3], cg[3], psun[2];
     uint32_t raynum;
     uint64_t sun_index;
-    Rng<R> rng;
+    Rng rng;
     if (sun_stage) {
         sun_index = a.sun_index[t];
         raynum = a.raynum_base + (uint32_t)t + 1u;
         rng.begin(a.seed, 0, sun_index);
-        gen_sun_ray<R>(rng, sv.sun, st, pg, cg, psun);
+        gen_sun_ray(rng, sv.sun, st, pg, cg, psun);
     } else {
         const DevRay r = a.in_rays[t];
         raynum = r.raynum;
@@ -300,7 +299,8 @@ __global__ __launch_bounds__(kBlock) void trace_stage_kernel(TraceArgs a) {
     double ps[3], cs[3];
     to_local(pg, cg, st.org, st.R, ps, cs);
 
-    R cos_out_el[3] = {R(0), R(0), R(0)};
+    constexpr bool FAST = sizeof(R) == 4;
+    double cos_out_el[3] = {0.0, 0.0, 0.0};
     uint32_t nrec = 0;
     int mhc = 0;
     bool passed = false;
@@ -356,7 +356,19 @@ __global__ __launch_bounds__(kBlock) void trace_stage_kernel(TraceArgs a) {
             break;
         }
         const DevElem &e = sv.elems[st.elem0 + b.j];
-        double xd[3] = {(double)b.h.X[0], (double)b.h.X[1], (double)b.h.X[2]};
+        double xd[3], df[3];
+        int back;
+        if constexpr (FAST) {
+            double pe[3], ce[3];
+            to_local(ps, cs, e.org, e.R, pe, ce);
+            pe[0] = pe[0] + 1.0e-5 * ce[0];
+            pe[1] = pe[1] + 1.0e-5 * ce[1];
+            pe[2] = pe[2] + 1.0e-5 * ce[2];
+            hit_finish_d(e, sv.aux, pe, ce, b.path, xd, df, &back);
+        } else {
+            for (int i = 0; i < 3; i++) { xd[i] = b.h.X[i]; df[i] = b.h.DF[i]; }
+            back = b.h.back;
+        }
         flush();
         to_ref(xd, b.ce, e.org, e.R, pend.pos, pend.cos);
         pend.element = b.j + 1;
@@ -369,9 +381,9 @@ __global__ __launch_bounds__(kBlock) void trace_stage_kernel(TraceArgs a) {
             fabs_ = false;
         }
 
-        R pos_out[3] = {b.h.X[0], b.h.X[1], b.h.X[2]};
-        R cos_out[3];
-        const R cin0[3] = {(R)b.ce[0], (R)b.ce[1], (R)b.ce[2]};
+        // directions from here on are double in both precisions
+        double cos_out[3];
+        const double *cin0 = b.ce;
         if (st.virt) {
             for (int i = 0; i < 3; i++) { cos_out[i] = cin0[i]; cos_out_el[i] = cin0[i]; }
         } else {
@@ -379,44 +391,42 @@ __global__ __launch_bounds__(kBlock) void trace_stage_kernel(TraceArgs a) {
                 atomicOr(a.err_flags, 1u);
                 break;
             }
-            const DevFace &f = sv.faces[b.h.back ? e.face_back : e.face_front];
-            R tv = test_value<R>(e.interaction, f, sv.aux, cin0, b.h.DF);
+            const DevFace &f = sv.faces[back ? e.face_back : e.face_front];
+            const double tv = test_value(e.interaction, f, sv.aux, cin0, df);
             if (tv <= rng.uniform()) {
                 pend.element = -pend.element;
                 if (fhave && b.j + 1 == a.flux.elem) fabs_ = true;
                 break;
             }
-            R cin[3] = {cin0[0], cin0[1], cin0[2]};
-            R df[3] = {b.h.DF[0], b.h.DF[1], b.h.DF[2]};
+            double cin[3] = {cin0[0], cin0[1], cin0[2]};
             if (a.inc_sun && sun_stage && mhc == 1) {
-                R co[3];
+                double co[3];
                 if (a.sun_pre != nullptr && rng.k == 3u && a.sun_pre[t].ok) {
                     rng.seek(a.sun_pre[t].kend);
-                    sun_error_pre<R>(rng, cin, (R)a.sun_pre[t].theta2, co);
+                    sun_error_pre<FAST>(rng, cin, a.sun_pre[t].theta2, co);
                 } else {
-                    errors<R>(rng, cin, 1, sv.sun, sv.aux, e.interaction, f, co, df);
+                    errors<FAST>(rng, cin, 1, sv.sun, sv.aux, e.interaction, f, co, df);
                 }
                 cin[0] = co[0]; cin[1] = co[1]; cin[2] = co[2];
             }
             if (a.inc_err) {
-                R co[3];
-                normal_errors<R>(rng, df, f, co);
+                double co[3];
+                normal_errors<FAST>(rng, df, f, co);
                 df[0] = co[0]; df[1] = co[1]; df[2] = co[2];
             }
-            interact<R>(rng, cin, df, e.interaction, f, cos_out_el);
+            interact(rng, cin, df, e.interaction, f, cos_out_el);
             if (a.inc_err) {
-                R base[3];
+                double base[3];
                 const bool diffuse = (f.dist == 'f' || f.dist == 'F');
                 for (int i = 0; i < 3; i++) base[i] = diffuse ? df[i] : cos_out_el[i];
-                R co[3];
-                errors<R>(rng, base, 2, sv.sun, sv.aux, e.interaction, f, co, df);
+                double co[3];
+                errors<FAST>(rng, base, 2, sv.sun, sv.aux, e.interaction, f, co, df);
                 cos_out_el[0] = co[0]; cos_out_el[1] = co[1]; cos_out_el[2] = co[2];
             }
             for (int i = 0; i < 3; i++) cos_out[i] = cos_out_el[i];
         }
-        double po[3] = {(double)pos_out[0], (double)pos_out[1], (double)pos_out[2]};
-        double co[3] = {(double)cos_out[0], (double)cos_out[1], (double)cos_out[2]};
-        to_ref(po, co, e.org, e.R, ps, cs);
+        double po[3] = {xd[0], xd[1], xd[2]};
+        to_ref(po, cos_out, e.org, e.R, ps, cs);
         to_ref(ps, cs, st.org, st.R, pg, cg);
         if (!st.multihit) {
             passed = true;

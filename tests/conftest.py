@@ -34,6 +34,8 @@ def oracle_lib():
     import ctypes as C
     lib.dll.st_oracle_set_rng.argtypes = [C.c_void_p, C.c_int]
     lib.dll.st_oracle_set_rng.restype = C.c_int
+    lib.dll.st_oracle_set_torus_start.argtypes = [C.c_void_p, C.c_int]
+    lib.dll.st_oracle_set_torus_start.restype = C.c_int
     lib.dll.st_oracle_ray_sun_index.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
     lib.dll.st_oracle_ray_sun_index.restype = C.c_int
     lib.dll.st_oracle_flux.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,

@@ -23,7 +23,15 @@ def run_scene(lib, scene, seed, setup=None, after=None):
 
 
 def counter_mode(lib, ctx):
-    lib.dll.st_oracle_set_rng(ctx, 1)
+    """Oracle set-up for comparisons with the HIP library: the per-ray Philox
+    streams, and torus solves started where the library starts them for
+    origins far outside the torus."""
+    for name in ("st_oracle_set_rng", "st_oracle_set_torus_start"):
+        f = getattr(lib.dll, name)
+        f.argtypes = [C.c_void_p, C.c_int]
+        f.restype = C.c_int
+    assert lib.dll.st_oracle_set_rng(ctx, 1) == 1
+    assert lib.dll.st_oracle_set_torus_start(ctx, 1) == 1
 
 
 def oracle_sun_index(lib, ctx):

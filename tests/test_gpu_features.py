@@ -1,8 +1,10 @@
 """GPU parity over every in-scope surface, aperture and optics branch: the
 HIP library (FP64 path) against the oracle on the same per-ray draws, the
 exact-cull check (sun-plane grid + BVH against a full scan), and the FP32
-path against the oracle (hit points within 1e-4 m for rays whose hit
-sequence matches; almost all rays must match)."""
+path against the oracle.  The FP32 path draws the same errors (every
+accept/reject test is double), so all but a handful of rays keep their hit
+sequence -- a float intersection can still land on the other side of an
+aperture edge -- and those that do must have hit points within 1e-4 m."""
 import os
 import tempfile
 
@@ -11,7 +13,7 @@ import pytest
 
 from soltrace_amd import workloads
 from harness import (assert_records_match, counter_mode, hip_sun_index, oracle_sun_index, run_scene)
-from test_oracle_vs_ref import CASES, _mini, _write
+from test_oracle_vs_ref import CASES, _mini, _write, torus_target_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -23,11 +25,9 @@ def _setup(bits=64, cull=1):
     return f
 
 
-# The torus scene of CASES sends sun rays from 10 km straight at a torus;
-# there the reference's quartic is ill-conditioned (a 1-ulp change of the
-# origin moves hits by metres), so the GPU starts the quartic at the torus'
-# bounding sphere and is checked by the two torus tests below instead.
-GPU_CASES = sorted(k for k in CASES if k != "torus")
+# The torus scene of CASES sends sun rays from 10 km at a torus; the oracle
+# then starts the reference solve where the library does (counter_mode).
+GPU_CASES = sorted(CASES)
 
 
 @pytest.mark.parametrize("name", GPU_CASES)
@@ -82,14 +82,14 @@ def test_tower_fp32_close_to_oracle(hip_lib, oracle_lib):
     assert n_g > 0, m_g
     so, sg = _by_ray(r_o, si_o), _by_ray(r_g, si_g)
     common = set(so) & set(sg)
-    assert len(common) >= 0.995 * max(len(so), len(sg))
+    assert len(common) >= 0.999 * max(len(so), len(sg))
     same, worst = 0, 0.0
     for k in common:
         io, ig = so[k], sg[k]
         if len(io) == len(ig) and np.array_equal(r_o.element[io], r_g.element[ig]):
             same += 1
             worst = max(worst, float(np.abs(r_o.pos[io] - r_g.pos[ig]).max()))
-    assert same >= 0.995 * len(common), (same, len(common))
+    assert same >= 0.999 * len(common), (same, len(common))
     assert worst <= 1e-4, worst
 
 
@@ -121,44 +121,28 @@ def test_flux_map_fp64_equals_oracle_binning(hip_lib, oracle_lib):
     np.testing.assert_array_equal(fg, fo)
 
 
-def _torus_target_scene(n=4000):
-    from soltrace_amd.scene import Element, Optic, OpticFace, Scene, Stage
-    s = Scene(sun_shape="p", sun_sigma=4.65, sun_xyz=(0.0, 0.0, 10.0), raycount=n, maxcount=10 ** 7)
-    s.optics = [Optic("m", OpticFace(dist="g", refl=0.95, slope=3.0, spec=1.0), OpticFace(dist="g", refl=0.95)),
-                Optic("abs", OpticFace(dist="g", refl=0.3, slope=1.0), OpticFace(dist="g", refl=0.3, slope=1.0))]
-    mirror = Element(xyz=(0, 0, 0), aim=(0, 0, 1), aperture="r", ap_params=(2.5, 2.5), surface="f", optic="m")
-    tor = Element(xyz=(0, 0, 1.5), aim=(0, 0, -1), aperture="a", ap_params=(0, 0, 300.0), surface="d",
-                  surf_params=(0.8, 0.3), optic="abs")
-    s.stages = [Stage(elements=[mirror]), Stage(elements=[tor])]
-    return s
-
-
-def test_torus_target_fp64(hip_lib, oracle_lib):
-    # rays reach the torus from about a metre away: well conditioned, and the
-    # GPU does the reference's arithmetic unchanged
-    sc = _torus_target_scene()
+@pytest.mark.parametrize("bits", [64, 32])
+def test_torus_target(hip_lib, oracle_lib, bits):
+    sc = torus_target_scene()
     n_o, r_o, _, _ = run_scene(oracle_lib, sc, 17, setup=counter_mode)
-    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 17, setup=_setup(64))
+    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 17, setup=_setup(bits))
     assert n_g == n_o and (np.abs(r_o.element[r_o.stage == 2]) == 1).sum() > 500, (n_g, n_o, m_g)
-    assert_records_match(r_g, r_o, pos_tol=1e-6, cos_tol=1e-6)
+    tol = 1e-6 if bits == 64 else 1e-4
+    assert_records_match(r_g, r_o, pos_tol=tol, cos_tol=tol)
 
 
 @pytest.mark.parametrize("bits", [64, 32])
-def test_torus_far_field_hits_lie_on_surface(hip_lib, bits):
-    # sun rays from 10 km onto a torus at the stage origin (element frame =
-    # stage frame): every hit must satisfy the torus equation
+def test_torus_far_field(hip_lib, oracle_lib, bits):
+    # sun rays from 10 km onto a torus whose axis is 30 degrees off the sun
     from soltrace_amd.scene import Element, Optic, OpticFace, Scene, Stage
-    Ra, Rs = 0.8, 0.3
     s = Scene(sun_shape="p", sun_sigma=4.65, sun_xyz=(0.2, 0.1, 10.0), raycount=5000, maxcount=10 ** 7)
     s.optics = [Optic("m", OpticFace(refl=0.5), OpticFace(refl=0.5))]
-    tor = Element(xyz=(0, 0, 0), aim=(0, 0, 1), aperture="a", ap_params=(0, 0, 360.0), surface="d",
-                  surf_params=(Ra, Rs), optic="m")
+    tor = Element(xyz=(0, 0, 0), aim=(0.5, 0.2, 1), aperture="a", ap_params=(0, 0, 360.0), surface="d",
+                  surf_params=(0.8, 0.3), optic="m")
     wide = Element(xyz=(0, 0, -0.5), aim=(0, 0, 1), aperture="r", ap_params=(3.0, 3.0), surface="f", optic="m")
     s.stages = [Stage(elements=[tor, wide])]
-    n, r, m, _ = run_scene(hip_lib, s, 3, setup=_setup(bits))
-    assert n > 0, m
-    hit = np.abs(r.element) == 1
-    assert hit.sum() > 1000
-    x, y, z = r.pos[hit].T
-    resid = np.abs(np.sqrt((np.hypot(x, y) - Ra) ** 2 + (z - Rs) ** 2) - Rs)
-    assert resid.max() < (1e-9 if bits == 64 else 2e-5), resid.max()
+    n_o, r_o, _, _ = run_scene(oracle_lib, s, 3, setup=counter_mode)
+    n_g, r_g, m_g, _ = run_scene(hip_lib, s, 3, setup=_setup(bits))
+    assert n_g == n_o and (np.abs(r_o.element) == 1).sum() > 1000, (n_g, n_o, m_g)
+    tol = 1e-6 if bits == 64 else 1e-4
+    assert_records_match(r_g, r_o, pos_tol=tol, cos_tol=tol)

@@ -111,6 +111,32 @@ def test_feature_scenes_bitexact(ref_lib, oracle_lib, name):
     assert n > 0
 
 
+def torus_target_scene(n=4000):
+    """Reflected rays meet a tilted torus about 40 degrees off its axis from
+    1.5-2.5 m away: generic directions, all inside the torus start sphere."""
+    s = Scene(sun_shape="p", sun_sigma=4.65, sun_xyz=(0.0, 0.0, 10.0), raycount=n, maxcount=10 ** 7)
+    s.optics = [Optic("m", OpticFace(dist="g", refl=0.95, slope=3.0, spec=1.0), OpticFace(dist="g", refl=0.95)),
+                Optic("abs", OpticFace(dist="g", refl=0.3, slope=1.0), OpticFace(dist="g", refl=0.3, slope=1.0))]
+    mirror = Element(xyz=(0, 0, 0), aim=(0, 0, 1), aperture="r", ap_params=(2.5, 2.5), surface="f", optic="m")
+    tor = Element(xyz=(0, 0, 1.5), aim=(0.7, 0.3, 0.5), aperture="a", ap_params=(0, 0, 300.0), surface="d",
+                  surf_params=(0.8, 0.3), optic="abs")
+    s.stages = [Stage(elements=[mirror]), Stage(elements=[tor])]
+    return s
+
+
+def test_torus_target_bitexact_with_start_sphere(ref_lib, oracle_lib):
+    # the oracle's torus start point (what the HIP library does for far
+    # origins) must leave near origins on the reference arithmetic
+    def start(lib, ctx):
+        assert lib.dll.st_oracle_set_torus_start(ctx, 1) == 1
+    sc = torus_target_scene()
+    n_r, r_r, m_r, _ = run_scene(ref_lib, sc, 17)
+    n_o, r_o, m_o, _ = run_scene(oracle_lib, sc, 17, setup=start)
+    assert n_r == n_o > 0, (n_r, n_o, m_r, m_o)
+    assert (np.abs(r_r.element[r_r.stage == 2]) == 1).sum() > 500
+    _bitexact(r_o, r_r)
+
+
 def _write(tmpdir, name, text):
     p = os.path.join(tmpdir, name)
     with open(p, "w") as f:
