@@ -188,6 +188,21 @@ def cpu_baseline(scene_fn, seed, target_s=12.0):
                       f"(dynamic grouping on), {dt:.1f} s wall; single-thread calibration {rate1:.0f} hits/s"}
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size: what FETCH_SIZE and
+    WRITE_SIZE read for it gives the factor between those counters and
+    bytes moved."""
+    import torch
+    n = mib << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n, "launches": 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,7 +211,12 @@ def main():
     ap.add_argument("--seed", type=int, default=123)
     ap.add_argument("--hits", type=int, default=N_HITS)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 4 times (scale for the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the trace path shards by ray ranges, see DESIGN.md, but no multi-GPU run is built)")

@@ -101,11 +101,15 @@ struct Context {
     DevBuf d_elems, d_faces, d_aux, d_stages, d_bvh, d_bvh_items, d_grid_cells, d_grid_items;
     // run scratch
     DevBuf d_mask, d_popc, d_offs, d_scan_tmp, d_total, d_hits, d_rec_slots, d_rec_count, d_rec_offs, d_ovf_rec,
-        d_ovf_meta, d_ovf_count, d_out_slots, d_out_flag, d_out_offs, d_err, d_flux;
+        d_ovf_meta, d_ovf_count, d_out_slots, d_out_flag, d_out_offs, d_flux;
     DevBuf d_rays_a, d_rays_b, d_sunpre;
 
     // results
     std::vector<StageRecords> recs;
+    // record buffers of earlier runs, reused so a run allocates no device
+    // memory once the buffers have grown to the scene's size
+    std::vector<std::shared_ptr<DevBuf>> rec_pool;
+    std::vector<hipEvent_t> ev_pool;  // kernel timing events, reused
     uint64_t sun_ray_count = 0;
     bool host_valid = false;
     std::vector<DevRecord> host_recs;  // merged order
@@ -120,6 +124,7 @@ struct Context {
         HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     }
     ~Context() {
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void errlog(const char *fmt, ...) {
@@ -131,6 +136,8 @@ struct Context {
         messages.push_back(b);
     }
     void clear_results() {
+        for (auto &st : recs)
+            for (auto &c : st.chunks) rec_pool.push_back(c);
         recs.clear();
         host_valid = false;
         host_recs.clear();
@@ -247,10 +254,11 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     }
     const SceneView sv = scene_view;
 
-    uint32_t *err = (uint32_t *)d_err.ensure(sizeof(uint32_t));
+    // total[0..1]: scan totals; total[3]: error flags, read with the totals
+    uint32_t *total = (uint32_t *)d_total.ensure(4 * sizeof(uint32_t));
+    uint32_t *err = total + 3;
     launch_fill_u32(err, 1, 0u, s);
     uint32_t *ovf_count = (uint32_t *)d_ovf_count.ensure(sizeof(uint32_t));
-    uint32_t *total = (uint32_t *)d_total.ensure(4 * sizeof(uint32_t));
     const uint32_t ovf_cap = kMaxChunk / 2;
     d_ovf_rec.ensure((size_t)ovf_cap * sizeof(DevRecord));
     d_ovf_meta.ensure((size_t)ovf_cap * 2 * sizeof(uint32_t));
@@ -273,17 +281,21 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         launch_fill_u32(d_flux.as<uint32_t>(), (size_t)fs.nx * fs.ny, 0u, s);
     }
 
-    hipEvent_t ev0, ev1;
-    HC(hipEventCreate(&ev0));
-    HC(hipEventCreate(&ev1));
+    // kernel times: events recorded around the launches, read once the run
+    // has finished (no synchronisation per launch)
+    std::vector<std::pair<double *, size_t>> ev_marks;
+    size_t ev_used = 0;
     auto timed = [&](double &acc, auto &&fn) {
-        HC(hipEventRecord(ev0, s));
+        while (ev_used + 2 > ev_pool.size()) {
+            hipEvent_t e;
+            HC(hipEventCreate(&e));
+            ev_pool.push_back(e);
+        }
+        HC(hipEventRecord(ev_pool[ev_used], s));
         fn();
-        HC(hipEventRecord(ev1, s));
-        HC(hipEventSynchronize(ev1));
-        float ms = 0;
-        HC(hipEventElapsedTime(&ms, ev0, ev1));
-        acc += ms;
+        HC(hipEventRecord(ev_pool[ev_used + 1], s));
+        ev_marks.emplace_back(&acc, ev_used);
+        ev_used += 2;
     };
     auto read_u32 = [&](const uint32_t *p) {
         uint32_t v = 0;
@@ -291,8 +303,26 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         HC(hipStreamSynchronize(s));
         return v;
     };
-    auto check_err = [&]() -> bool {
-        uint32_t e = read_u32(err);
+    auto take_buffer = [&](size_t bytes) -> std::shared_ptr<DevBuf> {
+        std::shared_ptr<DevBuf> b;
+        for (size_t i = 0; i < rec_pool.size(); i++)
+            if (rec_pool[i]->cap >= bytes) {
+                b = rec_pool[i];
+                rec_pool.erase(rec_pool.begin() + (long)i);
+                break;
+            }
+        if (!b) {
+            if (!rec_pool.empty()) {
+                b = rec_pool.back();
+                rec_pool.pop_back();
+            } else {
+                b = std::make_shared<DevBuf>();
+            }
+        }
+        b->ensure(bytes);
+        return b;
+    };
+    auto check_err = [&](uint32_t e) -> bool {
         if (e & 1u) {
             errlog("Bad optical interaction type (stage)");
             return false;
@@ -345,6 +375,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         a.flux = fs;
         a.flux_bins = fs.active ? d_flux.as<uint32_t>() : nullptr;
         a.counters = ctrs ? ctrs + 5 * (1 + si) : nullptr;
+        a.generic = cs.stages[si].generic;
         a.sun_pre = nullptr;
         if (si == 0 && err_sun && (sv.sun.shape == 'd' || sv.sun.shape == 'D') && !sv.sun.point_source) {
             // tabulated sunshape: draw the polar angles cooperatively first
@@ -368,19 +399,17 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         launch_exclusive_scan(a.rec_count, d_rec_offs.as<uint32_t>(), (size_t)n, d_scan_tmp.as<uint32_t>(), total, s);
         launch_exclusive_scan(a.out_flag, d_out_offs.as<uint32_t>(), (size_t)n, d_scan_tmp.as<uint32_t>(), total + 1,
                               s);
-        uint32_t tot[2];
-        HC(hipMemcpyAsync(tot, total, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        uint32_t tot[4];
+        HC(hipMemcpyAsync(tot, total, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HC(hipStreamSynchronize(s));
-        if (!check_err()) return false;
-        auto chunk = std::make_shared<DevBuf>();
-        chunk->ensure((size_t)std::max<uint32_t>(1, tot[0]) * sizeof(DevRecord));
+        if (!check_err(tot[3])) return false;
+        std::shared_ptr<DevBuf> chunk = take_buffer((size_t)std::max<uint32_t>(1, tot[0]) * sizeof(DevRecord));
         launch_compact_records(a.rec_slots, a.rec_count, d_rec_offs.as<uint32_t>(), n, kSlots,
                                chunk->as<DevRecord>(), s);
         launch_compact_overflow(a.ovf_rec, a.ovf_meta, ovf_count, ovf_cap, d_rec_offs.as<uint32_t>(),
                                 chunk->as<DevRecord>(), s);
         if (tot[1] && next_rays)
             launch_compact_rays(a.out_slots, a.out_flag, d_out_offs.as<uint32_t>(), n, next_rays + *n_next, s);
-        HC(hipStreamSynchronize(s));
         recs[si].chunks.push_back(chunk);
         recs[si].counts.push_back(tot[0]);
         recs[si].total += tot[0];
@@ -419,6 +448,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         ma.n = (uint32_t)batch;
         ma.mask = mask;
         ma.counters = ctrs;
+        ma.generic = cs.stages[0].generic;
         timing.mask_launches++;
         timed(timing.mask_ms, [&]() { launch_sun_mask<R>(ma, s); });
         timing.sun_rays += batch;
@@ -493,8 +523,12 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         prev_has = n_cur > 0;
         if (prev_has) last_prev = last_raynum_of(cur, n_cur);
     }
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
+    HC(hipStreamSynchronize(s));
+    for (const auto &m : ev_marks) {
+        float ms = 0;
+        HC(hipEventElapsedTime(&ms, ev_pool[m.second], ev_pool[m.second + 1]));
+        *m.first += ms;
+    }
     if (fs.active) {
         flux_host.assign((size_t)fs.nx * fs.ny, 0u);
         HC(hipMemcpyAsync(flux_host.data(), d_flux.p, flux_host.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,

@@ -51,13 +51,13 @@ template <> struct M<float> {
 STD void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
-        uint32_t lo0 = 0xD2511F53u * c[0];
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
-        uint32_t lo1 = 0xCD9E8D57u * c[2];
-        uint32_t n0 = hi1 ^ c[1] ^ k0;
-        uint32_t n2 = hi0 ^ c[3] ^ k1;
-        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        // full 64-bit products: one v_mad_u64_u32 each instead of a
+        // separate high and low multiply
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[0] = n0; c[1] = (uint32_t)p1; c[2] = n2; c[3] = (uint32_t)p0;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
@@ -805,14 +805,29 @@ struct ElemHit {
 // Ray vs one element: Intersect + DetermineElementIntersectionNew
 // (coretrace/intersect.cpp:140-1011, determineelementintersectionnew.cpp:58-633).
 // P, D: the nudged ray in the element frame (double).  Returns true on an
-// intercept (before the caller's closest-hit / ZAperture rule).
-template <typename R>
+// intercept (before the caller's closest-hit / ZAperture rule).  G = false
+// is the variant for stages whose elements are all closed-form quadrics or
+// flats: the torus and Newton code is left out of the kernel.
+template <typename R, bool G>
 STD bool element_hit(const DevElem &e, const double *aux, const double P[3], const double D[3], ElemHit<R> &h) {
     const int s = e.surf;
     R X[3], DF[3];
     double path;
     bool ok;
-    if (s == SURF_TORUS) {
+    if (s == SURF_FLAT) {
+        // The reference runs its Newton iteration on F = z from the start
+        // plane z = 0 (coretrace/intersect.cpp:589-592, 945-1011), which
+        // stops at the first iterate: that iterate in closed form.
+        const double S0 = (0.0 - P[2] == 0.0) ? 0.0 : (0.0 - P[2]) / (D[2] + 0.00000000001);
+        X[0] = (R)(P[0] + D[0] * S0);
+        X[1] = (R)(P[1] + D[1] * S0);
+        X[2] = R(0);
+        DF[0] = R(0); DF[1] = R(0); DF[2] = R(1);
+        path = S0 + 0.0;
+        ok = !(path < 0.0);
+    } else if (!G && s > SURF_CYLINDER) {
+        return false;  // not in this variant (the host never sends it here)
+    } else if (s == SURF_TORUS) {
         // The reference quartic solve (root432) is ill-conditioned for an
         // origin kilometres away (a sun ray starts 10 km back): there a
         // 1-ulp change of the origin flips the outcome of a sizeable share
@@ -920,10 +935,11 @@ STD bool element_hit(const DevElem &e, const double *aux, const double P[3], con
 // heliostat normal (curvature radius ~1 km) by ~1e-9 rad; reflected towards
 // a receiver hundreds of metres away at grazing incidence that grows to
 // 1e-4 m.  P, D: the nudged ray in the element frame.
+template <bool G>
 STD void hit_finish_d(const DevElem &e, const double *aux, const double P[3], const double D[3], double path,
                       double X[3], double DF[3], int *back) {
     const int s = e.surf;
-    if (s == SURF_FLAT) {
+    if (s == SURF_FLAT || (!G && s > SURF_CYLINDER)) {
         // the reference's start plane z = 0, taken as the hit
         const double S0 = (0.0 - P[2] == 0.0) ? 0.0 : (0.0 - P[2]) / (D[2] + 0.00000000001);
         X[0] = P[0] + D[0] * S0; X[1] = P[1] + D[1] * S0; X[2] = 0.0;
@@ -934,23 +950,23 @@ STD void hit_finish_d(const DevElem &e, const double *aux, const double P[3], co
     } else if (s <= SURF_CYLINDER) {
         for (int i = 0; i < 3; i++) X[i] = P[i] + path * D[i];
         // F and grad F of the implicit quadric (cancellation-free forms)
-        double F, G[3];
+        double F, Gr[3];
         const double x = X[0], y = X[1], z = X[2];
         if (s == SURF_PARABOLA) {
             const double a2 = 2.0 / e.vcx, b2 = e.vcy > 0.0 ? 2.0 / e.vcy : 1e10;
             const double ky = (e.stype == 7) ? 0.0 : 1.0;
             F = x * x / a2 + ky * y * y / b2 - z;
-            G[0] = 2.0 * x / a2; G[1] = 2.0 * ky * y / b2; G[2] = -1.0;
+            Gr[0] = 2.0 * x / a2; Gr[1] = 2.0 * ky * y / b2; Gr[2] = -1.0;
         } else if (s == SURF_CONIC) {
             F = x * x + y * y + e.kappa * z * z - 2.0 * z / e.vcx;
-            G[0] = 2.0 * x; G[1] = 2.0 * y; G[2] = 2.0 * e.kappa * z - 2.0 / e.vcx;
+            Gr[0] = 2.0 * x; Gr[1] = 2.0 * y; Gr[2] = 2.0 * e.kappa * z - 2.0 / e.vcx;
         } else {
             const double zc = (s == SURF_SPHERE) ? 1.0 / e.vcx : 1.0 / e.curv_rev;
             const double ky = (s == SURF_CYLINDER || e.stype == 7) ? 0.0 : 1.0;
             F = x * x + ky * y * y + z * (z - 2.0 * zc);
-            G[0] = 2.0 * x; G[1] = 2.0 * ky * y; G[2] = 2.0 * (z - zc);
+            Gr[0] = 2.0 * x; Gr[1] = 2.0 * ky * y; Gr[2] = 2.0 * (z - zc);
         }
-        const double gd = G[0] * D[0] + G[1] * D[1] + G[2] * D[2];
+        const double gd = Gr[0] * D[0] + Gr[1] * D[1] + Gr[2] * D[2];
         if (gd != 0.0) {
             const double t = path - F / gd;
             for (int i = 0; i < 3; i++) X[i] = P[i] + t * D[i];
@@ -980,6 +996,14 @@ STD void hit_finish_d(const DevElem &e, const double *aux, const double P[3], co
 // atan2/sin/cos, and the sines and cosines of the small perturbation angles,
 // whose relative error is what matters, are float.
 
+// 1/sqrt(x) in double: the hardware estimate refined by two Newton steps.
+STD double rsqrt_d(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * (1.5 - 0.5 * x * y * y);
+    y = y * (1.5 - 0.5 * x * y * y);
+    return y;
+}
+
 // Rotation taking the local z axis onto direction c, stored local ->
 // reference (coretrace/errors.cpp:98-121, surfacenormalerrors.cpp:87-106:
 // Euler angles alpha = atan2(cx, cz), beta = atan2(cy, sqrt(cx^2 + cz^2)),
@@ -994,13 +1018,13 @@ STD void ray_frame(const double c[3], double RT[9]) {
         if (c[2] == 0.0) {
             if (c[0] == 0.0) { ca = 1.0; sa = 0.0; cb = 0.0; sb = 1.0; }
             else {
-                const double h = ::fabs(c[0]), in = 1.0 / ::sqrt(h * h + c[1] * c[1]);
+                const double h = ::fabs(c[0]), in = rsqrt_d(h * h + c[1] * c[1]);
                 ca = 0.0; sa = 1.0; cb = h * in; sb = c[1] * in;
             }
         } else {
             const double h2 = c[0] * c[0] + c[2] * c[2];
-            const double h = ::sqrt(h2), ih = 1.0 / h, in = 1.0 / ::sqrt(h2 + c[1] * c[1]);
-            ca = c[2] * ih; sa = c[0] * ih; cb = h * in; sb = c[1] * in;
+            const double ih = rsqrt_d(h2), in = rsqrt_d(h2 + c[1] * c[1]);
+            ca = c[2] * ih; sa = c[0] * ih; cb = h2 * ih * in; sb = c[1] * in;
         }
     } else {
         double e0, e1;
@@ -1011,7 +1035,8 @@ STD void ray_frame(const double c[3], double RT[9]) {
             e0 = ::atan2(c[0], c[2]);
             e1 = ::atan2(c[1], ::sqrt(c[0] * c[0] + c[2] * c[2]));
         }
-        ca = ::cos(e0); cb = ::cos(e1); sa = ::sin(e0); sb = ::sin(e1);
+        ::sincos(e0, &sa, &ca);
+        ::sincos(e1, &sb, &cb);
     }
     RT[0] = ca;       RT[1] = -sa * sb; RT[2] = sa * cb;
     RT[3] = 0;        RT[4] = cb;       RT[5] = sb;
@@ -1019,24 +1044,42 @@ STD void ray_frame(const double c[3], double RT[9]) {
 }
 
 // Direction at polar angle sqrt(theta2) and azimuth 2 pi u about the frame's
-// z axis.  FAST: float sin of the polar angle and sinpi/cospi of 2u, with
-// cos(theta) = 1 - 2 sin^2(theta/2) formed in double (no rounding near 1).
+// z axis.  FAST: sin(theta)/theta and (1 - cos theta)/theta^2 from their
+// Taylor series in theta^2 (truncation below 1e-9 for theta <= pi/2; larger
+// angles take sinf/cosf), cos(theta) = 1 - theta^2 (1 - cos)/theta^2 formed
+// in double, azimuth from sinpi/cospi of 2u.
 template <bool FAST>
 STD void cone_dir(double theta2, double u, const double RT[9], double out[3]) {
     double l0, l1, l2;
     if constexpr (FAST) {
-        const float th = ::sqrtf((float)theta2);
-        const float st = ::sinf(th), sh = ::sinf(0.5f * th);
+        const float t2 = (float)theta2;
+        const float th = ::sqrtf(t2);
+        float st, vc;  // sin(theta), 1 - cos(theta)
+        if (t2 <= 2.5f) {
+            const float sinc =
+                1.0f - t2 / 6.0f * (1.0f - t2 / 20.0f * (1.0f - t2 / 42.0f * (1.0f - t2 / 72.0f *
+                (1.0f - t2 / 110.0f * (1.0f - t2 / 156.0f)))));
+            const float vers =
+                0.5f * (1.0f - t2 / 12.0f * (1.0f - t2 / 30.0f * (1.0f - t2 / 56.0f * (1.0f - t2 / 90.0f *
+                (1.0f - t2 / 132.0f * (1.0f - t2 / 182.0f))))));
+            st = th * sinc;
+            vc = t2 * vers;
+        } else {
+            st = ::sinf(th);
+            vc = 1.0f - ::cosf(th);
+        }
         float sp, cp;
         ::sincospif((float)(2.0 * u), &sp, &cp);
-        l0 = (double)st * (double)cp;
-        l1 = (double)st * (double)sp;
-        l2 = 1.0 - 2.0 * ((double)sh * (double)sh);
+        l0 = (double)(st * cp);
+        l1 = (double)(st * sp);
+        l2 = 1.0 - (double)vc;
     } else {
         const double phi = u * 2.0 * 3.1415926535897932385;
         const double th = ::sqrt(theta2);
-        const double st = ::sin(th);
-        l0 = st * ::cos(phi); l1 = st * ::sin(phi); l2 = ::cos(th);
+        double st, ct, sp, cp;
+        ::sincos(th, &st, &ct);
+        ::sincos(phi, &sp, &cp);
+        l0 = st * cp; l1 = st * sp; l2 = ct;
     }
     out[0] = RT[0] * l0 + RT[1] * l1 + RT[2] * l2;
     out[1] = RT[3] * l0 + RT[4] * l1 + RT[5] * l2;
@@ -1059,24 +1102,33 @@ STD int table_index(const double *ang, int n, double theta, bool ascending) {
     return i;
 }
 
-// One try of the 'd' sunshape rejection loop of coretrace/errors.cpp:160-185
-// from its three uniform words; *t2out gets theta^2 in mrad^2.  A precomputed
-// upper envelope of the profile per polar-angle bin (padded by more than the
-// float error of the bin index) rejects most tries before the table walk; it
-// only ever rejects what the exact double test rejects.
-STD bool sunshape_d_try(uint32_t w0, uint32_t w1, uint32_t w2, const DevSun &sun, const double *aux,
-                        double *t2out) {
-    const double ma = sun.max_angle, mi = sun.max_intensity;
+// The 'd' sunshape rejection loop of coretrace/errors.cpp:160-185, split
+// in two.  A try proposes (tx, ty) uniform over the square of half-side
+// max_angle and accepts with probability profile(theta)/max_intensity, and
+// only inside the circle of radius max_angle.  sunshape_d_screen rejects on
+// that circle and on a precomputed upper envelope of the profile per
+// polar-angle bin (padded by more than the float error of the bin index),
+// so it only ever rejects what the exact test rejects; sunshape_d_exact is
+// the reference's table walk and test.  theta^2 (t2) is in mrad^2.
+STD bool sunshape_d_screen(uint32_t w0, uint32_t w1, uint32_t w2, const DevSun &sun, const double *aux, double *t2out,
+                           double *u2out) {
+    const double ma = sun.max_angle;
     const double tx = 2.0 * ma * word_uniform(w0) - ma;
     const double ty = 2.0 * ma * word_uniform(w1) - ma;
     const double t2 = tx * tx + ty * ty;
     const double u2 = word_uniform(w2);
     *t2out = t2;
-    if (sun.env_n > 0 && !(t2 > ma * ma)) {
+    *u2out = u2;
+    if (t2 > ma * ma) return false;
+    if (sun.env_n > 0) {
         int b = (int)(::sqrtf((float)t2) * (float)sun.env_scale);
         b = b < 0 ? 0 : (b >= sun.env_n ? sun.env_n - 1 : b);
         if (u2 > aux[sun.env + b]) return false;
     }
+    return true;
+}
+
+STD bool sunshape_d_exact(double t2, double u2, const DevSun &sun, const double *aux) {
     const int n = sun.tab_n;
     const double *ang = aux + sun.tab, *inten = ang + n;
     const double th = ::sqrt(t2);
@@ -1084,14 +1136,22 @@ STD bool sunshape_d_try(uint32_t w0, uint32_t w1, uint32_t w2, const DevSun &sun
     double stest;
     if (i == 0) stest = inten[0];
     else stest = inten[i - 1] + (inten[i] - inten[i - 1]) * (th - ang[i - 1]) / (ang[i] - ang[i - 1]);
-    if (u2 > stest / mi) return false;
-    if (t2 > ma * ma) return false;
+    if (u2 > stest / sun.max_intensity) return false;
+    if (t2 > sun.max_angle * sun.max_angle) return false;
     return true;
+}
+
+STD bool sunshape_d_try(uint32_t w0, uint32_t w1, uint32_t w2, const DevSun &sun, const double *aux,
+                        double *t2out) {
+    double u2;
+    return sunshape_d_screen(w0, w1, w2, sun, aux, t2out, &u2) && sunshape_d_exact(*t2out, u2, sun, aux);
 }
 
 // Tries 4q .. 4q+3 of the 'd' loop for the stream (seed, stage 0, sun_index)
 // whose loop starts at draw k0 (try j uses draws k0+3j .. k0+3j+2).  Returns
-// the first accepted try or -1; *theta2 is its (mrad^2) sample.
+// the first accepted try or -1; *theta2 is its (mrad^2) sample.  The four
+// screens run first and branch-free; the exact test then runs only on the
+// tries that survive them, in order.
 STD int sunshape_d_group(uint32_t seed, uint64_t sun_index, uint32_t k0, uint32_t q, const DevSun &sun,
                          const double *aux, double *theta2) {
     const uint32_t kfirst = k0 + 12u * q;
@@ -1105,11 +1165,20 @@ STD int sunshape_d_group(uint32_t seed, uint64_t sun_index, uint32_t k0, uint32_
         }
         return pick4(b, k & 3u);
     };
+    double t2s[4], u2s[4];
+    uint32_t cand = 0;
+#pragma unroll
     for (int t = 0; t < 4; ++t) {
         const uint32_t k = kfirst + 3u * (uint32_t)t;
         const uint32_t w0 = word(k), w1 = word(k + 1), w2 = word(k + 2);
-        double t2;
-        if (sunshape_d_try(w0, w1, w2, sun, aux, &t2)) {
+        if (sunshape_d_screen(w0, w1, w2, sun, aux, &t2s[t], &u2s[t])) cand |= 1u << t;
+    }
+    while (cand != 0u) {
+        const int t = __ffs(cand) - 1;
+        cand &= cand - 1u;
+        const double t2 = t == 0 ? t2s[0] : (t == 1 ? t2s[1] : (t == 2 ? t2s[2] : t2s[3]));
+        const double u2 = t == 0 ? u2s[0] : (t == 1 ? u2s[1] : (t == 2 ? u2s[2] : u2s[3]));
+        if (sunshape_d_exact(t2, u2, sun, aux)) {
             *theta2 = t2;
             return t;
         }
@@ -1117,18 +1186,27 @@ STD int sunshape_d_group(uint32_t seed, uint64_t sun_index, uint32_t k0, uint32_
     return -1;
 }
 
-// Sunshape (source 1) or specularity (source 2) error (coretrace/errors.cpp:60-226).
+// One angular perturbation of a direction: the sunshape error
+// (coretrace/errors.cpp:60-226, source 1), the slope error of the surface
+// normal (coretrace/surfacenormalerrors.cpp:58-167; slope = true, only 'g'
+// and 'p' perturb) or the specularity error (errors.cpp, source 2).  The
+// polar angle is drawn by the distribution `dist` with width `delop` (rad),
+// or taken from `pre` (has_pre) when the 'd' rejection loop already ran
+// elsewhere;
+// then the azimuth draw and the rotation about `base`.  With resample (the
+// specular step of a reflection) a direction into the surface DF is drawn
+// again, up to 50000 times (errors.cpp:206-216).
 template <bool FAST>
-STD void errors(Rng &rng, const double cin[3], int source, const DevSun &sun, const double *aux, int interaction,
-                const DevFace &f, double cout[3], const double DF[3]) {
+STD void perturb(Rng &rng, const double base[3], int dist, double delop, bool slope, const DevSun &sun,
+                 const double *aux, bool has_pre, double pre, bool resample, const double DF[3], double out[3]) {
     double RT[9];
-    ray_frame<FAST>(cin, RT);
-    const int dist = (source == 1) ? sun.shape : f.dist;
-    const double delop = (source == 1) ? sun.sigma / 1000.0 : f.spec / 1000.0;
+    ray_frame<FAST>(base, RT);
     unsigned maxcall = 0;
-    double theta2 = 0;
     for (;;) {
-        if (dist == 'g' || dist == 'G') {
+        double theta2 = 0.0;
+        if (has_pre) {
+            theta2 = pre;
+        } else if (dist == 'g' || dist == 'G') {
             const double tx = rng.normal<FAST>(0.0, delop);
             const double ty = rng.normal<FAST>(0.0, delop);
             theta2 = tx * tx + ty * ty;
@@ -1139,55 +1217,23 @@ STD void errors(Rng &rng, const double cin[3], int source, const DevSun &sun, co
                 ty = 2.0 * delop * rng.uniform() - delop;
                 theta2 = tx * tx + ty * ty;
             } while (theta2 > delop * delop);
-        } else if (dist == 'd' || dist == 'D') {
+        } else if (!slope && (dist == 'd' || dist == 'D')) {
             for (;;) {
                 const uint32_t w0 = rng.next_word(), w1 = rng.next_word(), w2 = rng.next_word();
                 if (sunshape_d_try(w0, w1, w2, sun, aux, &theta2)) break;
             }
             theta2 = theta2 / 1000000.0;
-        } else if (dist == 'f' || dist == 'F') {
+        } else if (!slope && (dist == 'f' || dist == 'F')) {
             const double u = rng.uniform();
             double a;
             if constexpr (FAST) a = (double)::asinf(::sqrtf((float)u));
             else a = ::asin(::sqrt(u));
             theta2 = a * a;
         }
-        cone_dir<FAST>(theta2, rng.uniform(), RT, cout);
-        if (source == 2 && interaction == 2 && dot3<double>(cout, DF) < 0.0 && maxcall++ < 50000u) continue;
+        cone_dir<FAST>(theta2, rng.uniform(), RT, out);
+        if (resample && dot3<double>(out, DF) < 0.0 && maxcall++ < 50000u) continue;
         break;
     }
-}
-
-// Sunshape error when the polar angle was drawn ahead of time (the 'd'
-// rejection loop run cooperatively, see sunshape_d_kernel): frame, phi draw
-// and rotation exactly as errors() does after its loop.
-template <bool FAST>
-STD void sun_error_pre(Rng &rng, const double cin[3], double theta2, double cout[3]) {
-    double RT[9];
-    ray_frame<FAST>(cin, RT);
-    cone_dir<FAST>(theta2, rng.uniform(), RT, cout);
-}
-
-// Slope error of the surface normal (coretrace/surfacenormalerrors.cpp:58-167).
-template <bool FAST>
-STD void normal_errors(Rng &rng, const double cin[3], const DevFace &f, double cout[3]) {
-    double RT[9];
-    ray_frame<FAST>(cin, RT);
-    const double delop = f.slope / 1000.0;
-    double theta2 = 0;
-    if (f.dist == 'g' || f.dist == 'G') {
-        const double tx = rng.normal<FAST>(0.0, delop);
-        const double ty = rng.normal<FAST>(0.0, delop);
-        theta2 = tx * tx + ty * ty;
-    } else if (f.dist == 'p' || f.dist == 'P') {
-        double tx, ty;
-        do {
-            tx = 2.0 * delop * rng.uniform() - delop;
-            ty = 2.0 * delop * rng.uniform() - delop;
-            theta2 = tx * tx + ty * ty;
-        } while (theta2 > delop * delop);
-    }
-    cone_dir<FAST>(theta2, rng.uniform(), RT, cout);
 }
 
 // Reflection / refraction (coretrace/interaction.cpp:58-201); cout is left

@@ -72,6 +72,7 @@ struct TraceArgs {
     // event counters (nullptr: off): rays, element tests, BVH nodes,
     // uniforms drawn, records
     unsigned long long *counters;
+    int32_t generic;             // kernel variant: DevStage::generic of stage si
 };
 
 // Arguments of the sun-ray hit-mask launch.
@@ -82,6 +83,7 @@ struct MaskArgs {
     uint32_t n;                  // sun rays in this launch
     uint64_t *mask;              // one word per 64 sun rays
     unsigned long long *counters;  // as TraceArgs::counters (nullptr: off)
+    int32_t generic;             // kernel variant: DevStage::generic of stage 0
 };
 
 template <typename R> void launch_sun_mask(const MaskArgs &a, hipStream_t s);

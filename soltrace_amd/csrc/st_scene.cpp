@@ -689,6 +689,7 @@ bool compile_scene(SceneModel &m, bool use_cull, CompiledScene &out, std::string
                                   (hi[2] - lo[2]) * (hi[2] - lo[2]));
             for (int k = 0; k < 3; k++) d.bsphere_c[k] = c[k];
             d.bsphere_r = r;
+            if (d.surf != SURF_FLAT && d.surf > SURF_CYLINDER) ds.generic = 1;
             out.elems.push_back(d);
             if (e.enabled) {
                 // stage-frame box from the 8 corners of the element-frame box

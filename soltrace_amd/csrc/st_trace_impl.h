@@ -21,6 +21,7 @@
 namespace st {
 
 static constexpr int kBlock = 256;
+static constexpr int64_t kNoRecord = -(int64_t(1) << 62);  // a record lost to a full overflow buffer
 
 // ------------------------------------------------------ sun-ray source ----
 // coretrace/generateray.cpp:58-161 (the sun-plane draws are full 53-bit)
@@ -47,12 +48,12 @@ STD void gen_sun_ray(Rng &rng, const DevSun &sun, const DevStage &st, double pg[
 }
 
 // ------------------------------------------------------ closest hit -------
-template <typename R>
+// Only the winner's path and index are kept during the search; its hit
+// point and normal are recomputed once afterwards, which keeps the search
+// loop's live state small.
 struct Best {
     double path;
     int j;               // element index within the stage
-    ElemHit<R> h;
-    double ce[3];        // element-frame incoming direction
     uint32_t tests;      // element tests (event counter)
     uint32_t nodes;      // BVH nodes visited (event counter)
 };
@@ -60,103 +61,122 @@ struct Best {
 // Test element j; keep it when it is the closest accepted hit so far
 // (coretrace/raytrace.cpp:740-768; ties go to the lower element index, the
 // order a brute-force scan meets them).
-template <typename R>
-STD void consider(const SceneView &sv, const DevStage &st, int j, const double ps[3], const double cs[3],
-                  Best<R> &b) {
-    const DevElem &e = sv.elems[st.elem0 + j];
-    if (!e.enabled) return;
-    b.tests++;
-    double pe[3], ce[3];
+// The ray in element j's frame, nudged 1e-5 m along itself
+// (coretrace/raytrace.cpp:725-731).
+STD void element_ray(const DevElem &e, const double ps[3], const double cs[3], double pe[3], double ce[3]) {
     to_local(ps, cs, e.org, e.R, pe, ce);
     pe[0] = pe[0] + 1.0e-5 * ce[0];
     pe[1] = pe[1] + 1.0e-5 * ce[1];
     pe[2] = pe[2] + 1.0e-5 * ce[2];
+}
+
+template <typename R, bool G>
+STD void consider(const SceneView &sv, const DevStage &st, int j, const double ps[3], const double cs[3], Best &b) {
+    const DevElem &e = sv.elems[st.elem0 + j];
+    if (!e.enabled) return;
+    b.tests++;
+    double pe[3], ce[3];
+    element_ray(e, ps, cs, pe, ce);
     ElemHit<R> h;
-    if (!element_hit<R>(e, sv.aux, pe, ce, h)) return;
+    if (!element_hit<R, G>(e, sv.aux, pe, ce, h)) return;
     if (h.path < b.path || (h.path == b.path && j < b.j)) {
         if (!e.zap_filter || (double)h.X[2] <= e.zap) {
             b.path = h.path;
             b.j = j;
-            b.h = h;
-            b.ce[0] = ce[0]; b.ce[1] = ce[1]; b.ce[2] = ce[2];
         }
     }
 }
 
 // Candidates: the sun-plane grid cell for a first hit of a sun ray, the
-// stage BVH when present, every element otherwise.
-template <typename R>
-STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3], const double cs[3],
-                     const double *psun, Best<R> &b) {
+// stage BVH when present, every element otherwise.  One loop feeds them all
+// to a single consider() so the element code is instantiated once.
+// grid: the candidates are the sun-plane grid cell of (sx, sy).
+template <typename R, bool G>
+STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3], const double cs[3], bool grid,
+                     double sx, double sy, Best &b) {
     b.path = 1e99;
     b.j = 0x7fffffff;
-    if (psun != nullptr) {
+    const int32_t *items = nullptr;  // nullptr: the item is the element index
+    int k = 0, kend = 0;
+    bool bvh = false;
+    if (grid) {
         const DevSunGrid &g = sv.grid;
-        int cx = (int)((psun[0] - g.x0) * g.inv_dx);
-        int cy = (int)((psun[1] - g.y0) * g.inv_dy);
+        int cx = (int)((sx - g.x0) * g.inv_dx);
+        int cy = (int)((sy - g.y0) * g.inv_dy);
         cx = cx < 0 ? 0 : (cx >= g.nx ? g.nx - 1 : cx);
         cy = cy < 0 ? 0 : (cy >= g.ny ? g.ny - 1 : cy);
-        int c = cy * g.nx + cx;
-        int a = sv.grid_cells[c], z = sv.grid_cells[c + 1];
-        for (int k = a; k < z; ++k) consider<R>(sv, st, sv.grid_items[k], ps, cs, b);
-        return;
+        const int c = cy * g.nx + cx;
+        items = sv.grid_items;
+        k = sv.grid_cells[c];
+        kend = sv.grid_cells[c + 1];
+    } else if (st.nbvh > 0) {
+        bvh = true;
+        items = sv.bvh_items;
+    } else {
+        kend = st.nelem;
     }
-    if (st.nbvh > 0) {
-        // stack traversal; boxes are padded outward so a float slab test
-        // never drops a box the exact ray touches
-        const DevBvhNode *nodes = sv.bvh + st.bvh0;
-        float o[3] = {(float)ps[0], (float)ps[1], (float)ps[2]};
-        float inv[3];
-        for (int i = 0; i < 3; i++) {
-            float d = (float)cs[i];
-            inv[i] = (d == 0.0f) ? 3.0e38f : 1.0f / d;
+    // stackless depth-first BVH walk: descend on a box hit, else jump to the
+    // node's skip index (no per-lane stack, so no scratch memory); boxes are
+    // padded outward so a float slab test never drops a box the ray touches
+    const DevBvhNode *nodes = sv.bvh + st.bvh0;
+    float o[3], inv[3];
+    for (int q = 0; q < 3; q++) {
+        o[q] = (float)ps[q];
+        const float d = (float)cs[q];
+        inv[q] = (d == 0.0f) ? 3.0e38f : 1.0f / d;
+    }
+    int i = 0;
+    for (;;) {
+        if (k < kend) {
+            const int j = items != nullptr ? items[k] : k;
+            ++k;
+            consider<R, G>(sv, st, j, ps, cs, b);
+            continue;
         }
-        // stackless depth-first walk: descend on a box hit, else jump to the
-        // node's skip index (no per-lane stack, so no scratch memory)
-        int i = 0;
+        if (!bvh) break;
+        bool leaf = false;
         while (i < st.nbvh) {
             const DevBvhNode nd = nodes[i];
             b.nodes++;
             float t0 = 0.0f, t1 = 3.0e38f;
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                float ta = (nd.lo[k] - o[k]) * inv[k];
-                float tb = (nd.hi[k] - o[k]) * inv[k];
+            for (int q = 0; q < 3; q++) {
+                const float ta = (nd.lo[q] - o[q]) * inv[q];
+                const float tb = (nd.hi[q] - o[q]) * inv[q];
                 t0 = fmaxf(t0, fminf(ta, tb));
                 t1 = fminf(t1, fmaxf(ta, tb));
             }
             const bool enter = (t0 <= t1) && !((double)t0 > b.path * 1.000001 + 1.0e-3);
             if (!enter) {
                 i = nd.skip;
-                continue;
-            }
-            if (nd.leaf) {
-                const int first = nd.leaf >> 3, cnt = nd.leaf & 7;
-                for (int k = 0; k < cnt; ++k) consider<R>(sv, st, sv.bvh_items[first + k], ps, cs, b);
+            } else if (nd.leaf) {
+                k = nd.leaf >> 3;
+                kend = k + (nd.leaf & 7);
                 i = nd.skip;
+                leaf = true;
+                break;
             } else {
                 i = i + 1;
             }
         }
-        return;
+        if (!leaf) break;
     }
-    for (int j = 0; j < st.nelem; ++j) consider<R>(sv, st, j, ps, cs, b);
 }
 
 // Any accepted hit at all (the first-hit test of a sun ray).
-template <typename R>
-STD bool any_hit(const SceneView &sv, const DevStage &st, const double ps[3], const double cs[3],
-                 const double *psun, uint32_t *tests) {
-    Best<R> b;
+template <typename R, bool G>
+STD bool any_hit(const SceneView &sv, const DevStage &st, const double ps[3], const double cs[3], bool grid,
+                 double sx, double sy, uint32_t *tests) {
+    Best b;
     b.tests = 0;
     b.nodes = 0;
-    closest_hit<R>(sv, st, ps, cs, psun, b);
+    closest_hit<R, G>(sv, st, ps, cs, grid, sx, sy, b);
     *tests += b.tests;
     return b.path < 1e99;
 }
 
 // ---------------------------------------------------------- sun mask ------
-template <typename R>
+template <typename R, bool G>
 __global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     bool hit = false;
@@ -168,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
         gen_sun_ray(rng, a.sv.sun, st, pg, cg, psun);
         to_local(pg, cg, st.org, st.R, ps, cs);
         uint32_t tests = 0;
-        hit = any_hit<R>(a.sv, st, ps, cs, (a.sv.use_grid && !a.sv.sun.point_source) ? psun : nullptr, &tests);
+        hit = any_hit<R, G>(a.sv, st, ps, cs, a.sv.use_grid && !a.sv.sun.point_source, psun[0], psun[1], &tests);
         if (a.counters) {
             atomicAdd(&a.counters[0], 1ull);
             atomicAdd(&a.counters[1], (unsigned long long)tests);
@@ -190,9 +210,22 @@ __global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
 // iteration next_j + l/m of that ray's loop; each ray keeps the smallest
 // accepted iteration.  The result is exactly what the serial loop of
 // coretrace/errors.cpp:160-185 returns on the same stream.
+// Position of set bit number n (from 0) of v, which has more than n set
+// bits: six halving steps on population counts.
 STD int nth_set_bit(uint64_t v, int n) {
-    for (int i = 0; i < n; ++i) v &= v - 1;
-    return __ffsll((unsigned long long)v) - 1;
+    int pos = 0;
+    int c = __popc((uint32_t)v);
+    if (n >= c) { n -= c; v >>= 32; pos += 32; }
+    c = __popc((uint32_t)v & 0xFFFFu);
+    if (n >= c) { n -= c; v >>= 16; pos += 16; }
+    c = __popc((uint32_t)v & 0xFFu);
+    if (n >= c) { n -= c; v >>= 8; pos += 8; }
+    c = __popc((uint32_t)v & 0xFu);
+    if (n >= c) { n -= c; v >>= 4; pos += 4; }
+    c = __popc((uint32_t)v & 0x3u);
+    if (n >= c) { n -= c; v >>= 2; pos += 2; }
+    if (n >= (int)(v & 1u)) pos += 1;
+    return pos;
 }
 
 template <typename R>
@@ -217,9 +250,9 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
         // lanes that served this lane's ray: rank r_own, r_own + m, ...
         const bool owner = (U >> lane) & 1ull;
         const int r_own = __popcll(U & ((1ull << lane) - 1ull));
-        uint64_t mine = 0;
-        if (owner)
-            for (int q = r_own; q < 64; q += m) mine |= 1ull << q;
+        uint64_t pat = 1ull;  // bits 0, m, 2m, ... below 64
+        for (int sh = m; sh < 64; sh <<= 1) pat |= pat << sh;
+        const uint64_t mine = owner ? (pat << r_own) : 0ull;
         const uint64_t hit = accm & mine;
         const int first = hit ? (__ffsll((unsigned long long)hit) - 1) : 0;
         const double t2_first = __shfl(t2, first);
@@ -272,70 +305,100 @@ STD void grid_bin(double x, double lo, double b, int n, int *g) {
     *g = k - 1;
 }
 
-template <typename R>
-__global__ __launch_bounds__(kBlock) void trace_stage_kernel(TraceArgs a) {
+// Flux-map bin of a record on the flux element, -1 outside the grid
+// (app/src/project.cpp:1358-1415: stage -> element frame, cylinder
+// unrolled, linear-scan bin search).
+STD int flux_bin(const FluxSpec &fl, const double pos[3], const double cos[3]) {
+    double pe[3], ce[3];
+    to_local(pos, cos, fl.org, fl.R, pe, ce);
+    double x = pe[0], y = pe[1], z = pe[2];
+    const double Rr = fl.radius;
+    if (fl.kind == 1) {
+        if (z <= Rr) x = Rr * ::asin(x / Rr);
+        else if (z > Rr) {
+            if (x < 0) x = -(M_PI * Rr / 2.0 + Rr * ::acos(::fabs(x) / Rr));
+            if (x >= 0) x = M_PI * Rr / 2.0 + Rr * ::acos(x / Rr);
+        }
+    } else if (fl.kind == 2) {
+        x = Rr * ::asin(x / Rr);
+    }
+    int gx, gy;
+    grid_bin(x, fl.xmin, fl.bx, fl.nx, &gx);
+    grid_bin(y, fl.ymin, fl.by, fl.ny, &gy);
+    return (gx >= 0 && gx < fl.nx && gy >= 0 && gy < fl.ny) ? gx * fl.ny + gy : -1;
+}
+
+template <typename R, bool G>
+__global__ __launch_bounds__(kBlock, G ? 1 : 2) void trace_stage_kernel(TraceArgs a) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= a.n_rays) return;
     const SceneView &sv = a.sv;
     const DevStage st = sv.stages[a.si];
     const bool sun_stage = (a.si == 0);
+    constexpr bool FAST = sizeof(R) == 4;
 
-    double pg[3], cg[3], psun[2];
+    double ps[3], cs[3], psun[2];  // ray in the stage frame
     uint32_t raynum;
     uint64_t sun_index;
     Rng rng;
-    if (sun_stage) {
-        sun_index = a.sun_index[t];
-        raynum = a.raynum_base + (uint32_t)t + 1u;
-        rng.begin(a.seed, 0, sun_index);
-        gen_sun_ray(rng, sv.sun, st, pg, cg, psun);
-    } else {
-        const DevRay r = a.in_rays[t];
-        raynum = r.raynum;
-        sun_index = r.sun_index;
-        for (int i = 0; i < 3; i++) { pg[i] = r.pos[i]; cg[i] = r.cos[i]; }
-        rng.begin(a.seed, (uint32_t)a.si, sun_index);
+    {
+        double pg[3], cg[3];
+        if (sun_stage) {
+            sun_index = a.sun_index[t];
+            raynum = a.raynum_base + (uint32_t)t + 1u;
+            rng.begin(a.seed, 0, sun_index);
+            gen_sun_ray(rng, sv.sun, st, pg, cg, psun);
+        } else {
+            const DevRay r = a.in_rays[t];
+            raynum = r.raynum;
+            sun_index = r.sun_index;
+            for (int i = 0; i < 3; i++) { pg[i] = r.pos[i]; cg[i] = r.cos[i]; }
+            rng.begin(a.seed, (uint32_t)a.si, sun_index);
+        }
+        to_local(pg, cg, st.org, st.R, ps, cs);
     }
-    double ps[3], cs[3];
-    to_local(pg, cg, st.org, st.R, ps, cs);
 
-    constexpr bool FAST = sizeof(R) == 4;
-    double cos_out_el[3] = {0.0, 0.0, 0.0};
+    double cos_out_el[3] = {0.0, 0.0, 0.0};  // CosRayOutElement, kept across hits
     uint32_t nrec = 0;
     int mhc = 0;
     bool passed = false;
-    DevRecord pend;
-    bool has_pend = false;
-    // flux: last record of this ray on the flux element
-    const bool flux_here = a.flux.active && a.flux.stage == a.si;
-    double fpos[3] = {0, 0, 0}, fcos[3] = {0, 0, 0};
-    bool fhave = false, fabs_ = false;
-
-    auto flush = [&]() {
-        if (!has_pend) return;
+    // records go out as they are made; an absorbed hit later rewrites the
+    // element field of its own record (slot index, or -1 - overflow index)
+    int64_t last_loc = 0;
+    auto emit = [&](const double pos[3], const double cos[3], int element) {
+        DevRecord r;
+        for (int i = 0; i < 3; i++) { r.pos[i] = pos[i]; r.cos[i] = cos[i]; }
+        r.element = element;
+        r.raynum = raynum;
         if (nrec < (uint32_t)a.slots) {
-            a.rec_slots[(size_t)t * a.slots + nrec] = pend;
+            last_loc = (int64_t)t * a.slots + nrec;
+            a.rec_slots[last_loc] = r;
         } else {
             uint32_t o = atomicAdd(a.ovf_count, 1u);
             if (o < a.ovf_cap) {
-                a.ovf_rec[o] = pend;
+                a.ovf_rec[o] = r;
                 a.ovf_meta[2 * o] = (uint32_t)t;
                 a.ovf_meta[2 * o + 1] = nrec;
+                last_loc = -1 - (int64_t)o;
             } else {
                 atomicOr(a.err_flags, 4u);
+                last_loc = kNoRecord;
             }
         }
         nrec++;
-        has_pend = false;
     };
+    // flux: bin of this ray's last record on the flux element
+    const bool flux_here = a.flux.active && a.flux.stage == a.si;
+    int fbin = -1;
+    bool fabs_ = false;
 
     uint32_t ev_tests = 0, ev_nodes = 0;
     for (;;) {
-        Best<R> b;
+        Best b;
         b.tests = 0;
         b.nodes = 0;
         const bool first_sun = sun_stage && mhc == 0 && sv.use_grid && !sv.sun.point_source;
-        closest_hit<R>(sv, st, ps, cs, first_sun ? psun : nullptr, b);
+        closest_hit<R, G>(sv, st, ps, cs, first_sun, psun[0], psun[1], b);
         ev_tests += b.tests;
         ev_nodes += b.nodes;
         if (!(b.path < 1e99)) {
@@ -348,92 +411,85 @@ __global__ __launch_bounds__(kBlock) void trace_stage_kernel(TraceArgs a) {
                 passed = true;
                 if (raynum == a.last_raynum || mhc > 0) break;
             }
-            flush();
-            for (int i = 0; i < 3; i++) { pend.pos[i] = ps[i]; pend.cos[i] = cs[i]; }
-            pend.element = 0;
-            pend.raynum = raynum;
-            has_pend = true;
+            emit(ps, cs, 0);
             break;
         }
         const DevElem &e = sv.elems[st.elem0 + b.j];
-        double xd[3], df[3];
+        double pe[3], ce[3], xd[3], df[3];
         int back;
+        element_ray(e, ps, cs, pe, ce);
         if constexpr (FAST) {
-            double pe[3], ce[3];
-            to_local(ps, cs, e.org, e.R, pe, ce);
-            pe[0] = pe[0] + 1.0e-5 * ce[0];
-            pe[1] = pe[1] + 1.0e-5 * ce[1];
-            pe[2] = pe[2] + 1.0e-5 * ce[2];
-            hit_finish_d(e, sv.aux, pe, ce, b.path, xd, df, &back);
+            hit_finish_d<G>(e, sv.aux, pe, ce, b.path, xd, df, &back);
         } else {
-            for (int i = 0; i < 3; i++) { xd[i] = b.h.X[i]; df[i] = b.h.DF[i]; }
-            back = b.h.back;
+            // the winner again: the same arithmetic gives the same point
+            ElemHit<R> h;
+            element_hit<R, G>(e, sv.aux, pe, ce, h);
+            for (int i = 0; i < 3; i++) { xd[i] = h.X[i]; df[i] = h.DF[i]; }
+            back = h.back;
         }
-        flush();
-        to_ref(xd, b.ce, e.org, e.R, pend.pos, pend.cos);
-        pend.element = b.j + 1;
-        pend.raynum = raynum;
-        has_pend = true;
+        {
+            double rp[3], rc[3];
+            to_ref(xd, ce, e.org, e.R, rp, rc);
+            emit(rp, rc, b.j + 1);
+            if (flux_here && b.j + 1 == a.flux.elem) {
+                fbin = flux_bin(a.flux, rp, rc);
+                fabs_ = false;
+            }
+        }
         mhc++;
-        if (flux_here && b.j + 1 == a.flux.elem) {
-            for (int i = 0; i < 3; i++) { fpos[i] = pend.pos[i]; fcos[i] = pend.cos[i]; }
-            fhave = true;
-            fabs_ = false;
-        }
 
         // directions from here on are double in both precisions
         double cos_out[3];
-        const double *cin0 = b.ce;
         if (st.virt) {
-            for (int i = 0; i < 3; i++) { cos_out[i] = cin0[i]; cos_out_el[i] = cin0[i]; }
+            for (int i = 0; i < 3; i++) { cos_out[i] = ce[i]; cos_out_el[i] = ce[i]; }
         } else {
             if (e.interaction != 1 && e.interaction != 2) {
                 atomicOr(a.err_flags, 1u);
                 break;
             }
             const DevFace &f = sv.faces[back ? e.face_back : e.face_front];
-            const double tv = test_value(e.interaction, f, sv.aux, cin0, df);
+            const double tv = test_value(e.interaction, f, sv.aux, ce, df);
             if (tv <= rng.uniform()) {
-                pend.element = -pend.element;
-                if (fhave && b.j + 1 == a.flux.elem) fabs_ = true;
+                if (last_loc >= 0) a.rec_slots[last_loc].element = -(b.j + 1);
+                else if (last_loc != kNoRecord) a.ovf_rec[-1 - last_loc].element = -(b.j + 1);
+                if (flux_here && b.j + 1 == a.flux.elem) fabs_ = true;
                 break;
             }
-            double cin[3] = {cin0[0], cin0[1], cin0[2]};
-            if (a.inc_sun && sun_stage && mhc == 1) {
-                double co[3];
-                if (a.sun_pre != nullptr && rng.k == 3u && a.sun_pre[t].ok) {
-                    rng.seek(a.sun_pre[t].kend);
-                    sun_error_pre<FAST>(rng, cin, a.sun_pre[t].theta2, co);
-                } else {
-                    errors<FAST>(rng, cin, 1, sv.sun, sv.aux, e.interaction, f, co, df);
+            // sunshape error (first hit of a sun ray), slope error, reflection
+            // or refraction, specularity error -- one perturb() call site
+            double cin[3] = {ce[0], ce[1], ce[2]};
+            const bool sun_err = a.inc_sun && sun_stage && mhc == 1;
+            double pre = 0.0;
+            const bool use_pre = sun_err && a.sun_pre != nullptr && rng.k == 3u && a.sun_pre[t].ok;
+            if (use_pre) {
+                pre = a.sun_pre[t].theta2;
+                rng.seek(a.sun_pre[t].kend);
+            }
+            const bool diffuse = (f.dist == 'f' || f.dist == 'F');
+            for (int step = 0; step < 3; ++step) {
+                if (step == 2) interact(rng, cin, df, e.interaction, f, cos_out_el);
+                if (step == 0 ? !sun_err : !a.inc_err) continue;
+                double bs[3], co[3];
+                for (int i = 0; i < 3; i++)
+                    bs[i] = step == 0 ? cin[i] : ((step == 1 || diffuse) ? df[i] : cos_out_el[i]);
+                const int dist = step == 0 ? sv.sun.shape : f.dist;
+                const double delop = (step == 0 ? sv.sun.sigma : (step == 1 ? f.slope : f.spec)) / 1000.0;
+                perturb<FAST>(rng, bs, dist, delop, step == 1, sv.sun, sv.aux, step == 0 && use_pre, pre,
+                              step == 2 && e.interaction == 2, df, co);
+                for (int i = 0; i < 3; i++) {
+                    if (step == 0) cin[i] = co[i];
+                    else if (step == 1) df[i] = co[i];
+                    else cos_out_el[i] = co[i];
                 }
-                cin[0] = co[0]; cin[1] = co[1]; cin[2] = co[2];
-            }
-            if (a.inc_err) {
-                double co[3];
-                normal_errors<FAST>(rng, df, f, co);
-                df[0] = co[0]; df[1] = co[1]; df[2] = co[2];
-            }
-            interact(rng, cin, df, e.interaction, f, cos_out_el);
-            if (a.inc_err) {
-                double base[3];
-                const bool diffuse = (f.dist == 'f' || f.dist == 'F');
-                for (int i = 0; i < 3; i++) base[i] = diffuse ? df[i] : cos_out_el[i];
-                double co[3];
-                errors<FAST>(rng, base, 2, sv.sun, sv.aux, e.interaction, f, co, df);
-                cos_out_el[0] = co[0]; cos_out_el[1] = co[1]; cos_out_el[2] = co[2];
             }
             for (int i = 0; i < 3; i++) cos_out[i] = cos_out_el[i];
         }
-        double po[3] = {xd[0], xd[1], xd[2]};
-        to_ref(po, cos_out, e.org, e.R, ps, cs);
-        to_ref(ps, cs, st.org, st.R, pg, cg);
+        to_ref(xd, cos_out, e.org, e.R, ps, cs);
         if (!st.multihit) {
             passed = true;
             break;
         }
     }
-    flush();
     a.rec_count[t] = nrec;
     a.out_flag[t] = passed ? 1u : 0u;
     if (a.counters) {
@@ -445,45 +501,30 @@ __global__ __launch_bounds__(kBlock) void trace_stage_kernel(TraceArgs a) {
     }
     if (passed) {
         DevRay r;
-        for (int i = 0; i < 3; i++) { r.pos[i] = pg[i]; r.cos[i] = cg[i]; }
-        r.raynum = raynum;
-        r.pad = 0;
-        r.sun_index = sun_index;
+        if (mhc == 0) {
+            r = a.in_rays[t];  // passed through untouched: hand on the ray as it came
+        } else {
+            to_ref(ps, cs, st.org, st.R, r.pos, r.cos);
+            r.raynum = raynum;
+            r.pad = 0;
+            r.sun_index = sun_index;
+        }
         a.out_slots[t] = r;
     }
-    if (flux_here && fhave && (!a.flux.absorbed_only || fabs_)) {
-        // app/src/project.cpp:1358-1415: stage -> element frame, unroll the
-        // cylinder, linear-scan bin search, integer counts
-        double pe[3], ce[3];
-        to_local(fpos, fcos, a.flux.org, a.flux.R, pe, ce);
-        double x = pe[0], y = pe[1], z = pe[2];
-        const double Rr = a.flux.radius;
-        if (a.flux.kind == 1) {
-            if (z <= Rr) x = Rr * ::asin(x / Rr);
-            else if (z > Rr) {
-                if (x < 0) x = -(M_PI * Rr / 2.0 + Rr * ::acos(::fabs(x) / Rr));
-                if (x >= 0) x = M_PI * Rr / 2.0 + Rr * ::acos(x / Rr);
-            }
-        } else if (a.flux.kind == 2) {
-            x = Rr * ::asin(x / Rr);
-        }
-        int gx, gy;
-        grid_bin(x, a.flux.xmin, a.flux.bx, a.flux.nx, &gx);
-        grid_bin(y, a.flux.ymin, a.flux.by, a.flux.ny, &gy);
-        if (gx >= 0 && gx < a.flux.nx && gy >= 0 && gy < a.flux.ny)
-            atomicAdd(&a.flux_bins[gx * a.flux.ny + gy], 1u);
-    }
+    if (fbin >= 0 && (!a.flux.absorbed_only || fabs_)) atomicAdd(&a.flux_bins[fbin], 1u);
 }
 
 template <typename R>
 void launch_sun_mask(const MaskArgs &a, hipStream_t s) {
     const uint32_t blocks = (a.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(sun_mask_kernel<R>, dim3(blocks), dim3(kBlock), 0, s, a);
+    if (a.generic) hipLaunchKernelGGL((sun_mask_kernel<R, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((sun_mask_kernel<R, false>), dim3(blocks), dim3(kBlock), 0, s, a);
 }
 template <typename R>
 void launch_trace_stage(const TraceArgs &a, hipStream_t s) {
     const int blocks = (a.n_rays + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(trace_stage_kernel<R>, dim3(blocks), dim3(kBlock), 0, s, a);
+    if (a.generic) hipLaunchKernelGGL((trace_stage_kernel<R, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((trace_stage_kernel<R, false>), dim3(blocks), dim3(kBlock), 0, s, a);
 }
 
 }  // namespace st

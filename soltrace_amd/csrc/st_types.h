@@ -63,7 +63,7 @@ struct DevStage {
     int32_t elem0, nelem;   // slice of the element table
     int32_t virt, multihit, tracethrough;
     int32_t bvh0, nbvh;     // slice of the BVH node table (0 nodes: brute force)
-    int32_t pad;
+    int32_t generic;        // 1: an element needs the torus or a Newton surface other than flat
 };
 
 // Sun (coretrace/types.h:195-221) with the sun frame and sun-plane box.
