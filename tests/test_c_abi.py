@@ -1,0 +1,64 @@
+"""The drop-in boundary, checked without a GPU: the shared library
+(coretrace_api.so, no lib prefix, as coretrace/CMakeLists.txt:89-94 names it)
+and the static libcoretrace.a export every function include/stapi.h and
+include/stapi_amd.h declare; stapi.h declares exactly the reference's 55
+entry points (coretrace/stapi.h:68-176); the shared library loads with
+ctypes.  No test here calls into the library."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SO = os.path.join(ROOT, "soltrace_amd", "coretrace_api.so")
+AR = os.path.join(ROOT, "soltrace_amd", "libcoretrace.a")
+REF_HEADER = "/root/reference/coretrace/stapi.h"
+
+
+def declared(path):
+    """Names of the functions a C header declares (comments stripped)."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return set(re.findall(r"\b(st_\w+)\s*\(", text))
+
+
+def defined_symbols(path, dynamic):
+    args = ["nm", "--defined-only"] + (["-D"] if dynamic else []) + [path]
+    out = subprocess.run(args, capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if re.search(r" T ", line)}
+
+
+STAPI = declared(os.path.join(ROOT, "include", "stapi.h"))
+EXT = declared(os.path.join(ROOT, "include", "stapi_amd.h"))
+
+
+def test_stapi_header_has_the_55_reference_functions():
+    assert len(STAPI) == 55, sorted(STAPI)
+    assert not (STAPI & EXT)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_HEADER), reason="reference sources not present")
+def test_stapi_names_equal_the_reference_header():
+    ref = declared(REF_HEADER)
+    assert STAPI == ref, (sorted(STAPI - ref), sorted(ref - STAPI))
+
+
+def test_shared_library_exports_every_declaration():
+    assert os.path.exists(SO), "build first: python -c 'import __graft_entry__ as g; g.build()'"
+    missing = (STAPI | EXT) - defined_symbols(SO, dynamic=True)
+    assert not missing, sorted(missing)
+
+
+def test_static_library_exports_every_declaration():
+    assert os.path.exists(AR)
+    missing = (STAPI | EXT) - defined_symbols(AR, dynamic=False)
+    assert not missing, sorted(missing)
+
+
+def test_shared_library_loads_and_resolves():
+    lib = ctypes.CDLL(SO)
+    for name in sorted(STAPI | EXT):
+        assert getattr(lib, name) is not None
