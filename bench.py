@@ -48,11 +48,22 @@ NX = NY = 100
 FLOPS = {
     "sun_ray": 110,        # sun-plane point, sun->stage->global->stage transforms, grid cell
     "stage_ray": 33,       # global -> stage transform of an entering ray
-    "test_sphere": 120,    # element transform + nudge + quadric roots + normal + aperture
-    "test_flat": 70,       # element transform + nudge + start plane + 1 Newton step + aperture
+    "test_sphere": 115,    # element transform + nudge + rebase + float quadric roots + aperture
+    "test_flat": 50,       # element transform + nudge + start-plane point + aperture
     "bvh_node": 12,        # float slab test
-    "uniform": 5,          # conversion + rejection-loop arithmetic per uniform drawn
-    "record": 250,         # record transform, efficiency, sun/slope/specular frames, reflection, transforms back
+    "uniform": 5,          # conversion + rejection-loop arithmetic per uniform drawn in the kernel
+    "record": 420,         # double hit finish, record transform, efficiency, 2-3 perturbations, reflection
+    "sampler_try": 14,     # sunshape screen: proposal, radius, bin, envelope test (Philox is integer work)
+}
+
+# HBM bytes per unit (ray) of the dominant kernels on this workload, from
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script, scaled
+# by what the counters read for a plain 1 GiB device copy (FETCH_SIZE x 2.0,
+# WRITE_SIZE x 1.0); profiles/hbm_traffic_tower1000.md has the runs.
+PMC_BYTES_PER_UNIT = {
+    "trace_stage0": 26.1 + 129.7,
+    "sunshape": 8.6 + 16.0,
+    "sun_mask": 0.09 + 0.12,
 }
 
 
@@ -111,7 +122,7 @@ def _timed_steps(lib, ctx, steps, warmup, seed, sync):
     return (time.perf_counter() - t0) / steps
 
 
-def _flop_model(scene, ev):
+def _flop_model(scene, ev, sampler_units):
     """Algorithmic flops of each kernel of one run from its event counts."""
     launches_mask = ev[0]
     launches_trace = ev[1:17]
@@ -131,6 +142,9 @@ def _flop_model(scene, ev):
               nodes * FLOPS["bvh_node"] + unif * FLOPS["uniform"] + recs * FLOPS["record"])
         out[f"trace_stage{s}"] = dict(launches=launches_trace[s], flops=fl, units=r, tests=tests,
                                       nodes=nodes, uniforms=unif, records=recs)
+    if len(ev) >= 104 and ev[102] > 0:
+        out["sunshape"] = dict(launches=launches_trace[0], flops=ev[102] * FLOPS["sampler_try"],
+                               units=sampler_units, tries_evaluated=ev[102], tries_needed=ev[103])
     return out
 
 
@@ -233,28 +247,30 @@ def main():
     c64 = _context(lib, scene, 64)
 
     t32 = _timed_steps(lib, c32, args.steps, args.warmup, args.seed, sync)
-    tm32 = _arr(lib, "st_amd_timing", c32, 52)
+    tm32 = _arr(lib, "st_amd_timing", c32, 53)
     f32 = _flux(lib, c32)
     t64 = _timed_steps(lib, c64, max(1, args.steps // 2), 1, args.seed, sync)
-    tm64 = _arr(lib, "st_amd_timing", c64, 52)
+    tm64 = _arr(lib, "st_amd_timing", c64, 53)
     f64 = _flux(lib, c64)
     nrmse = float(np.sqrt(np.mean((f32 - f64) ** 2)) / max(1.0, f64.max()))
 
     # event counts of one FP32 run (a separate, instrumented run)
     lib.dll.st_amd_count_events(c32, 1)
     _run(lib, c32, args.seed)
-    ev = _arr(lib, "st_amd_events", c32, 102)
+    ev = _arr(lib, "st_amd_events", c32, 104)
     lib.dll.st_amd_count_events(c32, 0)
-    model = _flop_model(scene, ev)
+    model = _flop_model(scene, ev, sampler_units=tm32[20])
     # dominant kernel of the timed FP32 runs
-    kms = {"sun_mask": tm32[0]}
+    kms = {"sun_mask": tm32[0], "sunshape": tm32[52]}
     for s in range(len(scene.stages)):
         kms[f"trace_stage{s}"] = tm32[1 + s]
-    dom = max(kms, key=kms.get)
+    dom = max((k for k in kms if k in model), key=kms.get)
     launches = max(1.0, model[dom]["launches"])
     avg_ms = kms[dom] / launches
     flops_per_launch = model[dom]["flops"] / launches
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    units_per_launch = model[dom]["units"] / launches
+    traffic = PMC_BYTES_PER_UNIT[dom] * units_per_launch if dom in PMC_BYTES_PER_UNIT else None
 
     hits_s = args.hits / t32
     line = {
@@ -285,13 +301,15 @@ def main():
         "kernel_ms": {k: round(v, 3) for k, v in kms.items()},
         "roofline": {
             "bound": "mfma",
-            "pipe": "fp32 vector ALU (the ray algebra is not a contraction)",
+            "pipe": "fp32 vector ALU (the ray algebra is not a contraction; packed FP32 is not used, "
+                    "so the reachable FP32 FMA rate equals the 78.6 TF FP64 rate)",
             "kernel": dom,
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_gbps": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
             "flops_per_launch": flops_per_launch,
             "avg_launch_ms": avg_ms,
             "events_per_run": {k: {kk: vv for kk, vv in v.items() if kk != "flops"} for k, v in model.items()},

@@ -12,10 +12,12 @@
 extern "C" {
 #endif
 
-/* Arithmetic of the bulk ray algebra: 64 (default, the reference's FP64
- * path) or 32 (FP32 bulk; frame transforms, sun-plane draws, the torus
- * quartic and Newton start planes stay FP64).  Returns 1, or -1 for other
- * values. */
+/* Arithmetic of the closest-hit search: 64 (default, the reference's FP64
+ * path) or 32 (FP32 search: BVH, element tests, Newton iterations and
+ * aperture tests in float; frame transforms, sun-plane draws, every
+ * accept/reject draw, the torus quartic, Newton start planes and the
+ * per-hit finish -- hit point, normal, error rotations, reflection -- stay
+ * FP64).  Returns 1, or -1 for other values. */
 int st_amd_set_precision(st_context_t pcxt, int bits);
 
 /* 1 (default): stage-1 sun rays use the sun-plane grid and other rays a
@@ -55,8 +57,10 @@ int st_amd_count_events(st_context_t pcxt, int on);
 
 /* Launch counts and event counters of the last run: [0] sun-mask launches,
  * [1..16] trace launches per stage, then five counters for the sun mask and
- * five per stage (rays, element tests, BVH nodes, uniforms drawn, records).
- * Writes min(n, 102) values; returns 102. */
+ * five per stage (rays, element tests, BVH nodes, uniforms drawn, records),
+ * then [102] tries the tabulated-sunshape sampler evaluated and [103] the
+ * tries the serial rejection loop would have made.  Writes min(n, 104)
+ * values; returns 104. */
 int st_amd_events(st_context_t pcxt, double *out, int n);
 
 #ifdef __cplusplus

@@ -80,6 +80,7 @@ struct Timing {
     uint64_t mask_launches = 0, trace_launches[kMaxStages] = {0};
     // event counters per kernel: [0] sun mask, [1 + stage] trace
     unsigned long long ev[1 + kMaxStages][5] = {{0}};
+    unsigned long long ev_sun[2] = {0, 0};  // sampler: tries evaluated, tries needed
 };
 
 struct Context {
@@ -272,8 +273,8 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
 
     unsigned long long *ctrs = nullptr;
     if (count_events) {
-        ctrs = (unsigned long long *)d_counters.ensure((1 + kMaxStages) * 5 * sizeof(unsigned long long));
-        launch_fill_u32((uint32_t *)ctrs, (1 + kMaxStages) * 10, 0u, s);
+        ctrs = (unsigned long long *)d_counters.ensure(((1 + kMaxStages) * 5 + 2) * sizeof(unsigned long long));
+        launch_fill_u32((uint32_t *)ctrs, ((1 + kMaxStages) * 5 + 2) * 2, 0u, s);
     }
     FluxSpec fs = flux;
     if (fs.active) {
@@ -388,6 +389,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             pa.n = n;
             pa.out = (SunPre *)d_sunpre.ensure((size_t)kMaxChunk * sizeof(SunPre));
             pa.err_flags = err;
+            pa.counters = ctrs ? ctrs + 5 * (1 + kMaxStages) : nullptr;
             timed(timing.sunpre_ms, [&]() { launch_sunshape_d<R>(pa, s); });
             a.sun_pre = pa.out;
         }
@@ -537,6 +539,8 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     }
     if (ctrs) {
         HC(hipMemcpyAsync(&timing.ev[0][0], ctrs, sizeof(timing.ev), hipMemcpyDeviceToHost, s));
+        HC(hipMemcpyAsync(timing.ev_sun, ctrs + 5 * (1 + kMaxStages), sizeof(timing.ev_sun), hipMemcpyDeviceToHost,
+                          s));
         HC(hipStreamSynchronize(s));
     }
     timing.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
@@ -1100,13 +1104,15 @@ STCORE_API int st_amd_events(st_context_t p, double *out, int n) {
     for (int i = 0; i < kMaxStages; i++) v.push_back((double)c->timing.trace_launches[i]);
     for (int k = 0; k < 1 + kMaxStages; k++)
         for (int j = 0; j < 5; j++) v.push_back((double)c->timing.ev[k][j]);
+    v.push_back((double)c->timing.ev_sun[0]);
+    v.push_back((double)c->timing.ev_sun[1]);
     int m = std::min<int>(n, (int)v.size());
     for (int i = 0; i < m; i++) out[i] = v[i];
     return (int)v.size();
 }
 // Timing of the last run: [0] mask ms, [1..kMaxStages] trace ms per stage,
 // [17] other ms, [18] total ms, [19] sun rays, [20..] rays per stage,
-// [36..] records per stage.
+// [36..] records per stage, [52] sampler ms.
 STCORE_API int st_amd_timing(st_context_t p, double *out, int n) {
     CTX(p, -1);
     std::vector<double> v;

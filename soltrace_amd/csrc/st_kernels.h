@@ -40,6 +40,9 @@ struct SunPreArgs {
     int32_t n;
     SunPre *out;
     uint32_t *err_flags;
+    // event counters (nullptr: off): tries evaluated, tries the serial
+    // loop would have made
+    unsigned long long *counters;
 };
 
 // Arguments of one stage-trace launch.

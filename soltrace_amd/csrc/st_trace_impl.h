@@ -237,6 +237,7 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
     const uint32_t sid_lo = (uint32_t)sid, sid_hi = (uint32_t)(sid >> 32);
     uint64_t U = __ballot(valid);
     uint32_t next_g = 0;  // next group of four tries of this lane's ray
+    uint32_t ev_eval = 0, ev_need = 0;
     for (int round = 0; U != 0; ++round) {
         const int m = __popcll(U);
         const int rank = lane % m, rep = lane / m;
@@ -245,6 +246,7 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
         const uint64_t s = ((uint64_t)(uint32_t)__shfl((int)sid_hi, rl) << 32) | (uint32_t)__shfl((int)sid_lo, rl);
         double t2 = 0.0;
         const int off = sunshape_d_group(a.seed, s, a.k0, g, a.sun, a.aux, &t2);
+        ev_eval += 4u;
         const bool acc = off >= 0;
         const uint64_t accm = __ballot(acc);
         // lanes that served this lane's ray: rank r_own, r_own + m, ...
@@ -268,6 +270,7 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
                 p.ok = 1u;
                 a.out[myray] = p;
                 resolved = true;
+                ev_need += jj + 1u;
             } else {
                 next_g += (uint32_t)__popcll(mine);
                 if (next_g > (1u << 24)) {  // no acceptance in 2^26 tries: give up on this ray
@@ -282,6 +285,10 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
             }
         }
         U &= ~__ballot(resolved);
+    }
+    if (a.counters) {
+        atomicAdd(&a.counters[0], (unsigned long long)ev_eval);
+        atomicAdd(&a.counters[1], (unsigned long long)ev_need);
     }
 }
 
@@ -392,7 +399,7 @@ __global__ __launch_bounds__(kBlock, G ? 1 : 2) void trace_stage_kernel(TraceArg
     int fbin = -1;
     bool fabs_ = false;
 
-    uint32_t ev_tests = 0, ev_nodes = 0;
+    uint32_t ev_tests = 0, ev_nodes = 0, skipped = 0;
     for (;;) {
         Best b;
         b.tests = 0;
@@ -463,6 +470,7 @@ __global__ __launch_bounds__(kBlock, G ? 1 : 2) void trace_stage_kernel(TraceArg
             const bool use_pre = sun_err && a.sun_pre != nullptr && rng.k == 3u && a.sun_pre[t].ok;
             if (use_pre) {
                 pre = a.sun_pre[t].theta2;
+                skipped = a.sun_pre[t].kend - rng.k;  // drawn by the sampler kernel
                 rng.seek(a.sun_pre[t].kend);
             }
             const bool diffuse = (f.dist == 'f' || f.dist == 'F');
@@ -496,7 +504,7 @@ __global__ __launch_bounds__(kBlock, G ? 1 : 2) void trace_stage_kernel(TraceArg
         atomicAdd(&a.counters[0], 1ull);
         atomicAdd(&a.counters[1], (unsigned long long)ev_tests);
         atomicAdd(&a.counters[2], (unsigned long long)ev_nodes);
-        atomicAdd(&a.counters[3], (unsigned long long)rng.k);
+        atomicAdd(&a.counters[3], (unsigned long long)(rng.k - skipped));
         atomicAdd(&a.counters[4], (unsigned long long)nrec);
     }
     if (passed) {

@@ -62,3 +62,35 @@ def test_shared_library_loads_and_resolves():
     lib = ctypes.CDLL(SO)
     for name in sorted(STAPI | EXT):
         assert getattr(lib, name) is not None
+
+
+APP = r"""
+#include <stdio.h>
+#include "stapi.h"
+#include "stapi_amd.h"
+int main(void) {
+    st_context_t c = st_create_context();
+    if (!c) return 1;
+    st_amd_set_precision(c, 32);
+    printf("%d\n", st_num_stages(c));
+    return st_free_context(c) != 1;
+}
+"""
+
+
+@pytest.mark.parametrize("kind", ["static", "shared"])
+def test_c_program_links_against_the_library(tmp_path, kind):
+    # what the GUI and strace do with the static coretrace library, and any
+    # C caller with the shared one (linking only: running needs a GPU)
+    import shutil
+    if shutil.which("gcc") is None or shutil.which("g++") is None:
+        pytest.skip("gcc/g++ not available")
+    src, obj, exe = tmp_path / "app.c", tmp_path / "app.o", tmp_path / "app"
+    src.write_text(APP)
+    subprocess.run(["gcc", "-c", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(obj)], check=True)
+    if kind == "static":
+        cmd = ["g++", str(obj), AR, "-L/opt/rocm/lib", "-lamdhip64", "-o", str(exe)]
+    else:
+        cmd = ["gcc", str(obj), SO, "-o", str(exe)]
+    subprocess.run(cmd, check=True)
+    assert exe.exists()
