@@ -93,11 +93,19 @@ def test_tower_fp32_close_to_oracle(hip_lib, oracle_lib):
     assert worst <= 1e-4, worst
 
 
-def test_flux_map_fp64_equals_oracle_binning(hip_lib, oracle_lib):
+@pytest.mark.parametrize("receiver", ["flat", "cylinder"])
+def test_flux_map_fp64_equals_oracle_binning(hip_lib, oracle_lib, receiver):
+    # flat: the tower receiver; cylinder: the trough's absorber tube, whose x
+    # is unrolled to arc length over the whole circumference
     import ctypes as C
-    sc = workloads.tower(n_helio=100, raycount=20000, r_max=500.0)
-    nx = ny = 40
-    half = 7.5
+    if receiver == "flat":
+        sc = workloads.tower(n_helio=100, raycount=20000, r_max=500.0)
+        nx = ny = 40
+        half = 7.5
+    else:
+        sc = workloads.trough(raycount=20000)
+        nx, ny = 50, 50
+        half = 6.0
 
     def gpu_setup(lib, ctx):
         _setup(64)(lib, ctx)

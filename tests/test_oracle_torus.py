@@ -1,0 +1,77 @@
+"""Why the HIP library starts a far-field torus solve at the element's
+bounding sphere: the reference's quartic solve (root432, restated bit for
+bit by the oracle) is ill-conditioned for an origin kilometres away.
+Moving the sun by one ulp -- a 1e-17 rad change of every sun ray --
+changes which element a share of the rays hits first, or whether they hit
+at all, when the torus is met from 10 km (11 of 5,000 rays here), while
+hits on a flat in the same scene, and torus hits from a metre away, move by
+rounding only.  CPU only: the oracle with per-ray streams
+(so one ray's outcome cannot shift another's draws), the reference's torus
+arithmetic unchanged, rays matched by sun-ray index."""
+import ctypes as C
+
+import numpy as np
+
+from soltrace_amd.scene import Element, Optic, OpticFace, Scene, Stage
+from harness import oracle_sun_index, run_scene
+from test_oracle_vs_ref import torus_target_scene
+
+
+def _per_ray(lib, ctx):
+    assert lib.dll.st_oracle_set_rng(ctx, 1) == 1
+
+
+def _far_scene(sun_z):
+    s = Scene(sun_shape="p", sun_sigma=4.65, sun_xyz=(0.2, 0.1, sun_z), raycount=5000, maxcount=10 ** 7,
+              sunshape_err=False, optics_err=False)
+    s.optics = [Optic("m", OpticFace(refl=0.5), OpticFace(refl=0.5))]
+    tor = Element(xyz=(0, 0, 0), aim=(0.5, 0.2, 1), aperture="a", ap_params=(0, 0, 360.0), surface="d",
+                  surf_params=(0.8, 0.3), optic="m")
+    wide = Element(xyz=(0, 0, -0.5), aim=(0, 0, 1), aperture="r", ap_params=(3.0, 3.0), surface="f", optic="m")
+    s.stages = [Stage(elements=[tor, wide])]
+    return s
+
+
+def _first_hits(res, sun_index):
+    """Element and position of each ray's first record, keyed by sun-ray index."""
+    out = {}
+    for i in range(len(res)):
+        k = int(sun_index[res.raynum[i] - 1])
+        if k not in out:
+            out[k] = (int(res.element[i]), res.pos[i])
+    return out
+
+
+def test_far_field_torus_is_ill_conditioned(oracle_lib):
+    z0 = 10.0
+    z1 = float(np.nextafter(z0, 11.0))
+    _, a, _, sa = run_scene(oracle_lib, _far_scene(z0), 5, setup=_per_ray, after=oracle_sun_index)
+    _, b, _, sb = run_scene(oracle_lib, _far_scene(z1), 5, setup=_per_ray, after=oracle_sun_index)
+    ha, hb = _first_hits(a, sa), _first_hits(b, sb)
+    keys = sorted(set(ha) & set(hb))
+    n = len(keys)
+    ea = np.array([ha[k][0] for k in keys]); eb = np.array([hb[k][0] for k in keys])
+    pa = np.array([ha[k][1] for k in keys]); pb = np.array([hb[k][1] for k in keys])
+    flat = (np.abs(ea) == 2) & (np.abs(eb) == 2)
+    torus = (np.abs(ea) == 1) & (np.abs(eb) == 1)
+    assert flat.sum() > 200 and torus.sum() > 200
+    # the flat: rounding only
+    assert np.abs(pa[flat] - pb[flat]).max() < 1e-9
+    # the torus: outcomes change (first element, hit or miss, or a hit moved
+    # by more than a millimetre)
+    changed = (np.abs(pa[torus] - pb[torus]).max(axis=1) > 1e-3).sum() + (ea != eb).sum() + \
+        len(set(ha) ^ set(hb))
+    assert changed >= 5, (changed, n)
+
+
+def test_near_field_torus_is_well_conditioned(oracle_lib):
+    sa = torus_target_scene(2000)
+    sb = torus_target_scene(2000)
+    sb.sun_xyz = (0.0, 0.0, float(np.nextafter(10.0, 11.0)))
+    _, a, _, _ = run_scene(oracle_lib, sa, 17)
+    _, b, _, _ = run_scene(oracle_lib, sb, 17)
+    assert len(a) == len(b)
+    np.testing.assert_array_equal(a.element, b.element)
+    tor = (a.stage == 2) & (np.abs(a.element) == 1)
+    assert tor.sum() > 200
+    assert np.abs(a.pos[tor] - b.pos[tor]).max() < 1e-9
