@@ -24,7 +24,21 @@ static constexpr int kBlock = 256;
 static constexpr int64_t kNoRecord = -(int64_t(1) << 62);  // a record lost to a full overflow buffer
 
 // ------------------------------------------------------ sun-ray source ----
-// coretrace/generateray.cpp:58-161 (the sun-plane draws are full 53-bit)
+// coretrace/generateray.cpp:58-161 (the sun-plane draws are full 53-bit):
+// the point on the sun plane, then the ray it starts in the global frame.
+STD void sun_plane_point(Rng &rng, const DevSun &sun, double psun[2]) {
+    psun[0] = sun.minx + (sun.maxx - sun.minx) * rng.uniform();
+    psun[1] = sun.miny + (sun.maxy - sun.miny) * rng.uniform();
+}
+
+STD void sun_ray_from_plane(const DevSun &sun, const DevStage &st, const double psun[2], double pg[3],
+                            double cg[3]) {
+    const double ps[3] = {psun[0], psun[1], -10000.0}, cs[3] = {0.0, 0.0, 1.0};
+    double pst[3], cst[3];
+    to_ref(ps, cs, sun.pos_sun_stage, sun.R, pst, cst);
+    to_ref(pst, cst, st.org, st.R, pg, cg);
+}
+
 STD void gen_sun_ray(Rng &rng, const DevSun &sun, const DevStage &st, double pg[3], double cg[3],
                      double psun[2]) {
     if (sun.point_source) {
@@ -38,13 +52,20 @@ STD void gen_sun_ray(Rng &rng, const DevSun &sun, const DevStage &st, double pg[
         psun[0] = psun[1] = 0.0;
         return;
     }
-    double x = sun.minx + (sun.maxx - sun.minx) * rng.uniform();
-    double y = sun.miny + (sun.maxy - sun.miny) * rng.uniform();
-    psun[0] = x; psun[1] = y;
-    const double ps[3] = {x, y, -10000.0}, cs[3] = {0.0, 0.0, 1.0};
-    double pst[3], cst[3];
-    to_ref(ps, cs, sun.pos_sun_stage, sun.R, pst, cst);
-    to_ref(pst, cst, st.org, st.R, pg, cg);
+    sun_plane_point(rng, sun, psun);
+    sun_ray_from_plane(sun, st, psun, pg, cg);
+}
+
+// Candidate range of the sun-plane grid cell holding (sx, sy).
+STD void grid_cell_range(const SceneView &sv, double sx, double sy, int *k, int *kend) {
+    const DevSunGrid &g = sv.grid;
+    int cx = (int)((sx - g.x0) * g.inv_dx);
+    int cy = (int)((sy - g.y0) * g.inv_dy);
+    cx = cx < 0 ? 0 : (cx >= g.nx ? g.nx - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= g.ny ? g.ny - 1 : cy);
+    const int c = cy * g.nx + cx;
+    *k = sv.grid_cells[c];
+    *kend = sv.grid_cells[c + 1];
 }
 
 // ------------------------------------------------------ closest hit -------
@@ -100,15 +121,8 @@ STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3]
     int k = 0, kend = 0;
     bool bvh = false;
     if (grid) {
-        const DevSunGrid &g = sv.grid;
-        int cx = (int)((sx - g.x0) * g.inv_dx);
-        int cy = (int)((sy - g.y0) * g.inv_dy);
-        cx = cx < 0 ? 0 : (cx >= g.nx ? g.nx - 1 : cx);
-        cy = cy < 0 ? 0 : (cy >= g.ny ? g.ny - 1 : cy);
-        const int c = cy * g.nx + cx;
         items = sv.grid_items;
-        k = sv.grid_cells[c];
-        kend = sv.grid_cells[c + 1];
+        grid_cell_range(sv, sx, sy, &k, &kend);
     } else if (st.nbvh > 0) {
         bvh = true;
         items = sv.bvh_items;
@@ -181,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     bool hit = false;
     if (t < a.n) {
-        const DevStage st = a.sv.stages[0];
+        const DevStage &st = a.sv.stages[0];
         Rng rng;
         rng.begin(a.seed, 0, a.s0 + t);
         double pg[3], cg[3], psun[2], ps[3], cs[3];
@@ -336,7 +350,10 @@ STD int flux_bin(const FluxSpec &fl, const double pos[3], const double cos[3]) {
 }
 
 template <typename R, bool G>
-__global__ __launch_bounds__(kBlock, G ? 1 : 2) void trace_stage_kernel(TraceArgs a) {
+// Waves per SIMD: the generic variant needs the whole register file; the
+// basic FP32 variant runs 3 (a few spills are cheaper than the latency they
+// hide), the basic FP64 variant 2.
+__global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trace_stage_kernel(TraceArgs a) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= a.n_rays) return;
     const SceneView &sv = a.sv;
