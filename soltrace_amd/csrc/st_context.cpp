@@ -93,6 +93,8 @@ struct Context {
     DevBuf d_counters;
     std::vector<std::string> messages;
     hipStream_t stream = nullptr;
+    hipStream_t mask_stream = nullptr;  // next batch's sun mask, beside the current batch's trace
+    hipEvent_t ev_hits = nullptr, ev_ready = nullptr;
     int device = 0;
 
     CompiledScene cs;
@@ -101,6 +103,7 @@ struct Context {
     SceneView scene_view{};
     DevBuf d_elems, d_faces, d_aux, d_stages, d_bvh, d_bvh_items, d_grid_cells, d_grid_items;
     // run scratch
+    DevBuf d_mask_scan_tmp, d_mask_total;
     DevBuf d_mask, d_popc, d_offs, d_scan_tmp, d_total, d_hits, d_rec_slots, d_rec_count, d_rec_offs, d_ovf_rec,
         d_ovf_meta, d_ovf_count, d_out_slots, d_out_flag, d_out_offs, d_flux;
     DevBuf d_rays_a, d_rays_b, d_sunpre;
@@ -123,9 +126,15 @@ struct Context {
     Context() {
         HC(hipGetDevice(&device));
         HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HC(hipStreamCreateWithFlags(&mask_stream, hipStreamNonBlocking));
+        HC(hipEventCreateWithFlags(&ev_hits, hipEventDisableTiming));
+        HC(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
     }
     ~Context() {
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+        if (ev_hits) (void)hipEventDestroy(ev_hits);
+        if (ev_ready) (void)hipEventDestroy(ev_ready);
+        if (mask_stream) (void)hipStreamDestroy(mask_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void errlog(const char *fmt, ...) {
@@ -286,18 +295,19 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     // has finished (no synchronisation per launch)
     std::vector<std::pair<double *, size_t>> ev_marks;
     size_t ev_used = 0;
-    auto timed = [&](double &acc, auto &&fn) {
+    auto timed_on = [&](hipStream_t st, double &acc, auto &&fn) {
         while (ev_used + 2 > ev_pool.size()) {
             hipEvent_t e;
             HC(hipEventCreate(&e));
             ev_pool.push_back(e);
         }
-        HC(hipEventRecord(ev_pool[ev_used], s));
+        HC(hipEventRecord(ev_pool[ev_used], st));
         fn();
-        HC(hipEventRecord(ev_pool[ev_used + 1], s));
+        HC(hipEventRecord(ev_pool[ev_used + 1], st));
         ev_marks.emplace_back(&acc, ev_used);
         ev_used += 2;
     };
+    auto timed = [&](double &acc, auto &&fn) { timed_on(s, acc, fn); };
     auto read_u32 = [&](const uint32_t *p) {
         uint32_t v = 0;
         HC(hipMemcpyAsync(&v, p, sizeof v, hipMemcpyDeviceToHost, s));
@@ -423,68 +433,100 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     };
 
     // ------------------------------------------------------ stage 0 -------
+    // Batches of sun rays.  The sun mask of batch k+1 runs on the mask
+    // stream while batch k is sampled, traced and compacted on the trace
+    // stream; an event hands each hit list over.  The mask side has its own
+    // scan workspace and totals.
     DevRay *next = (DevRay *)d_rays_a.ensure((size_t)N * sizeof(DevRay));
     uint64_t *sun_index_all = (uint64_t *)d_hits.ensure((size_t)N * sizeof(uint64_t));
+    const uint64_t kMaxBatch = 1u << 25;
+    const size_t max_words = (size_t)kMaxBatch / 64;
+    uint64_t *mask = (uint64_t *)d_mask.ensure(max_words * sizeof(uint64_t));
+    uint32_t *popc = (uint32_t *)d_popc.ensure(max_words * sizeof(uint32_t));
+    uint32_t *offs = (uint32_t *)d_offs.ensure(max_words * sizeof(uint32_t));
+    uint32_t *mscan = (uint32_t *)d_mask_scan_tmp.ensure(scan_tmp_words(max_words) * sizeof(uint32_t));
+    uint32_t *mtotal = (uint32_t *)d_mask_total.ensure(4 * sizeof(uint32_t));
+    hipStream_t sm = mask_stream;
+    HC(hipEventRecord(ev_ready, s));  // counters and flags are cleared
+    HC(hipStreamWaitEvent(sm, ev_ready, 0));
     uint64_t n_next = 0;
     uint64_t hits = 0, s0 = 0;
     double rate = 0.0;  // hits per sun ray observed so far
-    while (hits < N) {
-        uint64_t want = N - hits;
+    auto plan = [&](uint64_t want) -> uint64_t {
         uint64_t batch = (rate > 0) ? (uint64_t)(want / rate * 1.05) + 4096 : std::min<uint64_t>(1u << 22, want * 4 + 4096);
-        batch = std::min<uint64_t>(batch, 1u << 25);
+        batch = std::min<uint64_t>(batch, kMaxBatch);
         batch = std::max<uint64_t>(batch, 4096);
         if (s0 + batch > maxrays) batch = maxrays - s0;
-        if (batch == 0) {
-            errlog("generated sun rays reached maximum count: %d", raymax);
-            return -1;
-        }
+        return batch;
+    };
+    auto launch_mask_batch = [&](uint64_t first, uint64_t batch) {
         const size_t nwords = (batch + 63) / 64;
-        uint64_t *mask = (uint64_t *)d_mask.ensure(nwords * sizeof(uint64_t));
-        uint32_t *popc = (uint32_t *)d_popc.ensure(nwords * sizeof(uint32_t));
-        uint32_t *offs = (uint32_t *)d_offs.ensure(nwords * sizeof(uint32_t));
-        d_scan_tmp.ensure(scan_tmp_words(nwords) * sizeof(uint32_t));
         MaskArgs ma{};
         ma.sv = sv;
         ma.seed = seed;
-        ma.s0 = s0;
+        ma.s0 = first;
         ma.n = (uint32_t)batch;
         ma.mask = mask;
         ma.counters = ctrs;
         ma.generic = cs.stages[0].generic;
         timing.mask_launches++;
-        timed(timing.mask_ms, [&]() { launch_sun_mask<R>(ma, s); });
+        timed_on(sm, timing.mask_ms, [&]() { launch_sun_mask<R>(ma, sm); });
         timing.sun_rays += batch;
-        launch_mask_popc(mask, popc, nwords, s);
-        launch_exclusive_scan(popc, offs, nwords, d_scan_tmp.as<uint32_t>(), total, s);
-        uint32_t h = read_u32(total);
-        uint32_t take = (uint32_t)std::min<uint64_t>(h, want);
+        launch_mask_popc(mask, popc, nwords, sm);
+        launch_exclusive_scan(popc, offs, nwords, mscan, mtotal, sm);
+    };
+    uint64_t batch = plan(N);
+    if (batch == 0) {
+        errlog("generated sun rays reached maximum count: %d", raymax);
+        return -1;
+    }
+    launch_mask_batch(s0, batch);
+    while (hits < N) {
+        const uint64_t want = N - hits;
+        const size_t nwords = (batch + 63) / 64;
+        uint32_t h = 0;
+        HC(hipMemcpyAsync(&h, mtotal, sizeof h, hipMemcpyDeviceToHost, sm));
+        HC(hipStreamSynchronize(sm));
+        const uint32_t take = (uint32_t)std::min<uint64_t>(h, want);
         uint64_t *hit_list = sun_index_all + hits;
-        if (take) launch_mask_compact(mask, offs, nwords, s0, take, hit_list, s);
+        if (take) launch_mask_compact(mask, offs, nwords, s0, take, hit_list, sm);
         if (take == want) {
             uint64_t last = 0;
-            HC(hipMemcpyAsync(&last, hit_list + take - 1, sizeof last, hipMemcpyDeviceToHost, s));
-            HC(hipStreamSynchronize(s));
+            HC(hipMemcpyAsync(&last, hit_list + take - 1, sizeof last, hipMemcpyDeviceToHost, sm));
+            HC(hipStreamSynchronize(sm));
             sun_ray_count = last + 1;
         } else {
             sun_ray_count = s0 + batch;
         }
-        for (uint32_t c = 0; c < take; c += kMaxChunk) {
-            int n = (int)std::min<uint32_t>(kMaxChunk, take - c);
-            if (!trace_chunk(0, n, hit_list + c, (uint32_t)(hits + c), nullptr, 0, next, &n_next)) return -1;
-        }
+        HC(hipEventRecord(ev_hits, sm));
+        const uint64_t hits_before = hits;
         hits += take;
-        n_hits_done = hits;
         s0 += batch;
         rate = (double)hits / (double)s0;
-        if (hits < N && s0 >= maxrays) {
-            errlog("generated sun rays reached maximum count: %d", raymax);
-            return -1;
+        if (hits < N) {
+            if (s0 >= maxrays) {
+                HC(hipStreamSynchronize(sm));
+                errlog("generated sun rays reached maximum count: %d", raymax);
+                return -1;
+            }
+            batch = plan(N - hits);
+            launch_mask_batch(s0, batch);  // overlaps the trace below
         }
+        HC(hipStreamWaitEvent(s, ev_hits, 0));
+        for (uint32_t c = 0; c < take; c += kMaxChunk) {
+            int n = (int)std::min<uint32_t>(kMaxChunk, take - c);
+            if (!trace_chunk(0, n, hit_list + c, (uint32_t)(hits_before + c), nullptr, 0, next, &n_next)) {
+                HC(hipStreamSynchronize(sm));
+                return -1;
+            }
+        }
+        n_hits_done = hits;
         if (cb && !cb((st_uint_t)traced_total, (st_uint_t)hits, (st_uint_t)N, 1, (st_uint_t)nst, cbdata)) {
             cancelled = true;
             break;
         }
     }
+    HC(hipStreamSynchronize(sm));
 
     // stage-0 pass-on rule: the last ray's pass counts only when an earlier
     // ray was passed too (coretrace/raytrace.cpp:786-797, 1081-1085)
