@@ -94,19 +94,20 @@ struct Context {
     std::vector<std::string> messages;
     hipStream_t stream = nullptr;
     hipStream_t mask_stream = nullptr;  // next batch's sun mask, beside the current batch's trace
-    hipEvent_t ev_hits = nullptr, ev_ready = nullptr;
+    hipEvent_t ev_hits[2] = {nullptr, nullptr}, ev_ready = nullptr, ev_tot = nullptr;
+    uint32_t *h_tot = nullptr;  // pinned: per-launch totals and error flags
     int device = 0;
 
     CompiledScene cs;
     bool dirty = true;            // a setter ran since the last compile
     bool compiled_cull = true;
     SceneView scene_view{};
-    DevBuf d_elems, d_faces, d_aux, d_stages, d_bvh, d_bvh_items, d_grid_cells, d_grid_items;
+    DevBuf d_elems, d_faces, d_aux, d_stages, d_bvh, d_bvh_items, d_grid_cells, d_grid_items, d_foot;
     // run scratch
     DevBuf d_mask_scan_tmp, d_mask_total;
     DevBuf d_mask, d_popc, d_offs, d_scan_tmp, d_total, d_hits, d_rec_slots, d_rec_count, d_rec_offs, d_ovf_rec,
         d_ovf_meta, d_ovf_count, d_out_slots, d_out_flag, d_out_offs, d_flux;
-    DevBuf d_rays_a, d_rays_b, d_sunpre;
+    DevBuf d_rays_a, d_rays_b, d_sunpre, d_sunpre_batch[2];
 
     // results
     std::vector<StageRecords> recs;
@@ -127,13 +128,18 @@ struct Context {
         HC(hipGetDevice(&device));
         HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HC(hipStreamCreateWithFlags(&mask_stream, hipStreamNonBlocking));
-        HC(hipEventCreateWithFlags(&ev_hits, hipEventDisableTiming));
+        for (auto &e : ev_hits) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         HC(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+        HC(hipEventCreateWithFlags(&ev_tot, hipEventDisableTiming));
+        HC(hipHostMalloc((void **)&h_tot, 16 * sizeof(uint32_t), hipHostMallocDefault));
     }
     ~Context() {
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
-        if (ev_hits) (void)hipEventDestroy(ev_hits);
+        for (auto e : ev_hits)
+            if (e) (void)hipEventDestroy(e);
         if (ev_ready) (void)hipEventDestroy(ev_ready);
+        if (ev_tot) (void)hipEventDestroy(ev_tot);
+        if (h_tot) (void)hipHostFree(h_tot);
         if (mask_stream) (void)hipStreamDestroy(mask_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -254,6 +260,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         v.bvh_items = upload(d_bvh_items, cs.bvh_items, s);
         v.grid_cells = upload(d_grid_cells, cs.grid_cells, s);
         v.grid_items = upload(d_grid_items, cs.grid_items, s);
+        v.foot = upload(d_foot, cs.foot, s);
         v.grid = cs.grid;
         v.use_grid = cs.use_grid ? 1 : 0;
         v.nstages = (int)cs.stages.size();
@@ -360,8 +367,38 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
 
     // One trace launch over n rays (chunk), then record and ray compaction.
     // Appends the passed rays to next_rays at *n_next.
-    auto trace_chunk = [&](int si, int n, const uint64_t *sun_idx, uint32_t raynum_base, const DevRay *in,
-                           uint32_t last_raynum, DevRay *next_rays, uint64_t *n_next) -> bool {
+    const bool presample = err_sun && (sv.sun.shape == 'd' || sv.sun.shape == 'D') && !sv.sun.point_source;
+    // Tabulated sunshape: the polar angles of a list of stage-0 rays, drawn
+    // cooperatively ahead of their trace.
+    auto launch_presample = [&](hipStream_t st, const uint64_t *sun_idx, int n, SunPre *out) {
+        SunPreArgs pa{};
+        pa.sun = sv.sun;
+        pa.aux = sv.aux;
+        pa.seed = seed;
+        pa.k0 = 3;  // two sun-plane draws and the reflectivity draw come first
+        pa.sun_index = sun_idx;
+        pa.n = n;
+        pa.out = out;
+        pa.err_flags = err;
+        pa.counters = ctrs ? ctrs + 5 * (1 + kMaxStages) : nullptr;
+        timed_on(st, timing.sunpre_ms, [&]() { launch_sunshape_d<R>(pa, st); });
+    };
+    // One trace launch over n rays (chunk), then record and ray compaction,
+    // in two halves so the host can queue other work between them: the
+    // launch half ends with an asynchronous copy of the totals into pinned
+    // memory; the finish half waits for it, checks the error flags, takes a
+    // record buffer and compacts.  Passed rays are appended to next_rays at
+    // *n_next.
+    struct Pending {
+        TraceArgs a;
+        int si, n;
+        DevRay *next_rays;
+        uint64_t *n_next;
+        std::chrono::steady_clock::time_point t0;
+    };
+    auto trace_launch = [&](int si, int n, const uint64_t *sun_idx, uint32_t raynum_base, const DevRay *in,
+                            uint32_t last_raynum, DevRay *next_rays, uint64_t *n_next,
+                            const SunPre *pre) -> Pending {
         TraceArgs a{};
         a.sv = sv;
         a.si = si;
@@ -387,56 +424,60 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         a.flux_bins = fs.active ? d_flux.as<uint32_t>() : nullptr;
         a.counters = ctrs ? ctrs + 5 * (1 + si) : nullptr;
         a.generic = cs.stages[si].generic;
-        a.sun_pre = nullptr;
-        if (si == 0 && err_sun && (sv.sun.shape == 'd' || sv.sun.shape == 'D') && !sv.sun.point_source) {
-            // tabulated sunshape: draw the polar angles cooperatively first
-            SunPreArgs pa{};
-            pa.sun = sv.sun;
-            pa.aux = sv.aux;
-            pa.seed = seed;
-            pa.k0 = 3;  // two sun-plane draws and the reflectivity draw come first
-            pa.sun_index = sun_idx;
-            pa.n = n;
-            pa.out = (SunPre *)d_sunpre.ensure((size_t)kMaxChunk * sizeof(SunPre));
-            pa.err_flags = err;
-            pa.counters = ctrs ? ctrs + 5 * (1 + kMaxStages) : nullptr;
-            timed(timing.sunpre_ms, [&]() { launch_sunshape_d<R>(pa, s); });
-            a.sun_pre = pa.out;
+        a.sun_pre = pre;
+        if (si == 0 && presample && pre == nullptr) {
+            SunPre *out = (SunPre *)d_sunpre.ensure((size_t)kMaxChunk * sizeof(SunPre));
+            launch_presample(s, sun_idx, n, out);
+            a.sun_pre = out;
         }
         timing.trace_launches[si]++;
         launch_fill_u32(ovf_count, 1, 0u, s);
         timed(timing.trace_ms[si], [&]() { launch_trace_stage<R>(a, s); });
         timing.rays_in[si] += (uint64_t)n;
-        auto t0 = std::chrono::steady_clock::now();
         launch_exclusive_scan(a.rec_count, d_rec_offs.as<uint32_t>(), (size_t)n, d_scan_tmp.as<uint32_t>(), total, s);
         launch_exclusive_scan(a.out_flag, d_out_offs.as<uint32_t>(), (size_t)n, d_scan_tmp.as<uint32_t>(), total + 1,
                               s);
-        uint32_t tot[4];
-        HC(hipMemcpyAsync(tot, total, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HC(hipStreamSynchronize(s));
+        HC(hipMemcpyAsync(h_tot, total, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HC(hipEventRecord(ev_tot, s));
+        return Pending{a, si, n, next_rays, n_next, std::chrono::steady_clock::now()};
+    };
+    auto trace_finish = [&](Pending &p) -> bool {
+        HC(hipEventSynchronize(ev_tot));
+        uint32_t tot[4] = {h_tot[0], h_tot[1], h_tot[2], h_tot[3]};
         if (!check_err(tot[3])) return false;
+        const TraceArgs &a = p.a;
+        const int n = p.n, si = p.si;
         std::shared_ptr<DevBuf> chunk = take_buffer((size_t)std::max<uint32_t>(1, tot[0]) * sizeof(DevRecord));
         launch_compact_records(a.rec_slots, a.rec_count, d_rec_offs.as<uint32_t>(), n, kSlots,
                                chunk->as<DevRecord>(), s);
         launch_compact_overflow(a.ovf_rec, a.ovf_meta, ovf_count, ovf_cap, d_rec_offs.as<uint32_t>(),
                                 chunk->as<DevRecord>(), s);
-        if (tot[1] && next_rays)
-            launch_compact_rays(a.out_slots, a.out_flag, d_out_offs.as<uint32_t>(), n, next_rays + *n_next, s);
+        if (tot[1] && p.next_rays)
+            launch_compact_rays(a.out_slots, a.out_flag, d_out_offs.as<uint32_t>(), n, p.next_rays + *p.n_next, s);
         recs[si].chunks.push_back(chunk);
         recs[si].counts.push_back(tot[0]);
         recs[si].total += tot[0];
         timing.records[si] += tot[0];
-        if (n_next) *n_next += tot[1];
-        timing.other_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (p.n_next) *p.n_next += tot[1];
+        timing.other_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p.t0).count();
         traced_total += (uint64_t)n;
         return true;
     };
+    auto trace_chunk = [&](int si, int n, const uint64_t *sun_idx, uint32_t raynum_base, const DevRay *in,
+                           uint32_t last_raynum, DevRay *next_rays, uint64_t *n_next,
+                           const SunPre *pre = nullptr) -> bool {
+        Pending p = trace_launch(si, n, sun_idx, raynum_base, in, last_raynum, next_rays, n_next, pre);
+        return trace_finish(p);
+    };
 
     // ------------------------------------------------------ stage 0 -------
-    // Batches of sun rays.  The sun mask of batch k+1 runs on the mask
-    // stream while batch k is sampled, traced and compacted on the trace
-    // stream; an event hands each hit list over.  The mask side has its own
-    // scan workspace and totals.
+    // Batches of sun rays, pipelined over two streams.  The mask stream
+    // draws batch k+1's sun-ray mask, compacts its hit list and samples its
+    // tabulated sunshape angles while the trace stream traces batch k; the
+    // host reads batch k+1's hit count after queueing batch k's trace and
+    // before waiting for batch k's totals.  Events hand each hit list over.
+    // The mask side has its own scan workspace and totals; the sunshape
+    // angles alternate between two buffers.
     DevRay *next = (DevRay *)d_rays_a.ensure((size_t)N * sizeof(DevRay));
     uint64_t *sun_index_all = (uint64_t *)d_hits.ensure((size_t)N * sizeof(uint64_t));
     const uint64_t kMaxBatch = 1u << 25;
@@ -450,21 +491,22 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     HC(hipEventRecord(ev_ready, s));  // counters and flags are cleared
     HC(hipStreamWaitEvent(sm, ev_ready, 0));
     uint64_t n_next = 0;
-    uint64_t hits = 0, s0 = 0;
+    uint64_t hits = 0, s0 = 0, batch = 0;
+    int parity = 0;
     double rate = 0.0;  // hits per sun ray observed so far
     auto plan = [&](uint64_t want) -> uint64_t {
-        uint64_t batch = (rate > 0) ? (uint64_t)(want / rate * 1.05) + 4096 : std::min<uint64_t>(1u << 22, want * 4 + 4096);
-        batch = std::min<uint64_t>(batch, kMaxBatch);
-        batch = std::max<uint64_t>(batch, 4096);
-        if (s0 + batch > maxrays) batch = maxrays - s0;
-        return batch;
+        uint64_t b = (rate > 0) ? (uint64_t)(want / rate * 1.05) + 4096 : std::min<uint64_t>(1u << 22, want * 4 + 4096);
+        b = std::min<uint64_t>(b, kMaxBatch);
+        b = std::max<uint64_t>(b, 4096);
+        if (s0 + b > maxrays) b = maxrays - s0;
+        return b;
     };
-    auto launch_mask_batch = [&](uint64_t first, uint64_t batch) {
+    auto launch_mask_batch = [&]() {
         const size_t nwords = (batch + 63) / 64;
         MaskArgs ma{};
         ma.sv = sv;
         ma.seed = seed;
-        ma.s0 = first;
+        ma.s0 = s0;
         ma.n = (uint32_t)batch;
         ma.mask = mask;
         ma.counters = ctrs;
@@ -475,58 +517,105 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         launch_mask_popc(mask, popc, nwords, sm);
         launch_exclusive_scan(popc, offs, nwords, mscan, mtotal, sm);
     };
-    uint64_t batch = plan(N);
-    if (batch == 0) {
-        errlog("generated sun rays reached maximum count: %d", raymax);
-        return -1;
-    }
-    launch_mask_batch(s0, batch);
-    while (hits < N) {
+    struct Ready {
+        uint64_t *hit_list;
+        uint32_t take;
+        uint64_t raynum_base;
+        SunPre *pre;
+        hipEvent_t ev;
+        bool more;           // another batch was launched behind this one
+        uint64_t sun_count;  // sun rays drawn up to this batch's last hit (or its end)
+    };
+    bool maxed = false;
+    // Read the pending mask batch's hit count, compact its hit list (and
+    // sample its sunshape angles), and launch the next mask batch if hits
+    // are still missing.
+    auto prepare = [&]() -> Ready {
+        Ready r{};
         const uint64_t want = N - hits;
         const size_t nwords = (batch + 63) / 64;
         uint32_t h = 0;
         HC(hipMemcpyAsync(&h, mtotal, sizeof h, hipMemcpyDeviceToHost, sm));
         HC(hipStreamSynchronize(sm));
-        const uint32_t take = (uint32_t)std::min<uint64_t>(h, want);
-        uint64_t *hit_list = sun_index_all + hits;
-        if (take) launch_mask_compact(mask, offs, nwords, s0, take, hit_list, sm);
-        if (take == want) {
-            uint64_t last = 0;
-            HC(hipMemcpyAsync(&last, hit_list + take - 1, sizeof last, hipMemcpyDeviceToHost, sm));
-            HC(hipStreamSynchronize(sm));
-            sun_ray_count = last + 1;
-        } else {
-            sun_ray_count = s0 + batch;
+        r.take = (uint32_t)std::min<uint64_t>(h, want);
+        r.hit_list = sun_index_all + hits;
+        r.raynum_base = hits;
+        if (r.take) launch_mask_compact(mask, offs, nwords, s0, r.take, r.hit_list, sm);
+        r.pre = nullptr;
+        if (presample && r.take) {
+            r.pre = (SunPre *)d_sunpre_batch[parity].ensure((size_t)r.take * sizeof(SunPre));
+            launch_presample(sm, r.hit_list, (int)r.take, r.pre);
         }
-        HC(hipEventRecord(ev_hits, sm));
-        const uint64_t hits_before = hits;
-        hits += take;
+        if (r.take == want) {
+            uint64_t last = 0;
+            HC(hipMemcpyAsync(&last, r.hit_list + r.take - 1, sizeof last, hipMemcpyDeviceToHost, sm));
+            HC(hipStreamSynchronize(sm));
+            r.sun_count = last + 1;
+        } else {
+            r.sun_count = s0 + batch;
+        }
+        r.ev = ev_hits[parity];
+        HC(hipEventRecord(r.ev, sm));
+        parity ^= 1;
+        hits += r.take;
         s0 += batch;
         rate = (double)hits / (double)s0;
+        r.more = false;
         if (hits < N) {
             if (s0 >= maxrays) {
-                HC(hipStreamSynchronize(sm));
-                errlog("generated sun rays reached maximum count: %d", raymax);
-                return -1;
-            }
-            batch = plan(N - hits);
-            launch_mask_batch(s0, batch);  // overlaps the trace below
-        }
-        HC(hipStreamWaitEvent(s, ev_hits, 0));
-        for (uint32_t c = 0; c < take; c += kMaxChunk) {
-            int n = (int)std::min<uint32_t>(kMaxChunk, take - c);
-            if (!trace_chunk(0, n, hit_list + c, (uint32_t)(hits_before + c), nullptr, 0, next, &n_next)) {
-                HC(hipStreamSynchronize(sm));
-                return -1;
+                maxed = true;
+            } else {
+                batch = plan(N - hits);
+                launch_mask_batch();
+                r.more = true;
             }
         }
-        n_hits_done = hits;
-        if (cb && !cb((st_uint_t)traced_total, (st_uint_t)hits, (st_uint_t)N, 1, (st_uint_t)nst, cbdata)) {
+        return r;
+    };
+    batch = plan(N);
+    if (batch == 0) {
+        errlog("generated sun rays reached maximum count: %d", raymax);
+        return -1;
+    }
+    launch_mask_batch();
+    Ready cur_b = prepare();
+    for (;;) {
+        HC(hipStreamWaitEvent(s, cur_b.ev, 0));
+        bool ok = true;
+        Ready next_b{};
+        bool have_next = false;
+        for (uint32_t c = 0; c < cur_b.take && ok; c += kMaxChunk) {
+            const int n = (int)std::min<uint32_t>(kMaxChunk, cur_b.take - c);
+            Pending p = trace_launch(0, n, cur_b.hit_list + c, (uint32_t)(cur_b.raynum_base + c), nullptr, 0, next,
+                                     &n_next, cur_b.pre != nullptr ? cur_b.pre + c : nullptr);
+            if (c == 0 && cur_b.more) {
+                next_b = prepare();  // overlaps the trace just queued
+                have_next = true;
+            }
+            ok = trace_finish(p);
+        }
+        if (!ok) {
+            HC(hipStreamSynchronize(sm));
+            return -1;
+        }
+        n_hits_done = (uint64_t)cur_b.raynum_base + cur_b.take;
+        sun_ray_count = cur_b.sun_count;
+        if (cb && !cb((st_uint_t)traced_total, (st_uint_t)n_hits_done, (st_uint_t)N, 1, (st_uint_t)nst, cbdata)) {
             cancelled = true;
             break;
         }
+        if (!have_next && cur_b.more) {  // an empty batch: nothing was traced
+            next_b = prepare();
+            have_next = true;
+        }
+        if (!have_next) break;
+        cur_b = next_b;
     }
     HC(hipStreamSynchronize(sm));
+    if (maxed && !cancelled) {
+        errlog("generated sun rays reached maximum count: %d", raymax);
+        return -1;
+    }
 
     // stage-0 pass-on rule: the last ray's pass counts only when an earlier
     // ray was passed too (coretrace/raytrace.cpp:786-797, 1081-1085)

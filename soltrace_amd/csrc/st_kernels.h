@@ -18,6 +18,7 @@ struct SceneView {
     const int32_t *bvh_items;
     const int32_t *grid_cells;   // ncell + 1 starts
     const int32_t *grid_items;
+    const DevFoot *foot;         // per stage-1 element
     DevSunGrid grid;
     int32_t use_grid;
     int32_t nstages;

@@ -409,8 +409,16 @@ static double bound_z(const HElement &e, double x, double y) {
     }
 }
 
-// Box in the element frame holding every hit point the trace can accept.
-static void element_bounds(const HElement &e, double lo[3], double hi[3]) {
+// Element-frame boxes holding every hit the element's intersection can
+// accept, as lo[b], hi[b].  Returns how many boxes bound the hits: 1, or 2
+// for a closed-form sphere or conic, whose solve takes the far root when the
+// near one lies above the z aperture (coretrace/quadricsurfaceclosedform.cpp
+// root selection) -- the other side of a sphere, the other sheet of a
+// hyperboloid -- and 0 for the torus, whose quartic solution
+// (coretrace/torusclosedform.cpp, root432.cpp) can land off the surface, far
+// outside the torus, so culling must always test it (box 0 is still its
+// nominal box).
+static int element_boxes(const HElement &e, double lo[2][3], double hi[2][3]) {
     const double A = e.p[0], B = e.p[1], C = e.p[2];
     double xl, xh, yl, yh;
     const char sh = (char)tolower(e.shape);
@@ -449,26 +457,61 @@ static void element_bounds(const HElement &e, double lo[3], double hi[3]) {
         break;
     }
     }
-    double zl = std::min(0.0, e.zap), zh = std::max(0.0, e.zap);
+    double zl[2] = {std::min(0.0, e.zap), HUGE_VAL}, zh[2] = {std::max(0.0, e.zap), -HUGE_VAL};
+    int nb = 1;
     if (sf == 't') {
-        zl = 0.0; zh = 2.0 / fabs(e.curv_rev);
+        zl[0] = 0.0; zh[0] = 2.0 / fabs(e.curv_rev);
     } else if (sf == 'd') {
-        zl = 0.0; zh = 2.0 * e.cross_r;
+        zl[0] = 0.0; zh[0] = 2.0 * e.cross_r;
     } else if (sf != 'f') {
+        // Near and far roots lie on the branches
+        // z = (1 -/+ sqrt(1 - k c^2 r^2)) / (k c), k = 1 for the sphere; both
+        // are monotonic in r^2, so the grid samples (corners included), the
+        // point nearest the axis and, where the box reaches past the
+        // surface's rim (q = 0), the rim height 1/(k c) bound them.
+        const bool two = sf == 's' || (sf == 'o' && e.stype == 1 && e.kappa != 0.0);
+        const double c = e.vcx, k = sf == 's' ? 1.0 : e.kappa;
+        const bool line = sf == 's' && e.stype == 7;
+        auto put = [&](int b, double z) {
+            if (std::isfinite(z)) { zl[b] = std::min(zl[b], z); zh[b] = std::max(zh[b], z); }
+        };
+        auto sample = [&](double x, double y) {
+            put(0, bound_z(e, x, y));
+            if (two && c != 0.0) {
+                const double q = 1.0 - k * c * c * (x * x + (line ? 0.0 : y * y));
+                if (q >= 0.0) put(1, (1.0 + sqrt(q)) / (k * c));
+            }
+        };
         const int G = 64;
         for (int i = 0; i <= G; i++)
-            for (int j = 0; j <= G; j++) {
-                double x = xl + (xh - xl) * i / G, y = yl + (yh - yl) * j / G;
-                double z = bound_z(e, x, y);
-                if (std::isfinite(z)) { zl = std::min(zl, z); zh = std::max(zh, z); }
+            for (int j = 0; j <= G; j++) sample(xl + (xh - xl) * i / G, yl + (yh - yl) * j / G);
+        sample(std::min(std::max(0.0, xl), xh), std::min(std::max(0.0, yl), yh));
+        if (two && c != 0.0 && k > 0.0) {
+            const double mx = std::max(fabs(xl), fabs(xh)), my = line ? 0.0 : std::max(fabs(yl), fabs(yh));
+            if (k * c * c * (mx * mx + my * my) >= 1.0) { put(0, 1.0 / (k * c)); put(1, 1.0 / (k * c)); }
+        }
+        for (int b = 0; b < 2; b++) {
+            if (!(zl[b] <= zh[b])) continue;
+            const double pad = 0.05 * (zh[b] - zl[b]) + 1e-3;
+            zl[b] -= pad; zh[b] += pad;
+        }
+        if (zl[1] <= zh[1]) {
+            if (zl[1] <= zh[0] && zl[0] <= zh[1]) {  // the branches meet: one box
+                zl[0] = std::min(zl[0], zl[1]);
+                zh[0] = std::max(zh[0], zh[1]);
+            } else {
+                nb = 2;
             }
-        double pad = 0.05 * (zh - zl) + 1e-3;
-        zl -= pad; zh += pad;
+        }
     }
-    double pad = 1e-4 + 1e-7 * std::max({fabs(xl), fabs(xh), fabs(yl), fabs(yh)});
-    lo[0] = xl - pad; hi[0] = xh + pad;
-    lo[1] = yl - pad; hi[1] = yh + pad;
-    lo[2] = zl - pad; hi[2] = zh + pad;
+    const double pad = 1e-4 + 1e-7 * std::max({fabs(xl), fabs(xh), fabs(yl), fabs(yh)});
+    for (int b = 0; b < 2; b++) {
+        const int src = b < nb ? b : 0;
+        lo[b][0] = xl - pad; hi[b][0] = xh + pad;
+        lo[b][1] = yl - pad; hi[b][1] = yh + pad;
+        lo[b][2] = zl[src] - pad; hi[b][2] = zh[src] + pad;
+    }
+    return sf == 'd' ? 0 : nb;
 }
 
 // ------------------------------------------------------------------- BVH --
@@ -478,9 +521,18 @@ struct BuildItem {
     int32_t idx;
 };
 
+static double box_area(const float lo[3], const float hi[3]) {
+    const double dx = (double)hi[0] - lo[0], dy = (double)hi[1] - lo[1], dz = (double)hi[2] - lo[2];
+    return 2.0 * (dx * dy + dy * dz + dz * dx);
+}
+
+// Binned surface-area-heuristic build: split where the expected cost of a
+// ray entering the node, node test (1) plus element tests (2 each) weighted
+// by the children's surface areas, is least; leaves hold at most 7 items
+// (the node's 3-bit count).
 static int build_bvh(std::vector<BuildItem> &items, int begin, int end, std::vector<DevBvhNode> &nodes,
                      std::vector<int32_t> &out_items) {
-    int me = (int)nodes.size();
+    const int me = (int)nodes.size();
     nodes.push_back(DevBvhNode{});
     float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
     float clo[3] = {3e38f, 3e38f, 3e38f}, chi[3] = {-3e38f, -3e38f, -3e38f};
@@ -492,20 +544,69 @@ static int build_bvh(std::vector<BuildItem> &items, int begin, int end, std::vec
     DevBvhNode nd{};
     for (int k = 0; k < 3; k++) { nd.lo[k] = lo[k]; nd.hi[k] = hi[k]; }
     const int n = end - begin;
-    if (n <= 4) {
+    auto make_leaf = [&]() {
         nd.leaf = ((int)out_items.size() << 3) | n;
         for (int i = begin; i < end; i++) out_items.push_back(items[i].idx);
         nd.skip = me + 1;
         nodes[me] = nd;
         return me;
+    };
+    if (n <= 2) return make_leaf();
+    constexpr int NB = 16;
+    const double area = std::max(box_area(lo, hi), 1e-30);
+    double best = HUGE_VAL;
+    int best_ax = -1, best_b = 0;
+    for (int ax = 0; ax < 3; ax++) {
+        const double ext = (double)chi[ax] - clo[ax];
+        if (!(ext > 0)) continue;
+        int cnt[NB] = {};
+        float blo[NB][3], bhi[NB][3];
+        for (int b = 0; b < NB; b++)
+            for (int k = 0; k < 3; k++) { blo[b][k] = 3e38f; bhi[b][k] = -3e38f; }
+        for (int i = begin; i < end; i++) {
+            const int b = std::min(NB - 1, (int)((items[i].c[ax] - clo[ax]) / ext * NB));
+            cnt[b]++;
+            for (int k = 0; k < 3; k++) {
+                blo[b][k] = std::min(blo[b][k], items[i].lo[k]);
+                bhi[b][k] = std::max(bhi[b][k], items[i].hi[k]);
+            }
+        }
+        double right_area[NB];
+        int right_cnt[NB];
+        float rlo[3] = {3e38f, 3e38f, 3e38f}, rhi[3] = {-3e38f, -3e38f, -3e38f};
+        int rc = 0;
+        for (int b = NB - 1; b >= 1; b--) {
+            rc += cnt[b];
+            for (int k = 0; k < 3; k++) { rlo[k] = std::min(rlo[k], blo[b][k]); rhi[k] = std::max(rhi[k], bhi[b][k]); }
+            right_cnt[b] = rc;
+            right_area[b] = rc ? box_area(rlo, rhi) : 0.0;
+        }
+        float llo[3] = {3e38f, 3e38f, 3e38f}, lhi[3] = {-3e38f, -3e38f, -3e38f};
+        int lc = 0;
+        for (int b = 1; b < NB; b++) {  // split before bin b
+            lc += cnt[b - 1];
+            for (int k = 0; k < 3; k++) {
+                llo[k] = std::min(llo[k], blo[b - 1][k]);
+                lhi[k] = std::max(lhi[k], bhi[b - 1][k]);
+            }
+            if (lc == 0 || right_cnt[b] == 0) continue;
+            const double c = 1.0 + 2.0 * (box_area(llo, lhi) * lc + right_area[b] * right_cnt[b]) / area;
+            if (c < best) { best = c; best_ax = ax; best_b = b; }
+        }
     }
-    int ax = 0;
-    float ext = chi[0] - clo[0];
-    for (int k = 1; k < 3; k++)
-        if (chi[k] - clo[k] > ext) { ext = chi[k] - clo[k]; ax = k; }
-    int mid = begin + n / 2;
-    std::nth_element(items.begin() + begin, items.begin() + mid, items.begin() + end,
-                     [ax](const BuildItem &a, const BuildItem &b) { return a.c[ax] < b.c[ax]; });
+    if (n <= 7 && !(best < 2.0 * n)) return make_leaf();
+    int mid;
+    if (best_ax >= 0) {
+        const int ax = best_ax;
+        const double ext = (double)chi[ax] - clo[ax];
+        auto it = std::partition(items.begin() + begin, items.begin() + end, [&](const BuildItem &a) {
+            return std::min(NB - 1, (int)((a.c[ax] - clo[ax]) / ext * NB)) < best_b;
+        });
+        mid = (int)(it - items.begin());
+    } else {
+        // every centroid equal: halve the range
+        mid = begin + n / 2;
+    }
     build_bvh(items, begin, mid, nodes, out_items);   // left child = me + 1
     build_bvh(items, mid, end, nodes, out_items);
     nd.leaf = 0;
@@ -682,21 +783,27 @@ bool compile_scene(SceneModel &m, bool use_cull, CompiledScene &out, std::string
                 d.aux_n = (int)e.poly.size();
                 out.aux.insert(out.aux.end(), e.poly.begin(), e.poly.end());
             }
-            double lo[3], hi[3];
-            element_bounds(e, lo, hi);
-            double c[3] = {(lo[0] + hi[0]) / 2, (lo[1] + hi[1]) / 2, (lo[2] + hi[2]) / 2};
-            double r = 0.5 * sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
-                                  (hi[2] - lo[2]) * (hi[2] - lo[2]));
+            double lo[2][3], hi[2][3];
+            const int nb = element_boxes(e, lo, hi);
+            out.enbox.push_back((int8_t)nb);
+            for (int q = 0; q < 2; q++)
+                out.ebox.insert(out.ebox.end(), {lo[q][0], lo[q][1], lo[q][2], hi[q][0], hi[q][1], hi[q][2]});
+            // sphere around box 0 (the torus solve starts on it for far origins)
+            double c[3] = {(lo[0][0] + hi[0][0]) / 2, (lo[0][1] + hi[0][1]) / 2, (lo[0][2] + hi[0][2]) / 2};
+            double r = 0.5 * sqrt((hi[0][0] - lo[0][0]) * (hi[0][0] - lo[0][0]) +
+                                  (hi[0][1] - lo[0][1]) * (hi[0][1] - lo[0][1]) +
+                                  (hi[0][2] - lo[0][2]) * (hi[0][2] - lo[0][2]));
             for (int k = 0; k < 3; k++) d.bsphere_c[k] = c[k];
             d.bsphere_r = r;
             if (d.surf != SURF_FLAT && d.surf > SURF_CYLINDER) ds.generic = 1;
             out.elems.push_back(d);
-            if (e.enabled) {
-                // stage-frame box from the 8 corners of the element-frame box
+            for (int q = 0; e.enabled && q < std::max(nb, 1); q++) {
+                // stage-frame box from the 8 corners of an element-frame box
                 BuildItem it;
                 double slo[3] = {1e300, 1e300, 1e300}, shi[3] = {-1e300, -1e300, -1e300};
-                for (int q = 0; q < 8; q++) {
-                    double v[3] = {(q & 1) ? hi[0] : lo[0], (q & 2) ? hi[1] : lo[1], (q & 4) ? hi[2] : lo[2]};
+                for (int v8 = 0; v8 < 8; v8++) {
+                    double v[3] = {(v8 & 1) ? hi[q][0] : lo[q][0], (v8 & 2) ? hi[q][1] : lo[q][1],
+                                   (v8 & 4) ? hi[q][2] : lo[q][2]};
                     double w[3];
                     mtv(e.R, v, w);
                     for (int k = 0; k < 3; k++) {
@@ -707,9 +814,9 @@ bool compile_scene(SceneModel &m, bool use_cull, CompiledScene &out, std::string
                 }
                 for (int k = 0; k < 3; k++) {
                     double pad = 1e-2 + 1e-5 * std::max(fabs(slo[k]), fabs(shi[k]));
-                    it.lo[k] = round_down(slo[k] - pad);
-                    it.hi[k] = round_up(shi[k] + pad);
-                    it.c[k] = 0.5f * (it.lo[k] + it.hi[k]);
+                    it.lo[k] = nb > 0 ? round_down(slo[k] - pad) : -1e15f;
+                    it.hi[k] = nb > 0 ? round_up(shi[k] + pad) : 1e15f;
+                    it.c[k] = nb > 0 ? 0.5f * (it.lo[k] + it.hi[k]) : (float)(0.5 * (slo[k] + shi[k]));
                 }
                 it.idx = (int32_t)j;
                 items.push_back(it);
@@ -731,6 +838,52 @@ bool compile_scene(SceneModel &m, bool use_cull, CompiledScene &out, std::string
 }  // namespace st
 
 namespace st {
+
+// Convex hull of sun-plane points as outward half-planes n.p <= off,
+// relative to the vertices' mean and padded by pad (monotone chain,
+// counter-clockwise).  False when degenerate or above 6 edges.
+static bool foot_hull(std::vector<std::pair<double, double>> pts, double pad, DevFoot &f) {
+    using P = std::pair<double, double>;
+    std::sort(pts.begin(), pts.end());
+    auto cross = [](const P &o, const P &a, const P &b) {
+        return (a.first - o.first) * (b.second - o.second) - (a.second - o.second) * (b.first - o.first);
+    };
+    std::vector<P> h(2 * pts.size());
+    int k = 0;
+    for (size_t i = 0; i < pts.size(); i++) {
+        while (k >= 2 && cross(h[k - 2], h[k - 1], pts[i]) <= 0) k--;
+        h[k++] = pts[i];
+    }
+    for (int i = (int)pts.size() - 2, t = k + 1; i >= 0; i--) {
+        while (k >= t && cross(h[k - 2], h[k - 1], pts[i]) <= 0) k--;
+        h[k++] = pts[i];
+    }
+    const int nv = k - 1;
+    if (nv < 3 || nv > 6) return false;
+    double cx = 0, cy = 0;
+    for (int i = 0; i < nv; i++) { cx += h[i].first; cy += h[i].second; }
+    cx /= nv;
+    cy /= nv;
+    for (int i = 0; i < 6; i++) {
+        f.e[i][0] = 0.0f;
+        f.e[i][1] = 0.0f;
+        f.e[i][2] = 1.0f;
+    }
+    for (int i = 0; i < nv; i++) {
+        const double ex = h[i + 1].first - h[i].first, ey = h[i + 1].second - h[i].second;
+        const double len = sqrt(ex * ex + ey * ey);
+        if (!(len > 0)) return false;
+        const double nx = ey / len, ny = -ex / len;
+        const double off = nx * (h[i].first - cx) + ny * (h[i].second - cy) + pad;
+        f.e[i][0] = (float)nx;
+        f.e[i][1] = (float)ny;
+        f.e[i][2] = (float)(off * (1.0 + 1e-6) + 1e-6);
+    }
+    f.cx = cx;
+    f.cy = cy;
+    f.kind = 0;
+    return true;
+}
 
 // Sun frame, sun-plane box and the stage-1 sun-plane grid
 // (coretrace/suntoprimarystage.cpp:59-293).
@@ -855,31 +1008,57 @@ bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
         }
     }
 
-    // sun-plane grid of stage-1 element footprints (bounding spheres)
+    // Sun-plane footprints of the stage-1 elements: each hit box projected
+    // along the sun direction.  Its convex hull (DevFoot) screens candidates
+    // and its bounding rectangle places the element in grid cells.  A
+    // footprint wholly outside the sun-plane box is never reached, since sun
+    // rays start inside that box.
     const DevStage &s0 = out.stages[0];
-    std::vector<double> fx, fy, fr;
-    std::vector<int32_t> fj;
-    double med = 0.0;
+    struct Rect { int j; double x0, x1, y0, y1; };
+    std::vector<Rect> rects;
+    std::vector<double> sizes;
+    out.foot.assign(2 * (size_t)std::max(1, s0.nelem), DevFoot{});
+    for (DevFoot &f : out.foot) f.kind = 2;
     for (int j = 0; j < s0.nelem; j++) {
         const DevElem &d = out.elems[s0.elem0 + j];
         if (!d.enabled) continue;
-        double w[3], c[3];
-        mtv(d.R, d.bsphere_c, w);
-        for (int k = 0; k < 3; k++) c[k] = w[k] + d.org[k] - pss[k];
-        double q[3];
-        mv(Rs, c, q);
-        fx.push_back(q[0]);
-        fy.push_back(q[1]);
-        fr.push_back(d.bsphere_r * (1.0 + 1e-9) + 1e-6 + 1e-12 * (fabs(q[0]) + fabs(q[1])));
-        fj.push_back(j);
+        const int nb = out.enbox[s0.elem0 + j];
+        if (nb == 0) {
+            out.foot[2 * j].kind = 1;
+            rects.push_back({j, -HUGE_VAL, HUGE_VAL, -HUGE_VAL, HUGE_VAL});
+            continue;
+        }
+        for (int b = 0; b < nb; b++) {
+            const double *bx = &out.ebox[12 * (size_t)(s0.elem0 + j) + 6 * b];
+            std::vector<std::pair<double, double>> pts;
+            double x0 = HUGE_VAL, x1 = -HUGE_VAL, y0 = HUGE_VAL, y1 = -HUGE_VAL;
+            for (int q = 0; q < 8; q++) {
+                const double v[3] = {(q & 1) ? bx[3] : bx[0], (q & 2) ? bx[4] : bx[1], (q & 4) ? bx[5] : bx[2]};
+                double w[3], c[3], u[3];
+                mtv(d.R, v, w);
+                for (int k = 0; k < 3; k++) c[k] = w[k] + d.org[k] - pss[k];
+                mv(Rs, c, u);
+                pts.emplace_back(u[0], u[1]);
+                x0 = std::min(x0, u[0]); x1 = std::max(x1, u[0]);
+                y0 = std::min(y0, u[1]); y1 = std::max(y1, u[1]);
+            }
+            // 1 mm, plus what rounding of sun-plane coordinates can reach
+            const double pad = 1e-3 + 1e-9 * (fabs(x0) + fabs(x1) + fabs(y0) + fabs(y1));
+            x0 -= pad; x1 += pad; y0 -= pad; y1 += pad;
+            if (x1 < minx || x0 > maxx || y1 < miny || y0 > maxy) continue;
+            if (!foot_hull(pts, pad, out.foot[2 * j + b])) out.foot[2 * j + b].kind = 1;
+            if (b == 1) out.foot[2 * j].more = 1;
+            rects.push_back({j, x0, x1, y0, y1});
+            sizes.push_back(std::max(x1 - x0, y1 - y0));
+        }
     }
-    if (!fr.empty()) {
-        std::vector<double> tmp(fr);
-        std::nth_element(tmp.begin(), tmp.begin() + tmp.size() / 2, tmp.end());
-        med = tmp[tmp.size() / 2];
+    double med = 0.0;
+    if (!sizes.empty()) {
+        std::nth_element(sizes.begin(), sizes.begin() + sizes.size() / 2, sizes.end());
+        med = sizes[sizes.size() / 2];
     }
     double w = maxx - minx, h = maxy - miny;
-    double cell = std::max(2.0 * med, std::max(w, h) / 4096.0);
+    double cell = std::max(med, std::max(w, h) / 4096.0);
     if (!(cell > 0)) cell = std::max(w, h) + 1.0;
     int nx = std::max(1, std::min(4096, (int)ceil(w / cell)));
     int ny = std::max(1, std::min(4096, (int)ceil(h / cell)));
@@ -891,29 +1070,30 @@ bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
     g.inv_dx = nx / (w > 0 ? w : 1.0);
     g.inv_dy = ny / (h > 0 ? h : 1.0);
     std::vector<int32_t> cnt((size_t)nx * ny + 1, 0);
-    auto span = [&](size_t k, int &ax, int &bx, int &ay, int &by) {
-        ax = (int)floor((fx[k] - fr[k] - g.x0) * g.inv_dx);
-        bx = (int)floor((fx[k] + fr[k] - g.x0) * g.inv_dx);
-        ay = (int)floor((fy[k] - fr[k] - g.y0) * g.inv_dy);
-        by = (int)floor((fy[k] + fr[k] - g.y0) * g.inv_dy);
-        ax = std::max(0, std::min(nx - 1, ax)); bx = std::max(0, std::min(nx - 1, bx));
-        ay = std::max(0, std::min(ny - 1, ay)); by = std::max(0, std::min(ny - 1, by));
+    auto cell_of = [](double v, int n) { return (int)std::max(0.0, std::min((double)(n - 1), floor(v))); };
+    struct Span { int ax, bx, ay, by; };
+    auto span = [&](const Rect &r) {
+        return Span{cell_of((r.x0 - g.x0) * g.inv_dx, nx), cell_of((r.x1 - g.x0) * g.inv_dx, nx),
+                    cell_of((r.y0 - g.y0) * g.inv_dy, ny), cell_of((r.y1 - g.y0) * g.inv_dy, ny)};
     };
-    for (size_t k = 0; k < fx.size(); k++) {
-        int ax, bx, ay, by;
-        span(k, ax, bx, ay, by);
-        for (int y = ay; y <= by; y++)
-            for (int x = ax; x <= bx; x++) cnt[(size_t)y * nx + x + 1]++;
-    }
+    // each element enters a cell once: a second box skips the first's cells
+    auto each_cell = [&](auto &&fn) {
+        for (size_t k = 0; k < rects.size(); k++) {
+            const Span s = span(rects[k]);
+            const bool second = k > 0 && rects[k - 1].j == rects[k].j;
+            const Span p = second ? span(rects[k - 1]) : Span{1, 0, 1, 0};
+            for (int y = s.ay; y <= s.by; y++)
+                for (int x = s.ax; x <= s.bx; x++) {
+                    if (x >= p.ax && x <= p.bx && y >= p.ay && y <= p.by) continue;
+                    fn((size_t)y * nx + x, rects[k].j);
+                }
+        }
+    };
+    each_cell([&](size_t c, int) { cnt[c + 1]++; });
     for (size_t c = 1; c < cnt.size(); c++) cnt[c] += cnt[c - 1];
     out.grid_items.assign(cnt.back(), 0);
     std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
-    for (size_t k = 0; k < fx.size(); k++) {
-        int ax, bx, ay, by;
-        span(k, ax, bx, ay, by);
-        for (int y = ay; y <= by; y++)
-            for (int x = ax; x <= bx; x++) out.grid_items[fill[(size_t)y * nx + x]++] = fj[k];
-    }
+    each_cell([&](size_t c, int j) { out.grid_items[fill[c]++] = j; });
     out.grid_cells.swap(cnt);
     out.use_grid = !sun.point_source;
     return true;

@@ -80,6 +80,9 @@ struct CompiledScene {
     std::vector<DevBvhNode> bvh;
     std::vector<int32_t> bvh_items;
     std::vector<int32_t> grid_cells, grid_items;
+    std::vector<DevFoot> foot;  // 2 per stage-1 element, one per hit box
+    std::vector<double> ebox;   // per element: 2 element-frame hit boxes lo[3], hi[3]
+    std::vector<int8_t> enbox;  // per element: boxes that bound its hits (0: none)
     DevSunGrid grid{};
     bool use_grid = false;
     DevSun sun{};

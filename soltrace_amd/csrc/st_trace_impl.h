@@ -108,13 +108,31 @@ STD void consider(const SceneView &sv, const DevStage &st, int j, const double p
     }
 }
 
-// Candidates: the sun-plane grid cell for a first hit of a sun ray, the
-// stage BVH when present, every element otherwise.  One loop feeds them all
-// to a single consider() so the element code is instantiated once.
-// grid: the candidates are the sun-plane grid cell of (sx, sy).
+// Whether sun-plane point (sx, sy) lies inside element j's footprints.  The
+// first record is read whole before any test, so its loads issue together.
+STD bool foot_contains(const DevFoot *foot, int j, double sx, double sy) {
+    const DevFoot f = foot[2 * j];
+    const float px = (float)(sx - f.cx), py = (float)(sy - f.cy);
+    bool in = true;
+#pragma unroll
+    for (int i = 0; i < 6; i++) in = in && (f.e[i][0] * px + f.e[i][1] * py <= f.e[i][2]);
+    in = f.kind == 0 ? in : f.kind == 1;
+    if (in || !f.more) return in;
+    const DevFoot g = foot[2 * j + 1];
+    const float qx = (float)(sx - g.cx), qy = (float)(sy - g.cy);
+    in = true;
+#pragma unroll
+    for (int i = 0; i < 6; i++) in = in && (g.e[i][0] * qx + g.e[i][1] * qy <= g.e[i][2]);
+    return g.kind == 0 ? in : g.kind == 1;
+}
+
+// Candidates: the sun-plane grid cell for a first hit of a sun ray (grid),
+// screened by the elements' footprints; the stage BVH when present; every
+// element otherwise.  One loop feeds them all to a single consider() so the
+// element code is instantiated once.
 template <typename R, bool G>
 STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3], const double cs[3], bool grid,
-                     double sx, double sy, Best &b) {
+                     bool screen, double sx, double sy, Best &b) {
     b.path = 1e99;
     b.j = 0x7fffffff;
     const int32_t *items = nullptr;  // nullptr: the item is the element index
@@ -144,6 +162,7 @@ STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3]
         if (k < kend) {
             const int j = items != nullptr ? items[k] : k;
             ++k;
+            if (screen && !foot_contains(sv.foot, j, sx, sy)) continue;
             consider<R, G>(sv, st, j, ps, cs, b);
             continue;
         }
@@ -184,7 +203,7 @@ STD bool any_hit(const SceneView &sv, const DevStage &st, const double ps[3], co
     Best b;
     b.tests = 0;
     b.nodes = 0;
-    closest_hit<R, G>(sv, st, ps, cs, grid, sx, sy, b);
+    closest_hit<R, G>(sv, st, ps, cs, grid, grid, sx, sy, b);
     *tests += b.tests;
     return b.path < 1e99;
 }
@@ -422,7 +441,7 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
         b.tests = 0;
         b.nodes = 0;
         const bool first_sun = sun_stage && mhc == 0 && sv.use_grid && !sv.sun.point_source;
-        closest_hit<R, G>(sv, st, ps, cs, first_sun, psun[0], psun[1], b);
+        closest_hit<R, G>(sv, st, ps, cs, first_sun, first_sun, psun[0], psun[1], b);
         ev_tests += b.tests;
         ev_nodes += b.nodes;
         if (!(b.path < 1e99)) {

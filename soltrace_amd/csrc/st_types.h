@@ -89,6 +89,19 @@ struct DevSunGrid {
     int32_t cell0, list0;   // offsets into the cell-start and list buffers
 };
 
+// Sun-plane footprint of a stage-1 element's hit box: the convex hull of the
+// box projected along the sun direction (a box projects to at most a
+// hexagon), as 6 outward half-planes n.p <= off (float, p relative to the
+// double centre, padded; unused edges are 0.p <= 1).  A sun ray can only hit
+// the element inside one of its footprints.  kind: 0 test the half-planes,
+// 1 contains every point, 2 contains none; more: the element's second
+// footprint (the next record) is to be tested too.
+struct DevFoot {
+    double cx, cy;
+    float e[6][3];          // nx, ny, off
+    int32_t kind, more;
+};
+
 // BVH node in depth-first order (an inner node's left child is the next
 // node): float box padded outward, the index to continue at when the box is
 // missed or the subtree is done, and the leaf item range.

@@ -64,6 +64,50 @@ def test_cull_equals_full_scan(hip_lib):
     np.testing.assert_array_equal(a[1].pos, b[1].pos)
 
 
+def _curved_field():
+    """Twelve closed-form curved elements and a torus, tilted every way under
+    an oblique sun, so rays reach far roots (the other side of a sphere, the
+    other sheet of a hyperboloid) and the torus solve's off-surface roots."""
+    from soltrace_amd.scene import Element, Optic, OpticFace, Scene, Stage
+    s = Scene(sun_shape="p", sun_sigma=4.65, sun_xyz=(0.6, -0.3, 1.0), raycount=20000, maxcount=10 ** 7)
+    s.optics = [Optic("m", OpticFace(refl=0.7), OpticFace(refl=0.7))]
+    surf = [("s", (0.4,)), ("o", (0.3, -0.5)), ("o", (0.3, 0.5)), ("s", (-0.3,)), ("p", (0.4, 0.2)), ("t", (2.5,))]
+    rng = np.random.default_rng(5)
+    els = []
+    for k in range(12):
+        name, sp = surf[k % len(surf)]
+        ap, app = ("l", (0.0, 0.0, 2.0)) if name == "t" else ("c", (2.0,))
+        xyz = (3.0 * (k % 4) - 4.5, 3.0 * (k // 4) - 3.0, 0.5 * (k % 3))
+        aim = tuple(np.add(xyz, rng.normal(0.0, 0.6, 3) + (0.0, 0.0, 1.0)))
+        els.append(Element(xyz=xyz, aim=aim, aperture=ap, ap_params=app, surface=name, surf_params=sp, optic="m"))
+    els.append(Element(xyz=(0, 0, 2.5), aim=(0.3, 0.1, 3.5), aperture="a", ap_params=(0, 0, 360.0), surface="d",
+                       surf_params=(0.8, 0.3), optic="m"))
+    roof = [Element(xyz=(2.0 * i - 5.0, 2.0 * j - 3.0, 6.0), aim=(2.0 * i - 5.0, 2.0 * j - 3.0, 0.0), aperture="r",
+                    ap_params=(1.9, 1.9), surface="s", surf_params=(0.05,), optic="m")
+             for i in range(6) for j in range(4)]
+    s.stages = [Stage(elements=els), Stage(elements=roof)]
+    return s
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_cull_equals_full_scan_curved(hip_lib, bits):
+    sc = _curved_field()
+    a = run_scene(hip_lib, sc, 9, setup=_setup(bits, 1))
+    b = run_scene(hip_lib, sc, 9, setup=_setup(bits, 0))
+    assert a[0] == b[0] > 0, (a[0], b[0], a[2])
+    for f in ("element", "stage", "raynum"):
+        np.testing.assert_array_equal(getattr(a[1], f), getattr(b[1], f))
+    np.testing.assert_array_equal(a[1].pos, b[1].pos)
+
+
+def test_curved_field_fp64(hip_lib, oracle_lib):
+    sc = _curved_field()
+    n_o, r_o, _, _ = run_scene(oracle_lib, sc, 9, setup=counter_mode)
+    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 9, setup=_setup(64))
+    assert n_g == n_o and n_o > 0, (n_g, n_o, m_g)
+    assert_records_match(r_g, r_o, pos_tol=1e-6, cos_tol=1e-6)
+
+
 def _by_ray(res, sun_idx):
     """Group stage-1 records per sun ray: element sequence and positions."""
     seqs = {}
