@@ -511,6 +511,8 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         ma.mask = mask;
         ma.counters = ctrs;
         ma.generic = cs.stages[0].generic;
+        // pairs pack the element index in 24 bits
+        ma.compact = sv.use_grid && !sv.sun.point_source && cs.stages[0].nelem < (1 << 24);
         timing.mask_launches++;
         timed_on(sm, timing.mask_ms, [&]() { launch_sun_mask<R>(ma, sm); });
         timing.sun_rays += batch;

@@ -88,6 +88,7 @@ struct MaskArgs {
     uint64_t *mask;              // one word per 64 sun rays
     unsigned long long *counters;  // as TraceArgs::counters (nullptr: off)
     int32_t generic;             // kernel variant: DevStage::generic of stage 0
+    int32_t compact;             // screen grid candidates into per-wave pair queues
 };
 
 template <typename R> void launch_sun_mask(const MaskArgs &a, hipStream_t s);

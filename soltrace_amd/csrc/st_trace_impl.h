@@ -2,9 +2,12 @@
 // st_trace_f64.hip and for FP32 in st_trace_f32.hip.
 //
 // Stage 0 runs in two launches per batch of sun rays:
-//   1. sun_mask   -- one lane per sun-ray index s: draw the two sun-plane
-//                    uniforms, find whether any stage-1 element accepts the
-//                    ray, ballot the answer into one 64-bit word per wave.
+//   1. sun_mask   -- per sun-ray index s: draw the two sun-plane uniforms,
+//                    find whether any stage-1 element accepts the ray, and
+//                    set bit s of the hit mask (one 64-bit word per 64 rays).
+//                    Over the sun-plane grid, each lane draws several rays
+//                    and screens their candidates, and the element tests run
+//                    on per-wave queues of the pairs that pass.
 //   2. trace_stage -- one lane per hitting sun ray, in sun-index order (ray
 //                    numbers are ranks among hitting sun rays, as the
 //                    reference numbers them, coretrace/raytrace.cpp:780-797).
@@ -79,9 +82,6 @@ struct Best {
     uint32_t nodes;      // BVH nodes visited (event counter)
 };
 
-// Test element j; keep it when it is the closest accepted hit so far
-// (coretrace/raytrace.cpp:740-768; ties go to the lower element index, the
-// order a brute-force scan meets them).
 // The ray in element j's frame, nudged 1e-5 m along itself
 // (coretrace/raytrace.cpp:725-731).
 STD void element_ray(const DevElem &e, const double ps[3], const double cs[3], double pe[3], double ce[3]) {
@@ -91,6 +91,9 @@ STD void element_ray(const DevElem &e, const double ps[3], const double cs[3], d
     pe[2] = pe[2] + 1.0e-5 * ce[2];
 }
 
+// Test element j; keep it when it is the closest accepted hit so far
+// (coretrace/raytrace.cpp:740-768; ties go to the lower element index, the
+// order a brute-force scan meets them).
 template <typename R, bool G>
 STD void consider(const SceneView &sv, const DevStage &st, int j, const double ps[3], const double cs[3], Best &b) {
     const DevElem &e = sv.elems[st.elem0 + j];
@@ -232,6 +235,116 @@ __global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
     if ((threadIdx.x & 63) == 0) {
         const uint32_t w = t >> 6;
         if ((uint64_t)w * 64 < a.n) a.mask[w] = word;
+    }
+}
+
+// Sun mask over the sun-plane grid, compacted per wave.  About one sun ray in
+// ten hits something, and those rays are spread over every wave, so a
+// lane-per-ray candidate loop runs the full element test in nearly every
+// wave for a few lanes.  Here each lane draws kMaskRays sun points and only
+// screens its candidates by their footprints; the (ray, element) pairs that
+// pass go to a per-wave queue in LDS, and the element tests run on the queue,
+// one pair per lane.  A ray hits when any of its pairs does, which is what
+// any_hit() answers for it.
+constexpr int kMaskRays = 4;
+constexpr int kMaskQueue = 128;      // ring of pairs per wave (holds up to 127)
+
+STD uint64_t lanes_below() { return (1ull << (threadIdx.x & 63)) - 1ull; }
+
+template <typename R, bool G>
+__global__ __launch_bounds__(kBlock) void sun_mask_grid_kernel(MaskArgs a) {
+    constexpr int W = kBlock / 64;
+    __shared__ uint32_t ring[W][kMaskQueue];
+    __shared__ double2 pts[W][kMaskRays * 64];
+    __shared__ uint32_t hitw[W][kMaskRays * 2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t ray0 = ((uint64_t)blockIdx.x * W + wv) * (64 * kMaskRays);
+    const SceneView &sv = a.sv;
+    const DevStage &st = sv.stages[0];
+    if (lane < kMaskRays * 2) hitw[wv][lane] = 0u;
+    int kb[kMaskRays], ke[kMaskRays];
+    uint32_t rays = 0, draws = 0;
+#pragma unroll
+    for (int r = 0; r < kMaskRays; r++) {
+        const uint64_t i = ray0 + (uint64_t)(r * 64 + lane);
+        double p[2] = {0.0, 0.0};
+        kb[r] = ke[r] = 0;
+        if (i < a.n) {
+            Rng rng;
+            rng.begin(a.seed, 0, a.s0 + i);
+            sun_plane_point(rng, sv.sun, p);
+            grid_cell_range(sv, p[0], p[1], &kb[r], &ke[r]);
+            rays++;
+            draws += rng.k;
+        }
+        pts[wv][r * 64 + lane] = make_double2(p[0], p[1]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t head = 0, count = 0, tests = 0;
+    // test the m pairs at the ring's head, one per lane
+    auto drain = [&](uint32_t m) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if ((uint32_t)lane < m) {
+            const uint32_t e = ring[wv][(head + lane) & (kMaskQueue - 1)];
+            const int slot = (int)(e >> 24), j = (int)(e & 0xFFFFFFu);
+            const double2 q = pts[wv][slot];
+            const double psun[2] = {q.x, q.y};
+            double pg[3], cg[3], ps[3], cs[3];
+            sun_ray_from_plane(sv.sun, st, psun, pg, cg);
+            to_local(pg, cg, st.org, st.R, ps, cs);
+            Best b;
+            b.path = 1e99;
+            b.j = 0x7fffffff;
+            b.tests = 0;
+            b.nodes = 0;
+            consider<R, G>(sv, st, j, ps, cs, b);
+            if (b.path < 1e99) atomicOr(&hitw[wv][(slot >> 6) * 2 + ((slot >> 5) & 1)], 1u << (slot & 31));
+        }
+        head += m;
+        count -= m;
+        tests += m;
+    };
+    int cr = 0, k = kb[0], kend = ke[0];
+    double csx = pts[wv][lane].x, csy = pts[wv][lane].y;
+    for (;;) {
+        while (k >= kend && cr < kMaskRays - 1) {
+            ++cr;
+            k = kb[cr];
+            kend = ke[cr];
+            const double2 q = pts[wv][cr * 64 + lane];
+            csx = q.x;
+            csy = q.y;
+        }
+        const bool active = k < kend;
+        if (__ballot(active) == 0) break;
+        bool pass = false;
+        int j = 0;
+        if (active) {
+            j = sv.grid_items[k];
+            ++k;
+            pass = foot_contains(sv.foot, j, csx, csy);
+        }
+        const uint64_t bits = __ballot(pass);
+        if (pass) {
+            const uint32_t pos = (head + count + (uint32_t)__popcll(bits & lanes_below())) & (kMaskQueue - 1);
+            ring[wv][pos] = ((uint32_t)(cr * 64 + lane) << 24) | (uint32_t)j;
+        }
+        count += (uint32_t)__popcll(bits);
+        if (count >= 64) drain(64);
+    }
+    if (count > 0) drain(count);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane < kMaskRays) {
+        const uint64_t w = ray0 / 64 + (uint64_t)lane;
+        if (w * 64 < a.n) a.mask[w] = (uint64_t)hitw[wv][lane * 2] | ((uint64_t)hitw[wv][lane * 2 + 1] << 32);
+    }
+    if (a.counters) {
+        atomicAdd(&a.counters[0], (unsigned long long)rays);
+        atomicAdd(&a.counters[3], (unsigned long long)draws);
+        if (lane == 0) atomicAdd(&a.counters[1], (unsigned long long)tests);
     }
 }
 
@@ -560,6 +673,13 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
 
 template <typename R>
 void launch_sun_mask(const MaskArgs &a, hipStream_t s) {
+    if (a.compact) {
+        const uint32_t per_block = kBlock * kMaskRays;
+        const uint32_t blocks = (a.n + per_block - 1) / per_block;
+        if (a.generic) hipLaunchKernelGGL((sun_mask_grid_kernel<R, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((sun_mask_grid_kernel<R, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+        return;
+    }
     const uint32_t blocks = (a.n + kBlock - 1) / kBlock;
     if (a.generic) hipLaunchKernelGGL((sun_mask_kernel<R, true>), dim3(blocks), dim3(kBlock), 0, s, a);
     else hipLaunchKernelGGL((sun_mask_kernel<R, false>), dim3(blocks), dim3(kBlock), 0, s, a);
