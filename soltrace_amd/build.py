@@ -69,7 +69,24 @@ def build(verbose: bool = False) -> str:
     if os.path.exists(lib):
         os.remove(lib)
     _run(["ar", "rcs", lib, *objs])
+    build_strace(so, verbose)
     return so
+
+
+def build_strace(so: str, verbose: bool = False) -> str:
+    """The strace command line (soltrace_amd/strace), host C++ over the
+    C API, linked against the shared library next to it."""
+    src = os.path.join(HERE, "strace")
+    exe = os.path.join(src, "strace")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(os.path.dirname(HERE), "include"),
+           os.path.join(src, "main.cpp"), os.path.join(src, "stinput.cpp"),
+           "-L" + os.path.dirname(so), "-l:" + os.path.basename(so),
+           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib", "-o", exe + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    os.replace(exe + ".tmp", exe)
+    return exe
 
 
 if __name__ == "__main__":

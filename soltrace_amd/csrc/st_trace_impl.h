@@ -248,6 +248,8 @@ __global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
 // any_hit() answers for it.
 constexpr int kMaskRays = 4;
 constexpr int kMaskQueue = 128;      // ring of pairs per wave (holds up to 127)
+// a pair packs its ray slot (r * 64 + lane) in 8 bits above a 24-bit element index
+static_assert(kMaskRays * 64 <= 256, "ray slot must fit in 8 bits");
 
 STD uint64_t lanes_below() { return (1ull << (threadIdx.x & 63)) - 1ull; }
 
@@ -554,7 +556,8 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
         b.tests = 0;
         b.nodes = 0;
         const bool first_sun = sun_stage && mhc == 0 && sv.use_grid && !sv.sun.point_source;
-        closest_hit<R, G>(sv, st, ps, cs, first_sun, first_sun, psun[0], psun[1], b);
+        // rays here are known to hit, so footprints would screen out little
+        closest_hit<R, G>(sv, st, ps, cs, first_sun, false, psun[0], psun[1], b);
         ev_tests += b.tests;
         ev_nodes += b.nodes;
         if (!(b.path < 1e99)) {
