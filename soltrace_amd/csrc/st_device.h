@@ -1147,31 +1147,33 @@ STD bool sunshape_d_try(uint32_t w0, uint32_t w1, uint32_t w2, const DevSun &sun
     return sunshape_d_screen(w0, w1, w2, sun, aux, t2out, &u2) && sunshape_d_exact(*t2out, u2, sun, aux);
 }
 
-// Tries 4q .. 4q+3 of the 'd' loop for the stream (seed, stage 0, sun_index)
-// whose loop starts at draw k0 (try j uses draws k0+3j .. k0+3j+2).  Returns
-// the first accepted try or -1; *theta2 is its (mrad^2) sample.  The four
-// screens run first and branch-free; the exact test then runs only on the
-// tries that survive them, in order.
+// Group q of the 'd' loop for the stream (seed, stage 0, sun_index) whose
+// loop starts at draw k0 (try j uses draws k0+3j .. k0+3j+2): tries
+// 4q - shift .. 4q - shift + 3, where shift = 3 k0 mod 4 puts the group's
+// twelve draws on three whole Philox blocks (tries before the first are
+// skipped).  Returns the first accepted try of the group as 0..3, or -1;
+// *theta2 is its (mrad^2) sample.  The screens run first and branch-free; the
+// exact test then runs only on the tries that survive them, in order.
+STD uint32_t sunshape_d_shift(uint32_t k0) { return (3u * k0) & 3u; }
+
 STD int sunshape_d_group(uint32_t seed, uint64_t sun_index, uint32_t k0, uint32_t q, const DevSun &sun,
                          const double *aux, double *theta2) {
-    const uint32_t kfirst = k0 + 12u * q;
-    uint32_t cur = 0xffffffffu;
-    uint32_t b[4];
-    auto word = [&](uint32_t k) {
-        if ((k >> 2) != cur) {
-            cur = k >> 2;
-            b[0] = (uint32_t)sun_index; b[1] = (uint32_t)(sun_index >> 32); b[2] = 0u; b[3] = cur;
-            philox10(b, seed, 0x53544F52u);
-        }
-        return pick4(b, k & 3u);
-    };
+    const int64_t j0 = 4 * (int64_t)q - (int64_t)sunshape_d_shift(k0);
+    const uint32_t blk0 = (uint32_t)((k0 + 3 * j0) >> 2);  // k0 + 3 j0 is a multiple of 4 and >= 0 for q >= 1
+    uint32_t w[12];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        uint32_t b[4] = {(uint32_t)sun_index, (uint32_t)(sun_index >> 32), 0u, blk0 + (uint32_t)c};
+        philox10(b, seed, 0x53544F52u);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w[4 * c + i] = b[i];
+    }
     double t2s[4], u2s[4];
     uint32_t cand = 0;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const uint32_t k = kfirst + 3u * (uint32_t)t;
-        const uint32_t w0 = word(k), w1 = word(k + 1), w2 = word(k + 2);
-        if (sunshape_d_screen(w0, w1, w2, sun, aux, &t2s[t], &u2s[t])) cand |= 1u << t;
+        if (j0 + t < 0) continue;
+        if (sunshape_d_screen(w[3 * t], w[3 * t + 1], w[3 * t + 2], sun, aux, &t2s[t], &u2s[t])) cand |= 1u << t;
     }
     while (cand != 0u) {
         const int t = __ffs(cand) - 1;

@@ -24,7 +24,7 @@
 namespace st {
 
 static constexpr int kBlock = 256;
-static constexpr int64_t kNoRecord = -(int64_t(1) << 62);  // a record lost to a full overflow buffer
+static constexpr int32_t kNoRecord = INT32_MIN;  // a record lost to a full overflow buffer
 
 // ------------------------------------------------------ sun-ray source ----
 // coretrace/generateray.cpp:58-161 (the sun-plane draws are full 53-bit):
@@ -59,14 +59,17 @@ STD void gen_sun_ray(Rng &rng, const DevSun &sun, const DevStage &st, double pg[
     sun_ray_from_plane(sun, st, psun, pg, cg);
 }
 
-// Candidate range of the sun-plane grid cell holding (sx, sy).
-STD void grid_cell_range(const SceneView &sv, double sx, double sy, int *k, int *kend) {
+// The sun-plane grid cell holding (sx, sy), and its candidate range.
+STD int grid_cell(const SceneView &sv, double sx, double sy) {
     const DevSunGrid &g = sv.grid;
     int cx = (int)((sx - g.x0) * g.inv_dx);
     int cy = (int)((sy - g.y0) * g.inv_dy);
     cx = cx < 0 ? 0 : (cx >= g.nx ? g.nx - 1 : cx);
     cy = cy < 0 ? 0 : (cy >= g.ny ? g.ny - 1 : cy);
-    const int c = cy * g.nx + cx;
+    return cy * g.nx + cx;
+}
+STD void grid_cell_range(const SceneView &sv, double sx, double sy, int *k, int *kend) {
+    const int c = grid_cell(sv, sx, sy);
     *k = sv.grid_cells[c];
     *kend = sv.grid_cells[c + 1];
 }
@@ -129,21 +132,22 @@ STD bool foot_contains(const DevFoot *foot, int j, double sx, double sy) {
     return g.kind == 0 ? in : g.kind == 1;
 }
 
-// Candidates: the sun-plane grid cell for a first hit of a sun ray (grid),
-// screened by the elements' footprints; the stage BVH when present; every
-// element otherwise.  One loop feeds them all to a single consider() so the
-// element code is instantiated once.
+// Candidates: sun-plane grid cell `cell` for a first hit of a sun ray (cell
+// >= 0), screened by the elements' footprints at (sx, sy) when `screen`; the
+// stage BVH when present; every element otherwise.  One loop feeds them all
+// to a single consider() so the element code is instantiated once.
 template <typename R, bool G>
-STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3], const double cs[3], bool grid,
+STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3], const double cs[3], int cell,
                      bool screen, double sx, double sy, Best &b) {
     b.path = 1e99;
     b.j = 0x7fffffff;
     const int32_t *items = nullptr;  // nullptr: the item is the element index
     int k = 0, kend = 0;
     bool bvh = false;
-    if (grid) {
+    if (cell >= 0) {
         items = sv.grid_items;
-        grid_cell_range(sv, sx, sy, &k, &kend);
+        k = sv.grid_cells[cell];
+        kend = sv.grid_cells[cell + 1];
     } else if (st.nbvh > 0) {
         bvh = true;
         items = sv.bvh_items;
@@ -206,7 +210,7 @@ STD bool any_hit(const SceneView &sv, const DevStage &st, const double ps[3], co
     Best b;
     b.tests = 0;
     b.nodes = 0;
-    closest_hit<R, G>(sv, st, ps, cs, grid, grid, sx, sy, b);
+    closest_hit<R, G>(sv, st, ps, cs, grid ? grid_cell(sv, sx, sy) : -1, grid, sx, sy, b);
     *tests += b.tests;
     return b.path < 1e99;
 }
@@ -411,7 +415,7 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
         if (owner) {
             if (hit) {
                 const uint32_t gg = next_g + (uint32_t)((first - r_own) / m);
-                const uint32_t jj = 4u * gg + (uint32_t)off_first;
+                const uint32_t jj = 4u * gg + (uint32_t)off_first - sunshape_d_shift(a.k0);
                 SunPre p;
                 p.theta2 = t2_first / 1000000.0;
                 p.kend = a.k0 + 3u * (jj + 1u);
@@ -495,26 +499,32 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
     const bool sun_stage = (a.si == 0);
     constexpr bool FAST = sizeof(R) == 4;
 
-    double ps[3], cs[3], psun[2];  // ray in the stage frame
+    // Loop-carried state is kept small for occupancy: the sun-ray index is
+    // read again at the end, the first hit's grid cell replaces the
+    // sun-plane point, and event counts live in LDS.
+    double ps[3], cs[3];  // ray in the stage frame
     uint32_t raynum;
-    uint64_t sun_index;
+    int cell = -1;        // sun-plane grid cell of a sun ray's first hit
     Rng rng;
     {
         double pg[3], cg[3];
         if (sun_stage) {
-            sun_index = a.sun_index[t];
+            const uint64_t sun_index = a.sun_index[t];
             raynum = a.raynum_base + (uint32_t)t + 1u;
             rng.begin(a.seed, 0, sun_index);
+            double psun[2];
             gen_sun_ray(rng, sv.sun, st, pg, cg, psun);
+            if (sv.use_grid && !sv.sun.point_source) cell = grid_cell(sv, psun[0], psun[1]);
         } else {
             const DevRay r = a.in_rays[t];
             raynum = r.raynum;
-            sun_index = r.sun_index;
             for (int i = 0; i < 3; i++) { pg[i] = r.pos[i]; cg[i] = r.cos[i]; }
-            rng.begin(a.seed, (uint32_t)a.si, sun_index);
+            rng.begin(a.seed, (uint32_t)a.si, r.sun_index);
         }
         to_local(pg, cg, st.org, st.R, ps, cs);
     }
+    __shared__ uint32_t ev_lds[3][kBlock];  // element tests, BVH nodes, draws made by the sampler
+    if (a.counters) ev_lds[0][threadIdx.x] = ev_lds[1][threadIdx.x] = ev_lds[2][threadIdx.x] = 0u;
 
     double cos_out_el[3] = {0.0, 0.0, 0.0};  // CosRayOutElement, kept across hits
     uint32_t nrec = 0;
@@ -522,14 +532,14 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
     bool passed = false;
     // records go out as they are made; an absorbed hit later rewrites the
     // element field of its own record (slot index, or -1 - overflow index)
-    int64_t last_loc = 0;
+    int32_t last_loc = 0;  // < 2^24 slots and overflow entries per launch
     auto emit = [&](const double pos[3], const double cos[3], int element) {
         DevRecord r;
         for (int i = 0; i < 3; i++) { r.pos[i] = pos[i]; r.cos[i] = cos[i]; }
         r.element = element;
         r.raynum = raynum;
         if (nrec < (uint32_t)a.slots) {
-            last_loc = (int64_t)t * a.slots + nrec;
+            last_loc = t * a.slots + (int32_t)nrec;
             a.rec_slots[last_loc] = r;
         } else {
             uint32_t o = atomicAdd(a.ovf_count, 1u);
@@ -537,7 +547,7 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
                 a.ovf_rec[o] = r;
                 a.ovf_meta[2 * o] = (uint32_t)t;
                 a.ovf_meta[2 * o + 1] = nrec;
-                last_loc = -1 - (int64_t)o;
+                last_loc = -1 - (int32_t)o;
             } else {
                 atomicOr(a.err_flags, 4u);
                 last_loc = kNoRecord;
@@ -550,16 +560,16 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
     int fbin = -1;
     bool fabs_ = false;
 
-    uint32_t ev_tests = 0, ev_nodes = 0, skipped = 0;
     for (;;) {
         Best b;
         b.tests = 0;
         b.nodes = 0;
-        const bool first_sun = sun_stage && mhc == 0 && sv.use_grid && !sv.sun.point_source;
         // rays here are known to hit, so footprints would screen out little
-        closest_hit<R, G>(sv, st, ps, cs, first_sun, false, psun[0], psun[1], b);
-        ev_tests += b.tests;
-        ev_nodes += b.nodes;
+        closest_hit<R, G>(sv, st, ps, cs, mhc == 0 ? cell : -1, false, 0.0, 0.0, b);
+        if (a.counters) {
+            ev_lds[0][threadIdx.x] += b.tests;
+            ev_lds[1][threadIdx.x] += b.nodes;
+        }
         if (!(b.path < 1e99)) {
             if (sun_stage) {
                 if (mhc == 0) atomicOr(a.err_flags, 2u);
@@ -622,7 +632,7 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
             const bool use_pre = sun_err && a.sun_pre != nullptr && rng.k == 3u && a.sun_pre[t].ok;
             if (use_pre) {
                 pre = a.sun_pre[t].theta2;
-                skipped = a.sun_pre[t].kend - rng.k;  // drawn by the sampler kernel
+                if (a.counters) ev_lds[2][threadIdx.x] = a.sun_pre[t].kend - rng.k;  // drawn by the sampler kernel
                 rng.seek(a.sun_pre[t].kend);
             }
             const bool diffuse = (f.dist == 'f' || f.dist == 'F');
@@ -654,9 +664,9 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
     a.out_flag[t] = passed ? 1u : 0u;
     if (a.counters) {
         atomicAdd(&a.counters[0], 1ull);
-        atomicAdd(&a.counters[1], (unsigned long long)ev_tests);
-        atomicAdd(&a.counters[2], (unsigned long long)ev_nodes);
-        atomicAdd(&a.counters[3], (unsigned long long)(rng.k - skipped));
+        atomicAdd(&a.counters[1], (unsigned long long)ev_lds[0][threadIdx.x]);
+        atomicAdd(&a.counters[2], (unsigned long long)ev_lds[1][threadIdx.x]);
+        atomicAdd(&a.counters[3], (unsigned long long)(rng.k - ev_lds[2][threadIdx.x]));
         atomicAdd(&a.counters[4], (unsigned long long)nrec);
     }
     if (passed) {
@@ -667,7 +677,7 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
             to_ref(ps, cs, st.org, st.R, r.pos, r.cos);
             r.raynum = raynum;
             r.pad = 0;
-            r.sun_index = sun_index;
+            r.sun_index = sun_stage ? a.sun_index[t] : a.in_rays[t].sun_index;
         }
         a.out_slots[t] = r;
     }
