@@ -40,6 +40,41 @@ int st_amd_flux_request(st_context_t pcxt, int stage, int elem, int nx, int ny, 
  * NULL) receives xmin, ymin, bin width, bin height.  Returns nx*ny or -1. */
 int st_amd_flux_result(st_context_t pcxt, unsigned int *counts, double *bin_dims);
 
+/* The GUI's flux statistics of element `elem` (0-based) of stage `stage`
+ * over the last run's records (app/src/project.cpp:1139-1436,
+ * ElementStatistics::Compute and BinRaysXY), binned on the GPU where the
+ * records are.  Flat ('f') and cylindrical ('t') surfaces and partial
+ * cylinders ('s' with aperture 'l'), as the GUI allows.
+ *   nx, ny         bins (> 1 each)
+ *   autoscale      1: x/y range from every record on the element; 0: from
+ *                  bounds
+ *   absorbed_only  1: bin only absorbed last records
+ *   dni            W/m2, for the power per ray
+ *   bounds[4]      in (autoscale 0) and out: xmin, xmax, ymin, ymax, after
+ *                  the cylinder unroll
+ *   grid           (may be NULL) nx*ny ray counts, grid[ix*ny + iy]
+ *   stats[15]      power per ray, peak flux, min flux, average flux, sigma,
+ *                  uniformity, peak flux uncertainty %, average flux
+ *                  uncertainty %, centroid x, y, z (element frame, to 1e-6 m),
+ *                  rays binned, z value, bin size x, bin size y
+ * Returns 1, or -1 for an element the GUI cannot bin, bad sizes, an empty
+ * range or no run. */
+int st_amd_element_stats(st_context_t pcxt, int stage, int elem, int nx, int ny, int autoscale, int absorbed_only,
+                         double dni, double bounds[4], unsigned int *grid, double *stats);
+
+/* Read a SolTrace .stinput system into the context with the headless strace
+ * reader (coretrace/strace/main.cpp:358-397).  Returns 1, or -1 when the file
+ * cannot be opened or is malformed. */
+int st_amd_read_stinput(st_context_t pcxt, const char *path);
+
+/* Write the last run's records to a file: format 0 is strace's .rays text
+ * (coretrace/strace/main.cpp:399-483), format 1 the GUI's binary ray file
+ * (app/src/project.cpp:915-948: six doubles SunXMin, SunXMax, SunYMin,
+ * SunYMax, SunRayCount, Length, then nine doubles per record: x, y, z,
+ * cos x, cos y, cos z, stage, element, ray number; native byte order).
+ * Returns 1, or -1 on a bad format or a failed write. */
+int st_amd_write_ray_file(st_context_t pcxt, const char *path, int format);
+
 /* 0-based sun-ray index of rays 1..n of the last run (out[raynum-1]).
  * Returns the number written, or -1. */
 int st_amd_ray_sun_index(st_context_t pcxt, unsigned long long *out, int n);

@@ -779,4 +779,123 @@ int st_oracle_flux(st_context_t p, int stage_idx, int elem_idx, int nx, int ny, 
     }
     return 1;
 }
+
+// The GUI's element statistics in full (app/src/project.cpp:1139-1436):
+// the binning above, then the centroid and z value of BinRaysXY
+// (:1340-1436) and the flux statistics of Compute (:1248-1288), serially in
+// record order.  Same signature and stats[15] layout as the library's
+// st_amd_element_stats.
+int st_oracle_element_stats(st_context_t p, int stage_idx, int elem_idx, int nx, int ny, int autoscale,
+                            int absorbed_only, double dni, double bounds[4], unsigned int *grid, double *stats) {
+    SYS(p, -1);
+    if (stage_idx < 0 || stage_idx >= (int)sys->stages.size()) return -1;
+    Stage &st = *sys->stages[stage_idx];
+    if (elem_idx < 0 || elem_idx >= (int)st.elements.size()) return -1;
+    Element &e = *st.elements[elem_idx];
+    char surf = (char)tolower(e.surf), aper = (char)tolower(e.shape);
+    if (surf != 't' && surf != 'f' && !(aper == 'l' && surf == 's')) return -1;
+    const std::vector<Record> &rd = sys->all;
+    double minx = bounds[0], maxx = bounds[1], miny = bounds[2], maxy = bounds[3];
+    if (autoscale) {
+        minx = miny = 1e199;
+        maxx = maxy = -1e199;
+        for (size_t i = 0; i < rd.size(); i++) {
+            if (std::abs(rd[i].element) == elem_idx + 1 && rd[i].stage == stage_idx + 1) {
+                double pe[3], ce[3];
+                to_local(rd[i].pos, rd[i].cos, e.origin, e.R, pe, ce);
+                if (pe[0] < minx) minx = pe[0];
+                if (pe[0] > maxx) maxx = pe[0];
+                if (pe[1] < miny) miny = pe[1];
+                if (pe[1] > maxy) maxy = pe[1];
+            }
+        }
+    }
+    if (nx <= 1 || ny <= 1 || maxx <= minx || maxy <= miny) return -1;
+    double gx = maxx - minx, gy = maxy - miny;
+    double radius = 0.0;
+    if (surf == 't' || surf == 's') {
+        radius = 1.0 / (surf == 't' ? e.curv_rev : e.vcx);
+        if (surf == 't') {
+            minx = -M_PI * radius;
+            maxx = M_PI * radius;
+            gx = 2.0 * M_PI * radius;
+        } else {
+            double amin = asin(e.p[0] / radius), amax = asin(e.p[1] / radius);
+            minx = radius * amin;
+            maxx = radius * amax;
+            gx = (amax - amin) * radius;
+        }
+    }
+    const double bx = gx / nx, by = gy / ny;
+    std::vector<double> fg((size_t)nx * ny, 0.0);
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < rd.size(); i++)
+        if (std::abs(rd[i].element) == elem_idx + 1 && rd[i].stage == stage_idx + 1) idx.push_back(i);
+    const bool cyl = surf == 't', pcyl = (surf == 's' && aper == 'l');
+    size_t binned = 0, npoints = 0;
+    double cen[3] = {0, 0, 0}, zval = 0.0;
+    for (size_t j = 0; j < idx.size(); j++) {
+        if (!(j == idx.size() - 1 || rd[idx[j]].raynum != rd[idx[j + 1]].raynum)) continue;
+        const Record &r = rd[idx[j]];
+        if (absorbed_only && r.element >= 0) continue;
+        double pe[3], ce[3];
+        to_local(r.pos, r.cos, e.origin, e.R, pe, ce);
+        double x = pe[0], y = pe[1], z = pe[2];
+        cen[0] += x;
+        cen[1] += y;
+        cen[2] += z;
+        npoints++;
+        if (cyl) {
+            zval = radius;
+            if (z <= radius)
+                x = radius * asin(x / radius);
+            else if (z > radius) {
+                if (x < 0) x = -(M_PI * radius / 2.0 + radius * acos(fabs(x) / radius));
+                if (x >= 0) x = M_PI * radius / 2.0 + radius * acos(x / radius);
+            }
+        } else if (pcyl) {
+            zval = radius;
+            x = radius * asin(x / radius);
+        } else {
+            zval = z;
+        }
+        int gxi = -1, gyi = -1;
+        while ((minx + (gxi + 1) * bx) < x) gxi++;
+        while ((miny + (gyi + 1) * by) < y) gyi++;
+        if (gxi >= 0 && gxi < nx && gyi >= 0 && gyi < ny) {
+            fg[(size_t)gxi * ny + gyi] += 1;
+            binned++;
+        }
+    }
+    if (npoints > 0)
+        for (int k = 0; k < 3; k++) cen[k] /= (double)npoints;
+    const double ppr = dni * (sys->sun.maxx - sys->sun.minx) * (sys->sun.maxy - sys->sun.miny) / sys->sunraycount;
+    const double zscale = ppr / (bx * by);
+    double peak = 0, sum = 0, sum2 = 0, minf = 1e99;
+    int npeak = -1;
+    for (int r = 0; r < nx; r++)
+        for (int c = 0; c < ny; c++) {
+            double z = fg[(size_t)r * ny + c] * zscale;
+            sum += z;
+            sum2 += z * z;
+            if (z > peak) {
+                peak = z;
+                npeak = (int)fg[(size_t)r * ny + c];
+            }
+            if (z < minf) minf = z;
+        }
+    const double nb = (double)nx * (double)ny;
+    const double ave = sum / nb;
+    const double sigma = sqrt((nb * sum2 - sum * sum) / (nb * nb));
+    const double out[15] = {ppr, peak, minf, ave, sigma, sigma / ave, 100.0 / sqrt((double)npeak),
+                            100.0 / sqrt((double)rd.size()), cen[0], cen[1], cen[2], (double)binned, zval, bx, by};
+    for (int k = 0; k < 15; k++) stats[k] = out[k];
+    bounds[0] = minx;
+    bounds[1] = maxx;
+    bounds[2] = miny;
+    bounds[3] = maxy;
+    if (grid)
+        for (size_t k = 0; k < fg.size(); k++) grid[k] = (unsigned int)fg[k];
+    return 1;
+}
 }  // extern "C"

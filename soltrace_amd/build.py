@@ -18,10 +18,11 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("SOLTRACE_AMD_ARCH", "gfx950")
 
-SOURCES = ["st_trace_f64.hip", "st_trace_f32.hip", "st_compact.hip", "st_scene.cpp", "st_context.cpp"]
+SOURCES = ["st_trace_f64.hip", "st_trace_f32.hip", "st_compact.hip", "st_stats.hip", "st_scene.cpp", "st_context.cpp"]
 # The FP64 trace keeps separate multiply/add roundings (no FMA contraction),
 # the arithmetic the reference's CPU build performs.
-EXTRA = {"st_trace_f64.hip": ["-ffp-contract=off"], "st_trace_f32.hip": ["-ffp-contract=fast"]}
+EXTRA = {"st_trace_f64.hip": ["-ffp-contract=off"], "st_trace_f32.hip": ["-ffp-contract=fast"],
+         "st_stats.hip": ["-ffp-contract=off"]}
 
 # -packed-fp32-ops off: packed FP32 forms are excluded from every kernel.
 HIP_FLAGS = [
@@ -49,6 +50,13 @@ def _run(cmd):
     return r
 
 
+# Host C++ built into the library beside the C API: the .stinput reader and
+# the ray-file writers (they call only stapi.h).
+STRACE = os.path.join(HERE, "strace")
+HOST_SOURCES = ["stinput.cpp", "stinput_capi.cpp"]
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     cc = hipcc()
@@ -57,6 +65,14 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
         cmd = [cc, *lang, *HIP_FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run(cmd)
+        objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + INCLUDE, "-c", os.path.join(STRACE, src),
+               "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         _run(cmd)
@@ -75,11 +91,10 @@ def build(verbose: bool = False) -> str:
 
 def build_strace(so: str, verbose: bool = False) -> str:
     """The strace command line (soltrace_amd/strace), host C++ over the
-    C API, linked against the shared library next to it."""
+    C API; its reader and writer are those built into the shared library."""
     src = os.path.join(HERE, "strace")
     exe = os.path.join(src, "strace")
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(os.path.dirname(HERE), "include"),
-           os.path.join(src, "main.cpp"), os.path.join(src, "stinput.cpp"),
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, os.path.join(src, "main.cpp"),
            "-L" + os.path.dirname(so), "-l:" + os.path.basename(so),
            "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib", "-o", exe + ".tmp"]
     if verbose:

@@ -113,6 +113,13 @@ void launch_compact_records(const DevRecord *slots, const uint32_t *count, const
                             int nslots, DevRecord *out, hipStream_t s);
 void launch_compact_overflow(const DevRecord *ovf, const uint32_t *meta, const uint32_t *n_ovf_dev, uint32_t cap,
                              const uint32_t *offsets, DevRecord *out, hipStream_t s);
+// Flux statistics over resident records (st_stats.hip).  mm: min x, max x,
+// min y, max y as order keys; acc: centroid sums (um), considered, binned,
+// 1 + last considered index.
+void launch_stats_bounds(const DevRecord *rec, uint32_t n, int32_t elem, const FluxSpec &fl, unsigned long long *mm,
+                         hipStream_t s);
+void launch_stats_bin(const DevRecord *rec, uint32_t n, int32_t elem, const FluxSpec &fl, uint64_t base,
+                      uint32_t *bins, unsigned long long *acc, hipStream_t s);
 void launch_compact_rays(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, int n_rays, DevRay *out,
                          hipStream_t s);
 

@@ -495,7 +495,7 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= a.n_rays) return;
     const SceneView &sv = a.sv;
-    const DevStage st = sv.stages[a.si];
+    const DevStage &st = sv.stages[a.si];  // read where used: a copy would hold ~33 scalar registers
     const bool sun_stage = (a.si == 0);
     constexpr bool FAST = sizeof(R) == 4;
 

@@ -250,6 +250,42 @@ bool write_rays(const char *path, st_context_t cxt) {
     return ok;
 }
 
+bool write_rays_binary(const char *path, st_context_t cxt) {
+    FILE *out = fopen(path, "wb");
+    if (!out) return false;
+    double xmin = 0, xmax = 0, ymin = 0, ymax = 0;
+    int nsun = 0;
+    st_sun_stats(cxt, &xmin, &xmax, &ymin, &ymax, &nsun);
+    const int n = std::max(0, st_num_intersections(cxt));
+    std::vector<double> px((size_t)n), py((size_t)n), pz((size_t)n), cx((size_t)n), cy((size_t)n), cz((size_t)n);
+    std::vector<int> elem((size_t)n), stage((size_t)n), ray((size_t)n);
+    if (n > 0) {
+        st_locations(cxt, px.data(), py.data(), pz.data());
+        st_cosines(cxt, cx.data(), cy.data(), cz.data());
+        st_elementmap(cxt, elem.data());
+        st_stagemap(cxt, stage.data());
+        st_raynumbers(cxt, ray.data());
+    }
+    const double head[6] = {xmin, xmax, ymin, ymax, (double)nsun, (double)n};
+    bool ok = fwrite(head, sizeof(double), 6, out) == 6;
+    // records in blocks, each record nine doubles as the GUI writes them
+    std::vector<double> buf;
+    const int kBlock = 65536;
+    for (int i0 = 0; ok && i0 < n; i0 += kBlock) {
+        const int m = std::min(kBlock, n - i0);
+        buf.resize((size_t)m * 9);
+        for (int k = 0; k < m; k++) {
+            const int i = i0 + k;
+            double *r = &buf[(size_t)k * 9];
+            r[0] = px[i]; r[1] = py[i]; r[2] = pz[i];
+            r[3] = cx[i]; r[4] = cy[i]; r[5] = cz[i];
+            r[6] = stage[i]; r[7] = elem[i]; r[8] = ray[i];
+        }
+        ok = fwrite(buf.data(), sizeof(double), buf.size(), out) == buf.size();
+    }
+    return fclose(out) == 0 && ok;
+}
+
 namespace {
 int progress(st_uint_t, st_uint_t, st_uint_t, st_uint_t curstage, st_uint_t, void *data) {
     long *last = static_cast<long *>(data);
@@ -261,7 +297,8 @@ int progress(st_uint_t, st_uint_t, st_uint_t, st_uint_t curstage, st_uint_t, voi
 }
 }  // namespace
 
-int strace_main(int argc, char **argv, void (*prepare)(st_context_t cxt)) {
+int strace_main(int argc, char **argv, void (*prepare)(st_context_t cxt),
+                void (*after)(st_context_t cxt, const char *file)) {
     if (argc < 7) {
         printf("strace: too few arguments.  usage:\n\t"
                "strace <system.stinput> <number of rays> <max rays> <seed> <sunshape? 0/1> <errors? 0/1> "
@@ -296,6 +333,7 @@ int strace_main(int argc, char **argv, void (*prepare)(st_context_t cxt)) {
         printf("trace finished with code %d\n", code);
         const std::string out = std::string(file) + ".rays";
         if (!write_rays(out.c_str(), cxt)) printf("failed to write %s\n", out.c_str());
+        if (after) after(cxt, file);
     } else {
         print_messages(cxt, stdout);
     }

@@ -20,11 +20,20 @@ bool read_system(FILE *fp, st_context_t cxt, FILE *log);
 // cannot be opened.
 bool write_rays(const char *path, st_context_t cxt);
 
+// Write the context's last run as the GUI's binary ray file
+// (app/src/project.cpp:915-948, RayData::WriteDataFile): six doubles
+// SunXMin, SunXMax, SunYMin, SunYMax, SunRayCount, Length, then nine doubles
+// per record: x, y, z, cos x, cos y, cos z, stage, element, ray number.
+// False when the file cannot be written.
+bool write_rays_binary(const char *path, st_context_t cxt);
+
 // The strace command line: <system.stinput> <rays> <max rays> <seed>
 // <sunshape 0/1> <errors 0/1> [<power tower 0/1>].  Reads the system, runs
 // it and writes <system.stinput>.rays.  Returns the process exit code.
 // prepare (may be null) is called on the context after the system is read
-// and before the run.
-int strace_main(int argc, char **argv, void (*prepare)(st_context_t cxt));
+// and before the run; after (may be null) once the run succeeded, with the
+// system's path.
+int strace_main(int argc, char **argv, void (*prepare)(st_context_t cxt),
+                void (*after)(st_context_t cxt, const char *file) = nullptr);
 
 }  // namespace st_io

@@ -1,0 +1,81 @@
+"""The GUI's element statistics (app/src/project.cpp:1139-1436) restated in
+the oracle, pinned by what uniform illumination fixes.  A 10 x 10 m flat
+absorber under an overhead pillbox sun fills half of the sun-plane box (the
+box pads the rectangle by its half-diagonal, coretrace/suntoprimarystage.cpp),
+so every hit carries DNI x box area / sun rays and the average flux over the
+element is DNI within binomial noise; bin counts scatter as Poisson, so the
+uniformity (sigma / average) is 1/sqrt(rays per bin); the centroid is the
+element's centre and the z value its plane."""
+import math
+
+import numpy as np
+import pytest
+
+from flux_stats import element_stats
+from harness import run_scene
+from soltrace_amd.scene import Element, Optic, OpticFace, Scene, Stage
+
+
+def absorber_scene(n):
+    s = Scene(sun_shape="p", sun_sigma=4.65, sun_xyz=(0.0, 0.0, 100.0), raycount=n, maxcount=100 * n,
+              sunshape_err=False, optics_err=False)
+    s.optics = [Optic("abs", OpticFace(dist="g", refl=0.0), OpticFace(dist="g", refl=0.0))]
+    s.stages = [Stage(elements=[Element(xyz=(0, 0, 0), aim=(0, 0, 1), aperture="r", ap_params=(10.0, 10.0),
+                                        surface="f", optic="abs")])]
+    return s
+
+
+def test_uniform_absorber_statistics(oracle_lib):
+    n, nx, dni = 200_000, 20, 1000.0
+    out = {}
+
+    def after(lib, ctx):
+        out["fixed"] = element_stats(lib, ctx, "st_oracle_element_stats", 0, 0, nx, nx, 0, 1, dni,
+                                     (-5.0, 5.0, -5.0, 5.0))
+        out["auto"] = element_stats(lib, ctx, "st_oracle_element_stats", 0, 0, nx, nx, 1, 1, dni)
+
+    nrec, r, msgs, _ = run_scene(oracle_lib, absorber_scene(n), 5, after=after)
+    assert nrec == n, msgs
+    bounds, grid, st = out["fixed"]
+    sun_rays = r.nsunrays
+    assert grid.sum() == n and st["binned"] == n
+    # power per ray: DNI x sun box / sun rays (box 2 x 7.071 m square)
+    assert st["power_per_ray"] == pytest.approx(dni * 200.0 / sun_rays, rel=1e-9)
+    frac = n / sun_rays
+    assert abs(frac - 0.5) < 4 * math.sqrt(0.25 / sun_rays)
+    assert st["ave"] == pytest.approx(dni * 2.0 * frac, rel=1e-12)
+    assert st["uniformity"] == pytest.approx(1.0 / math.sqrt(n / nx ** 2), rel=0.15)
+    assert st["peak"] == pytest.approx(grid.max() * st["power_per_ray"] / (st["bx"] * st["by"]), rel=1e-12)
+    assert st["peak_unc"] == pytest.approx(100.0 / math.sqrt(grid.max()), rel=1e-12)
+    assert abs(st["cx"]) < 0.05 and abs(st["cy"]) < 0.05 and abs(st["cz"]) < 1e-9 and abs(st["zval"]) < 1e-9
+    # autoscale: the range of the hits themselves, inside the 10 m square
+    ab, ag, ast = out["auto"]
+    assert -5.0 <= ab[0] < -4.99 and 4.99 < ab[1] <= 5.0 and -5.0 <= ab[2] < -4.99 and 4.99 < ab[3] <= 5.0
+    assert ag.sum() + (n - ast["binned"]) == n
+
+
+@pytest.mark.parametrize("autoscale", [0, 1])
+def test_stats_grid_equals_flux_binning(oracle_lib, autoscale):
+    # the trough's absorber tube (a full cylinder, x unrolled to arc length)
+    import ctypes as C
+    from harness import counter_mode
+    from soltrace_amd import workloads
+    nx = ny = 30
+    out = {}
+
+    def after(lib, ctx):
+        out["stats"] = element_stats(lib, ctx, "st_oracle_element_stats", 1, 0, nx, ny, autoscale, 0, 950.0,
+                                     (-1.0, 1.0, -6.0, 6.0))
+        vals = [C.c_double(v) for v in (-1.0, 1.0, -6.0, 6.0)]
+        counts = (C.c_double * (nx * ny))()
+        st = (C.c_double * 3)()
+        assert lib.dll.st_oracle_flux(ctx, 1, 0, nx, ny, 0, autoscale, *[C.byref(v) for v in vals], counts, st) == 1
+        out["flux"] = np.frombuffer(counts, dtype=np.float64).reshape(nx, ny).copy(), [v.value for v in vals], list(st)
+
+    run_scene(oracle_lib, workloads.trough(raycount=20000), 3, setup=counter_mode, after=after)
+    bounds, grid, st = out["stats"]
+    counts, fb, fst = out["flux"]
+    np.testing.assert_array_equal(grid, counts)
+    assert list(bounds) == [fb[0], fb[1], fb[2], fb[3]]
+    assert st["bx"] == fst[0] and st["by"] == fst[1] and st["binned"] == fst[2] > 0
+    assert st["zval"] == pytest.approx(0.035, rel=1e-12)  # a cylinder's z value is its radius

@@ -41,3 +41,42 @@ def test_strace_hip_equals_oracle(hip_lib, tmp_path, name):
     np.testing.assert_array_equal(rg[:, 6:], ro[:, 6:])
     tol = 1e-6 + 1e-6 * np.abs(ro[:, :6])
     assert np.all(np.abs(rg[:, :6] - ro[:, :6]) <= tol), np.abs(rg[:, :6] - ro[:, :6]).max()
+
+
+def test_read_stinput_and_ray_files_through_the_library(hip_lib, tmp_path):
+    # st_amd_read_stinput + st_sim_run + st_amd_write_ray_file give the CLI's
+    # .rays text byte for byte, and the binary file holds the getters' doubles
+    import ctypes as C
+    import shutil
+
+    from soltrace_amd import stapi
+    from test_strace import GOLDEN, read_binary_rays
+    d = hip_lib.dll
+    d.st_amd_read_stinput.argtypes = [C.c_void_p, C.c_char_p]
+    d.st_amd_write_ray_file.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+    args = ["2000", "100000", "5", "1", "1", "0"]
+    (tmp_path / "cli").mkdir()
+    cli_text, r = run_cli(STRACE, tmp_path / "cli", "lens", args)
+    assert r.returncode == 0 and cli_text is not None, r.stdout
+    src = tmp_path / "lens.stinput"
+    shutil.copy(os.path.join(GOLDEN, "lens.stinput"), src)
+    ctx = hip_lib.st_create_context()
+    try:
+        assert d.st_amd_read_stinput(ctx, str(src).encode()) == 1
+        hip_lib.st_sim_params(ctx, 2000, 100000, 0)
+        hip_lib.st_sim_errors(ctx, 1, 1)
+        assert stapi.run(hip_lib, ctx, 5) > 0
+        assert d.st_amd_write_ray_file(ctx, str(tmp_path / "lib.rays").encode(), 0) == 1
+        assert d.st_amd_write_ray_file(ctx, str(tmp_path / "lib.bin").encode(), 1) == 1
+        assert d.st_amd_write_ray_file(ctx, str(tmp_path / "bad").encode(), 7) == -1
+        res = stapi.fetch_results(hip_lib, ctx)
+    finally:
+        hip_lib.st_free_context(ctx)
+    assert (tmp_path / "lib.rays").read_text() == cli_text
+    head, rec = read_binary_rays(tmp_path / "lib.bin")
+    assert list(head) == [*res.sun_box, res.nsunrays, len(res)]
+    np.testing.assert_array_equal(rec[:, 0:3], res.pos)
+    np.testing.assert_array_equal(rec[:, 3:6], res.cos)
+    np.testing.assert_array_equal(rec[:, 6], res.stage)
+    np.testing.assert_array_equal(rec[:, 7], res.element)
+    np.testing.assert_array_equal(rec[:, 8], res.raynum)

@@ -101,3 +101,32 @@ def test_reader_rejects_headerless_and_short_lines(tmp_path):
     short.write_text("\n".join(lines))
     r = subprocess.run([STRACE_ORACLE, str(short), "10", "1000", "1", "1", "1", "0"], capture_output=True, text=True)
     assert r.returncode == 255 and "too few tokens for element" in r.stdout
+
+
+def read_binary_rays(path):
+    """The GUI's binary ray file (app/src/project.cpp:915-988): six doubles
+    of header, then nine doubles per record."""
+    import numpy as np
+    raw = np.fromfile(path, dtype="<f8")
+    head = raw[:6]
+    n = int(head[5])
+    assert raw.size == 6 + 9 * n
+    return head, raw[6:].reshape(n, 9)
+
+
+@needs_oracle
+def test_binary_ray_file_matches_text(tmp_path):
+    import numpy as np
+    shutil.copy(os.path.join(GOLDEN, "Refraction.stinput"), tmp_path / "r.stinput")
+    r = subprocess.run([STRACE_ORACLE, "--binary", str(tmp_path / "r.stinput"), *GOLDEN_ARGS], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout
+    head, rec = read_binary_rays(tmp_path / "r.stinput.bin")
+    text = (tmp_path / "r.stinput.rays").read_text().splitlines()
+    thead = np.array([float(v) for v in text[1].split()])
+    trow = np.array([[float(v) for v in ln.split()] for ln in text[3:]])
+    # the text carries 7 significant digits of the same doubles
+    np.testing.assert_allclose(head, thead, rtol=6e-7, atol=0)
+    assert rec.shape == trow.shape and head[5] == len(trow) > 0
+    np.testing.assert_array_equal(rec[:, 6:], trow[:, 6:])
+    np.testing.assert_allclose(rec[:, :6], trow[:, :6], rtol=6e-7, atol=1e-300)
