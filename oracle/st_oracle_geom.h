@@ -415,7 +415,38 @@ inline void torus_closed_form(const Element &e, const double P[3], const double 
     DF[0] = -Fx; DF[1] = -Fy; DF[2] = -Fz;
 }
 
-// coretrace/surface.cpp:57-413 (types 1/7 incl. cone, 3, 6, 8, 9)
+// Measured slopes at (x, y) by the modified Shepard weighting of the VSHOT
+// points within the region of interest (coretrace/vshot.cpp:290-367).
+inline void vshot_shepard(const Element &e, double x, double y, double *zx, double *zy) {
+    const size_t n = e.vshot.size() / 5;
+    const double density = (double)n / e.area;
+    const double roi2 = 30.0 / density / 3.14159;
+    double s = 0.0, wx = 0.0, wy = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        const double *v = &e.vshot[i * 5];
+        const double xx = v[0] - x, yy = v[1] - y;
+        if (xx == 0.0 && yy == 0.0) {
+            *zx = v[2];
+            *zy = v[3];
+            return;
+        }
+        const double r2 = xx * xx + yy * yy;
+        if (r2 < roi2) {
+            s = s + 1.0 / r2;
+            wx = wx + v[2] / r2;
+            wy = wy + v[3] / r2;
+        }
+    }
+    if (s == 0.0) {
+        *zx = 0.0;
+        *zy = 0.0;
+        return;
+    }
+    *zx = wx / s;
+    *zy = wy / s;
+}
+
+// coretrace/surface.cpp:57-413 (types 1/7 incl. cone, 3, 5, 6, 8, 9)
 inline void surface_eval(double X[3], const Element &e, double *F, double DF[3], int *err) {
     double x = X[0], y = X[1], z = X[2];
     double dfx = 0, dfy = 0, dfz = 0;
@@ -441,6 +472,24 @@ inline void surface_eval(double X[3], const Element &e, double *F, double DF[3],
         dfy = e.alpha[1];
         dfz = e.alpha[2];
         *F = dfx * x + dfy * y + dfz * z - e.alpha[3];
+    } else if (e.stype == 5) {
+        // VSHOT (coretrace/surface.cpp:244-298): z from the monomial fit, the
+        // normal from the interpolated measured slopes (angles, rad)
+        if (x * x + y * y == 0.0) {
+            *F = z;
+            DF[0] = 0.0;
+            DF[1] = 0.0;
+            DF[2] = 1.0;
+            return;
+        }
+        double zm = eval_mono(x, y, e);
+        double zx = 0.0, zy = 0.0;
+        vshot_shepard(e, x, y, &zx, &zy);
+        X[2] = zm;
+        *F = z - zm;
+        dfx = -tan(zx);
+        dfy = -tan(zy);
+        dfz = 1.0;
     } else if (e.stype == 6) {
         double zz = eval_mono(x, y, e);
         mono_slope(e, x, y, &dfx, &dfy);
@@ -506,6 +555,10 @@ inline void surface_z_at_xy(const double X[3], const Element &e, double *F) {
     }
     if (e.stype == 6) {
         *F = eval_mono(x, y, e);
+        return;
+    }
+    if (e.stype == 5) {  // coretrace/surfacezatxypair.cpp:195-207
+        *F = (x * x + y * y == 0.0) ? 0.0 : eval_mono(x, y, e);
         return;
     }
     if (e.stype == 8) {

@@ -60,6 +60,7 @@ struct Element {
     std::vector<double> bc;       // row-major (i, j)
     std::vector<double> poly;     // .ply coefficients
     std::vector<double> sx, sy, sy2;  // cubic spline data
+    std::vector<double> vshot;        // VSHOT points, 5 per row: x, y, x slope, y slope, (unused)
     int interaction = 0;
     OpticSet *optics = nullptr;
     std::string optic_name;

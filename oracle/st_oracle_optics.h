@@ -320,6 +320,17 @@ inline void aperture_plane(Element &e) {
     case 'i': case 'I':
         e.zap = e.sy[e.sy.size() - 1];
         break;
+    case 'v': case 'V': {
+        // the largest fitted z over the data points, at least 0
+        // (coretrace/apertureplane.cpp:474-488)
+        double lastz = 0.0;
+        for (size_t i = 0; i < e.vshot.size() / 5; i++) {
+            double zm = eval_mono(e.vshot[i * 5 + 0], e.vshot[i * 5 + 1], e);
+            if (zm > lastz) lastz = zm;
+        }
+        e.zap = lastz;
+        break;
+    }
     default:
         break;
     }
@@ -385,8 +396,8 @@ inline bool translate_surface_params(System &sys, Element &e, const double p[8])
     }
 }
 
-// Surface files .mon (Zernike), .csi (cubic spline) and .ply (polynomial),
-// coretrace/input.cpp:368-509.
+// Surface files .mon (Zernike), .sht (VSHOT), .csi (cubic spline) and .ply
+// (polynomial), coretrace/input.cpp:368-509.
 inline bool read_surface_file(const char *file, Element &e, System &sys) {
     const char *dot = strrchr(file, '.');
     if (!dot) {
@@ -420,6 +431,33 @@ inline bool read_surface_file(const char *file, Element &e, System &sys) {
         e.fit_order = order;
         e.surf = 'm';
         e.stype = 6;
+    } else if (ext == "sht") {
+        // coretrace/input.cpp:415-452: title, radius/focal length/target
+        // distance, (idum, order, points), RMS slope/scale, the lower
+        // triangle of B, then five columns per point
+        readln();
+        readln();
+        int idum = 0, order = 0, npts = 0;
+        readln();
+        sscanf(line, "%d %d %d", &idum, &order, &npts);
+        readln();
+        e.fit_order = order;
+        e.bc_n = order + 1;
+        e.bc.assign((size_t)e.bc_n * e.bc_n, 0.0);
+        for (int k = 0; k <= order; k++)
+            for (int m = 0; m <= k; m++) {
+                readln();
+                e.bc[(size_t)k * e.bc_n + m] = atof(line);
+            }
+        e.vshot.assign((size_t)npts * 5, 0.0);
+        for (int i = 0; i < npts; i++) {
+            double v[5] = {0, 0, 0, 0, 0};
+            readln();
+            sscanf(line, "%lg %lg %lg %lg %lg", &v[0], &v[1], &v[2], &v[3], &v[4]);
+            for (int c = 0; c < 5; c++) e.vshot[(size_t)i * 5 + c] = v[c];
+        }
+        e.surf = 'v';
+        e.stype = 5;
     } else if (ext == "ply") {
         readln();
         int order = atoi(line);

@@ -598,6 +598,35 @@ STD void polyrev(const DevElem &e, const double *aux, R x, R y, R *z, R *dzdx, R
     *dzdy = dr * (y / r);
 }
 
+// Measured slopes (angles, rad) at (x, y): modified Shepard weights 1/r^2 of
+// the VSHOT points within the region of interest, the point's own slopes on
+// an exact hit, zero with none in reach (coretrace/vshot.cpp:290-367).
+STD void vshot_slopes(const DevElem &e, const double *aux, double x, double y, double *zx, double *zy) {
+    const double *v = aux + e.aux2;
+    double s = 0.0, wx = 0.0, wy = 0.0;
+    for (int i = 0; i < e.aux2_n; i++, v += 4) {
+        const double xx = v[0] - x, yy = v[1] - y;
+        if (xx == 0.0 && yy == 0.0) {
+            *zx = v[2];
+            *zy = v[3];
+            return;
+        }
+        const double r2 = xx * xx + yy * yy;
+        if (r2 < e.roi2) {
+            s = s + 1.0 / r2;
+            wx = wx + v[2] / r2;
+            wy = wy + v[3] / r2;
+        }
+    }
+    if (s == 0.0) {
+        *zx = 0.0;
+        *zy = 0.0;
+        return;
+    }
+    *zx = wx / s;
+    *zy = wy / s;
+}
+
 // F and grad F for Newton (coretrace/surface.cpp:57-413).  X[2] is replaced
 // by the surface z for the tabulated surfaces, as in the reference.
 template <typename R>
@@ -646,6 +675,23 @@ STD int surface_eval(const DevElem &e, const double *aux, R X[3], R *F, R DF[3])
         DF[0] = -dx; DF[1] = -dy; DF[2] = 1;
         return 0;
     }
+    case SURF_VSHOT: {
+        // z from the monomial fit, the normal from the interpolated measured
+        // slopes (coretrace/surface.cpp:244-298)
+        if (x * x + y * y == R(0)) {
+            *F = z;
+            DF[0] = 0; DF[1] = 0; DF[2] = 1;
+            return 0;
+        }
+        R zz, dx, dy;
+        zernike<R>(e, aux, x, y, &zz, &dx, &dy, false);
+        double zx, zy;
+        vshot_slopes(e, aux, (double)x, (double)y, &zx, &zy);
+        X[2] = zz;
+        *F = z - zz;
+        DF[0] = (R)-::tan(zx); DF[1] = (R)-::tan(zy); DF[2] = 1;
+        return 0;
+    }
     default: {  // SURF_SPLINE
         R rho = M<R>::sqrt_(x * x + y * y);
         R drdx = x / rho, drdy = y / rho;
@@ -677,6 +723,12 @@ STD bool surface_z_at_xy(const DevElem &e, const double *aux, double x, double y
     case SURF_ZERNIKE: {
         double z, dx, dy;
         zernike<double>(e, aux, x, y, &z, &dx, &dy, false);
+        *F = z;
+        return true;
+    }
+    case SURF_VSHOT: {  // coretrace/surfacezatxypair.cpp:195-207
+        double z = 0.0, dx, dy;
+        if (x * x + y * y != 0.0) zernike<double>(e, aux, x, y, &z, &dx, &dy, false);
         *F = z;
         return true;
     }

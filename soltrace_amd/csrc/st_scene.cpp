@@ -174,6 +174,32 @@ bool read_surface_file(HElement &e, const char *file, std::string &msg) {
             }
         e.surf = 'm';
         e.stype = 6;
+    } else if (ext == "sht") {
+        // coretrace/input.cpp:415-452: title; radius, focal length, target
+        // distance; (unused, order, points); RMS slope and scale; the lower
+        // triangle of B; five columns per point (the fifth unused)
+        next();
+        next();
+        int idum = 0, order = 0, npts = 0;
+        next();
+        sscanf(line, "%d %d %d", &idum, &order, &npts);
+        next();
+        e.fit_order = order;
+        e.zern.assign((size_t)(order + 1) * (order + 1), 0.0);
+        for (int k = 0; k <= order; k++)
+            for (int m = 0; m <= k; m++) {
+                next();
+                e.zern[(size_t)k * (order + 1) + m] = atof(line);
+            }
+        e.vshot.assign((size_t)std::max(npts, 0) * 4, 0.0);
+        for (int i = 0; i < npts; i++) {
+            double v[5] = {0, 0, 0, 0, 0};
+            next();
+            sscanf(line, "%lg %lg %lg %lg %lg", &v[0], &v[1], &v[2], &v[3], &v[4]);
+            for (int c = 0; c < 4; c++) e.vshot[(size_t)i * 4 + c] = v[c];
+        }
+        e.surf = 'v';
+        e.stype = 5;
     } else if (ext == "ply") {
         next();
         int order = atoi(line);
@@ -321,8 +347,52 @@ static void aperture_plane(HElement &e) {
     case 'i':
         e.zap = e.sy.empty() ? 0.0 : e.sy.back();
         break;
+    case 'v': {
+        // the largest fitted z over the data points, at least 0
+        // (coretrace/apertureplane.cpp:474-488)
+        const int n = e.fit_order + 1;
+        double lastz = 0.0;
+        for (size_t i = 0; i < e.vshot.size() / 4; i++) {
+            const double x = e.vshot[i * 4], y = e.vshot[i * 4 + 1];
+            double zm = 0.0;
+            for (int k = 0; k <= e.fit_order; k++)
+                for (int m = 0; m <= k; m++) zm += e.zern[(size_t)k * n + m] * pow(x, m) * pow(y, k - m);
+            if (zm > lastz) lastz = zm;
+        }
+        e.zap = lastz;
+        break;
+    }
     default:
         break;
+    }
+}
+
+// Aperture area (coretrace/input.cpp:204-254).
+static double aperture_area(const HElement &e) {
+    auto sq = [](double v) { return v * v; };
+    const double *P = e.p;
+    switch (tolower(e.shape)) {
+    case 'c': return M_PI * sq(P[0] / 2);
+    case 'h': return 5.0 * sq(P[0] / 2.0) * cos(30.0 * kDegRad) * sin(30.0 * kDegRad);
+    case 'r': return P[0] * P[1];
+    case 't': return 0.25 * sq(P[0] * cos(30.0 * kDegRad)) * sqrt(3.0);
+    case 'a': return P[2] * kDegRad * (P[1] - P[0]);
+    case 'l': return (P[1] - P[0]) * P[2];
+    case 'i': {
+        const double v11 = P[0] - P[2], v12 = P[1] - P[3], v21 = P[4] - P[2], v22 = P[5] - P[3];
+        const double v1m = sqrt(v11 * v11 + v12 * v12), v2m = sqrt(v21 * v21 + v22 * v22);
+        return 0.5 * v1m * v2m * sin(acos((v11 * v21 + v12 * v22) / (v1m * v2m)));
+    }
+    case 'q': {
+        const double v11 = P[0] - P[2], v12 = P[1] - P[3], v21 = P[4] - P[2], v22 = P[5] - P[3];
+        const double v31 = P[4] - P[6], v32 = P[5] - P[7], v41 = P[0] - P[6], v42 = P[1] - P[7];
+        const double v1m = sqrt(v11 * v11 + v12 * v12), v2m = sqrt(v21 * v21 + v22 * v22);
+        const double v3m = sqrt(v31 * v31 + v32 * v32), v4m = sqrt(v41 * v41 + v42 * v42);
+        const double t1 = acos((v11 * v21 + v12 * v22) / (v1m * v2m));
+        const double t2 = acos((v31 * v41 + v32 * v42) / (v3m * v4m));
+        return 0.5 * (v1m * v2m * sin(t1) + v3m * v4m * sin(t2));
+    }
+    default: return 0.0;
     }
 }
 
@@ -349,6 +419,7 @@ static int surf_kind(const HElement &e) {
     case 'd': return SURF_TORUS;
     case 'f': return SURF_FLAT;
     case 'm': return SURF_ZERNIKE;
+    case 'v': return SURF_VSHOT;
     case 'i': return SURF_SPLINE;
     case 'r': return SURF_POLY;
     default: return SURF_GENERAL;  // 'g', 'c'
@@ -376,7 +447,7 @@ static double bound_z(const HElement &e, double x, double y) {
     }
     case 'c':
         return sqrt(x * x + y * y) / tan(e.cone_half * kDegRad);
-    case 'm': {
+    case 'm': case 'v': {
         int n = e.fit_order + 1;
         double z = 0.0;
         for (int i = 0; i <= e.fit_order; i++)
@@ -768,10 +839,20 @@ bool compile_scene(SceneModel &m, bool use_cull, CompiledScene &out, std::string
                         d.face_back = optic_face0[k] + 1;
                     }
             }
-            if (d.surf == SURF_ZERNIKE) {
+            if (d.surf == SURF_ZERNIKE || d.surf == SURF_VSHOT) {
                 d.aux = (int)out.aux.size();
                 d.aux_n = e.fit_order + 1;
                 out.aux.insert(out.aux.end(), e.zern.begin(), e.zern.end());
+                if (d.surf == SURF_VSHOT) {
+                    // density = points / aperture area, and the region of
+                    // interest of the Shepard weights (coretrace/surface.cpp:
+                    // 258, vshot.cpp:317)
+                    d.aux2 = (int)out.aux.size();
+                    d.aux2_n = (int)(e.vshot.size() / 4);
+                    out.aux.insert(out.aux.end(), e.vshot.begin(), e.vshot.end());
+                    const double density = (double)d.aux2_n / aperture_area(e);
+                    d.roi2 = 30.0 / density / 3.14159;
+                }
             } else if (d.surf == SURF_SPLINE) {
                 d.aux = (int)out.aux.size();
                 d.aux_n = (int)e.sx.size();

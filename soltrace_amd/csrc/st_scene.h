@@ -44,6 +44,7 @@ struct HElement {
     std::vector<double> zern;            // (order+1)^2 row-major B(i,j)
     std::vector<double> poly;            // .ply coefficients
     std::vector<double> sx, sy, sy2;     // cubic spline
+    std::vector<double> vshot;           // VSHOT points: x, y, x slope, y slope
     int interaction = 0;
     std::string optic_name;
     // filled by compile()

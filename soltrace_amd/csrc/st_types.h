@@ -21,6 +21,7 @@ enum SurfKind : int32_t {
     SURF_SPLINE = 8,      // 'i'  Newton      (type 9)
     SURF_GENERAL = 9,     // 'g'/'c'/'o'(type 7) Newton on the conic (type 1/7)
     SURF_POLY = 10,       // 'r'  Newton      (type 8)
+    SURF_VSHOT = 11,      // 'v'  Newton      (type 5): monomial z, measured slopes
 };
 
 enum ApKind : int32_t {
@@ -54,7 +55,9 @@ struct DevElem {
     int32_t aux, aux_n;     // Zernike B (n x n) or spline x,y,y2 (3 x n)
     int32_t fit_order, enabled;
     int32_t zap_filter;     // 0 for 'm' and 'r' (raytrace.cpp:747-751)
+    int32_t aux2, aux2_n;   // VSHOT points (x, y, x slope, y slope) and their count
     int32_t pad1;
+    double roi2;            // VSHOT region of interest, squared radius (m^2)
 };
 
 struct DevStage {

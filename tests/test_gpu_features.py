@@ -13,7 +13,7 @@ import pytest
 
 from soltrace_amd import workloads
 from harness import (assert_records_match, counter_mode, hip_sun_index, oracle_sun_index, run_scene)
-from test_oracle_vs_ref import CASES, _mini, _write, torus_target_scene
+from test_oracle_vs_ref import CASES, _mini, _write, torus_target_scene, vshot_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -62,6 +62,35 @@ def test_cull_equals_full_scan(hip_lib):
     for f in ("element", "stage", "raynum"):
         np.testing.assert_array_equal(getattr(a[1], f), getattr(b[1], f))
     np.testing.assert_array_equal(a[1].pos, b[1].pos)
+
+
+def test_vshot_surface_fp64(hip_lib, oracle_lib):
+    # the reference's VSHOT sample: monomial z, Shepard-interpolated measured
+    # slopes as the normal (coretrace/surface.cpp:244-298, vshot.cpp:290-367)
+    sc = vshot_scene(n=3000)
+    n_o, r_o, _, _ = run_scene(oracle_lib, sc, 17, setup=counter_mode)
+    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 17, setup=_setup(64))
+    assert n_g == n_o and n_o > 0, (n_g, n_o, m_g)
+    assert_records_match(r_g, r_o, pos_tol=1e-6, cos_tol=1e-6)
+
+
+def test_vshot_surface_fp32(hip_lib, oracle_lib):
+    sc = vshot_scene(n=3000)
+    n_o, r_o, _, si_o = run_scene(oracle_lib, sc, 17, setup=counter_mode, after=oracle_sun_index)
+    n_g, r_g, m_g, si_g = run_scene(hip_lib, sc, 17, setup=_setup(32),
+                                    after=lambda lib, ctx: hip_sun_index(lib, ctx, 3000))
+    assert n_g > 0, m_g
+    so, sg = _by_ray(r_o, si_o), _by_ray(r_g, si_g)
+    common = set(so) & set(sg)
+    assert len(common) >= 0.995 * max(len(so), len(sg))
+    same, worst = 0, 0.0
+    for k in common:
+        io, ig = so[k], sg[k]
+        if len(io) == len(ig) and np.array_equal(r_o.element[io], r_g.element[ig]):
+            same += 1
+            worst = max(worst, float(np.abs(r_o.pos[io] - r_g.pos[ig]).max()))
+    assert same >= 0.995 * len(common), (same, len(common))
+    assert worst <= 1e-4, worst
 
 
 def _curved_field():

@@ -159,6 +159,21 @@ def test_zernike_and_spline_files_bitexact(ref_lib, oracle_lib):
         assert _compare(ref_lib, oracle_lib, sc, 5) > 0
 
 
+VSHOT_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "vshot_test.sht")
+
+
+def vshot_scene(n=1500):
+    # the reference's VSHOT sample (app/deploy/samples/VSHOT Test.sht, 4,373
+    # measured points over x in [-1.62, -0.03]) on a 3.3 x 1.7 m facet: the
+    # left half interpolates measured slopes, the right half has none in
+    # reach (zero slopes)
+    return _mini(surface="v", sp=(), ap="r", app=(3.3, 1.7), surf_file=VSHOT_FILE, n=n)
+
+
+def test_vshot_file_bitexact(ref_lib, oracle_lib):
+    assert _compare(ref_lib, oracle_lib, vshot_scene(), 6) > 0
+
+
 def test_error_paths_match(ref_lib, oracle_lib):
     sc = workloads.trough(raycount=1000)
     sc.maxcount = 50  # sun-ray cap reached (coretrace/raytrace.cpp:600-604)
