@@ -643,9 +643,15 @@ int st_sim_run(st_context_t p, unsigned int seed, progress_cb cb, void *data) {
             if (sys->torus_start && e->stype == 10) torus_start_sphere(*e);
         }
     bool cancelled = false;
-    if (!trace(*sys, seed, (uint64_t)(int64_t)sys->raycount, (uint64_t)(int64_t)sys->raymax, sys->err_sunshape,
-               sys->err_optical, cb, data, &cancelled))
+    try {
+        if (!trace(*sys, seed, (uint64_t)(int64_t)sys->raycount, (uint64_t)(int64_t)sys->raymax, sys->err_sunshape,
+                   sys->err_optical, cb, data, &cancelled))
+            return -1;
+    } catch (const FeFailure &) {
+        // the reference throws out of Trace here (coretrace/vshot.cpp:177)
+        sys->errlog("Interpolation failed to find nearby nodes to ray hit in FEA surface.");
         return -1;
+    }
     merge_records(*sys);
     return (int)sys->all.size();
 }

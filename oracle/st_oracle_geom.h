@@ -446,7 +446,120 @@ inline void vshot_shepard(const Element &e, double x, double y, double *zx, doub
     *zy = wy / s;
 }
 
-// coretrace/surface.cpp:57-413 (types 1/7 incl. cone, 3, 5, 6, 8, 9)
+// Local kriging model of a finite-element surface at (x, y)
+// (coretrace/vshot.cpp:166-238, FEInterpKD, with the GaussMarkov model and
+// LU solve of interpolate.cpp): the nodes of the hash-tree zone holding the
+// point and of its neighbours, sorted by distance; the nearest seven fit a
+// power variogram (beta 1.999, no nugget); z there and one-sided
+// differences `step` away give the slopes.
+struct FeFailure {};
+inline void fe_interp(const Element &e, double x, double y, double step, double *zr, double *dzdx, double *dzdy) {
+    std::vector<int> near;
+    if (!e.fe_tree.data_at(x, y, near) || near.size() < 3) throw FeFailure();
+    struct RE { double r; int i; };
+    std::vector<RE> nd;
+    for (int i : near) {
+        const double *v = &e.fe_nodes[3 * (size_t)i];
+        nd.push_back(RE{std::sqrt(std::pow(x - v[0], 2) + std::pow(y - v[1], 2)), i});
+    }
+    std::sort(nd.begin(), nd.end(), [](const RE &a, const RE &b) { return a.r < b.r; });
+    const int m = (int)nd.size() > 7 ? 7 : (int)nd.size();
+    std::vector<double> px(m), py(m), pz(m + 1);
+    for (int k = 0; k < m; k++) {
+        const double *v = &e.fe_nodes[3 * (size_t)nd[k].i];
+        px[k] = v[0]; py[k] = v[1]; pz[k] = v[2];
+    }
+    const double bet = 1.999;
+    double num = 0., den = 0.;
+    for (int i = 0; i < m; i++)
+        for (int j = i + 1; j < m; j++) {
+            double rb = 0.;
+            rb += (px[i] - px[j]) * (px[i] - px[j]);
+            rb += (py[i] - py[j]) * (py[i] - py[j]);
+            rb = std::pow(rb, 0.5 * bet);
+            num += rb * (0.5 * ((pz[i] - pz[j]) * (pz[i] - pz[j])) - 0.0);
+            den += rb * rb;
+        }
+    const double alph = num / den;
+    auto vg = [&](double r) { return 0.0 + alph * std::pow(r, bet); };
+    auto rd = [&](double ax, double ay, double bx, double by) {
+        double d = 0.;
+        d += (ax - bx) * (ax - bx);
+        d += (ay - by) * (ay - by);
+        return std::sqrt(d);
+    };
+    const int n = m + 1;
+    std::vector<std::vector<double>> lu(n, std::vector<double>(n, 0.0));
+    for (int i = 0; i < m; i++) {
+        for (int j = i; j < m; j++) lu[i][j] = lu[j][i] = vg(rd(px[i], py[i], px[j], py[j]));
+        lu[i][m] = lu[m][i] = 1.;
+    }
+    lu[m][m] = pz[m] = 0.;
+    // LU decomposition with implicit partial pivoting (Numerical Recipes)
+    std::vector<int> indx(n);
+    std::vector<double> vv(n);
+    for (int i = 0; i < n; i++) {
+        double big = 0.0, temp;
+        for (int j = 0; j < n; j++)
+            if ((temp = std::fabs(lu[i][j])) > big) big = temp;
+        if (big == 0.0) throw FeFailure();
+        vv[i] = 1.0 / big;
+    }
+    for (int k = 0; k < n; k++) {
+        double big = 0.0, temp;
+        int imax = k;
+        for (int i = k; i < n; i++) {
+            temp = vv[i] * std::fabs(lu[i][k]);
+            if (temp > big) { big = temp; imax = i; }
+        }
+        if (k != imax) {
+            for (int j = 0; j < n; j++) std::swap(lu[imax][j], lu[k][j]);
+            vv[imax] = vv[k];
+        }
+        indx[k] = imax;
+        if (lu[k][k] == 0.0) lu[k][k] = 1.0e-40;
+        for (int i = k + 1; i < n; i++) {
+            temp = lu[i][k] /= lu[k][k];
+            for (int j = k + 1; j < n; j++) lu[i][j] -= temp * lu[k][j];
+        }
+    }
+    std::vector<double> yv(pz);
+    {
+        int ii = 0;
+        for (int i = 0; i < n; i++) {
+            const int ip = indx[i];
+            double sum = yv[ip];
+            yv[ip] = yv[i];
+            if (ii != 0)
+                for (int j = ii - 1; j < i; j++) sum -= lu[i][j] * yv[j];
+            else if (sum != 0.0)
+                ii = i + 1;
+            yv[i] = sum;
+        }
+        for (int i = n - 1; i >= 0; i--) {
+            double sum = yv[i];
+            for (int j = i + 1; j < n; j++) sum -= lu[i][j] * yv[j];
+            yv[i] = sum / lu[i][i];
+        }
+    }
+    auto interp = [&](double qx, double qy) {
+        double v = 0.;
+        for (int i = 0; i < m; i++) v += yv[i] * vg(rd(qx, qy, px[i], py[i]));
+        v += yv[m] * 1.;
+        return v;
+    };
+    double xs = x, ys = y;
+    *zr = interp(xs, ys);
+    xs += step;
+    const double za = interp(xs, ys);
+    xs -= step;
+    ys += step;
+    const double zb = interp(xs, ys);
+    *dzdx = (*zr - za) / step;
+    *dzdy = (*zr - zb) / step;
+}
+
+// coretrace/surface.cpp:57-413 (types 1/7 incl. cone, 3, 4, 5, 6, 8, 9)
 inline void surface_eval(double X[3], const Element &e, double *F, double DF[3], int *err) {
     double x = X[0], y = X[1], z = X[2];
     double dfx = 0, dfy = 0, dfz = 0;
@@ -472,6 +585,24 @@ inline void surface_eval(double X[3], const Element &e, double *F, double DF[3],
         dfy = e.alpha[1];
         dfz = e.alpha[2];
         *F = dfx * x + dfy * y + dfz * z - e.alpha[3];
+    } else if (e.stype == 4) {
+        // finite element (coretrace/surface.cpp:218-242)
+        if (x * x + y * y == 0.0) {
+            *F = z;
+            DF[0] = 0.0;
+            DF[1] = 0.0;
+            DF[2] = 1.0;
+            return;
+        }
+        const double density = (double)(e.fe_nodes.size() / 3) / e.area;
+        const double delta = 0.001 / sqrt(density);
+        double zr = 0.0, dzx = 0.0, dzy = 0.0;
+        fe_interp(e, x, y, delta, &zr, &dzx, &dzy);
+        X[2] = zr;
+        *F = z - zr;
+        dfx = dzx;
+        dfy = dzy;
+        dfz = 1.0;
     } else if (e.stype == 5) {
         // VSHOT (coretrace/surface.cpp:244-298): z from the monomial fit, the
         // normal from the interpolated measured slopes (angles, rad)

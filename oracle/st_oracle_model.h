@@ -15,6 +15,10 @@
 #include <string>
 #include <vector>
 
+// the finite-element hash tree restated from coretrace/treemesh.cpp, shared
+// with the scene compiler (pinned against the reference build)
+#include "../soltrace_amd/csrc/st_fetree.h"
+
 namespace oracle {
 
 static const double kDegToRad = 0.017453292519943295;  // types.h:64
@@ -61,6 +65,8 @@ struct Element {
     std::vector<double> poly;     // .ply coefficients
     std::vector<double> sx, sy, sy2;  // cubic spline data
     std::vector<double> vshot;        // VSHOT points, 5 per row: x, y, x slope, y slope, (unused)
+    std::vector<double> fe_nodes;     // finite-element nodes, x y z per node
+    st_fe::Tree fe_tree;              // their spatial hash tree
     int interaction = 0;
     OpticSet *optics = nullptr;
     std::string optic_name;

@@ -320,6 +320,17 @@ inline void aperture_plane(Element &e) {
     case 'i': case 'I':
         e.zap = e.sy[e.sy.size() - 1];
         break;
+    case 'e': case 'E': {
+        // the largest node *y* (column 1), at least 0, as the reference has it
+        // (coretrace/apertureplane.cpp:458-471)
+        double lastz = 0.0;
+        for (size_t i = 0; i < e.fe_nodes.size() / 3; i++) {
+            double newz = e.fe_nodes[i * 3 + 1];
+            if (newz > lastz) lastz = newz;
+        }
+        e.zap = lastz;
+        break;
+    }
     case 'v': case 'V': {
         // the largest fitted z over the data points, at least 0
         // (coretrace/apertureplane.cpp:474-488)
@@ -396,8 +407,8 @@ inline bool translate_surface_params(System &sys, Element &e, const double p[8])
     }
 }
 
-// Surface files .mon (Zernike), .sht (VSHOT), .csi (cubic spline) and .ply
-// (polynomial), coretrace/input.cpp:368-509.
+// Surface files .mon (Zernike), .sht (VSHOT), .csi (cubic spline), .ply
+// (polynomial) and .fed (finite element), coretrace/input.cpp:368-561.
 inline bool read_surface_file(const char *file, Element &e, System &sys) {
     const char *dot = strrchr(file, '.');
     if (!dot) {
@@ -458,6 +469,24 @@ inline bool read_surface_file(const char *file, Element &e, System &sys) {
         }
         e.surf = 'v';
         e.stype = 5;
+    } else if (ext == "fed") {
+        // coretrace/input.cpp:510-561: title, node count, x y z per node,
+        // then the spatial hash tree over the nodes
+        readln();
+        readln();
+        int n = atoi(line);
+        e.fe_nodes.assign((size_t)std::max(n, 0) * 3, 0.0);
+        for (int i = 0; i < n; i++) {
+            double x = 0, y = 0, z = 0;
+            readln();
+            sscanf(line, "%lg %lg %lg", &x, &y, &z);
+            e.fe_nodes[3 * (size_t)i] = x;
+            e.fe_nodes[3 * (size_t)i + 1] = y;
+            e.fe_nodes[3 * (size_t)i + 2] = z;
+        }
+        st_fe::build(e.fe_tree, e.fe_nodes);
+        e.surf = 'e';
+        e.stype = 4;
     } else if (ext == "ply") {
         readln();
         int order = atoi(line);

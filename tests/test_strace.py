@@ -130,3 +130,27 @@ def test_binary_ray_file_matches_text(tmp_path):
     assert rec.shape == trow.shape and head[5] == len(trow) > 0
     np.testing.assert_array_equal(rec[:, 6:], trow[:, 6:])
     np.testing.assert_allclose(rec[:, :6], trow[:, :6], rtol=6e-7, atol=1e-300)
+
+
+REF_SAMPLES = "/root/reference/app/deploy/samples"
+
+
+@needs_oracle
+@pytest.mark.skipif(not (os.path.exists(STRACE_REF) and os.path.isdir(REF_SAMPLES)),
+                    reason="reference build and samples absent")
+@pytest.mark.parametrize("args", [["600", "200000", "4", "1", "1", "0"], ["300", "200000", "9", "0", "0", "0"]])
+def test_finite_element_sample_equals_reference_build(tmp_path, args):
+    # the reference's own finite-element sample: two facets meshed by 10,000
+    # nodes each (a structured and an unstructured mesh), kriging over the
+    # hash-tree neighbourhood (coretrace/vshot.cpp:166-238, treemesh.cpp)
+    name = "Surface Sample-Finite Element"
+    outs = []
+    for exe, sub in ((STRACE_ORACLE, "ours"), (STRACE_REF, "ref")):
+        d = tmp_path / sub
+        d.mkdir()
+        for f in (name + ".stinput", "sample.fed", "sample_un.fed"):
+            shutil.copy(os.path.join(REF_SAMPLES, f), d / f)
+        r = subprocess.run([exe, name + ".stinput", *args], cwd=d, capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stdout[-2000:]
+        outs.append((d / (name + ".stinput.rays")).read_text())
+    assert outs[0] == outs[1] and len(outs[0].splitlines()) > 300
