@@ -648,8 +648,9 @@ int st_sim_run(st_context_t p, unsigned int seed, progress_cb cb, void *data) {
                    sys->err_optical, cb, data, &cancelled))
             return -1;
     } catch (const FeFailure &) {
-        // the reference throws out of Trace here (coretrace/vshot.cpp:177)
-        sys->errlog("Interpolation failed to find nearby nodes to ray hit in FEA surface.");
+        // the reference throws out of Trace here (coretrace/vshot.cpp:177,
+        // raytrace.cpp:1108-1112)
+        sys->errlog("trace error: Interpolation failed to find nearby nodes to ray hit in FEA surface.");
         return -1;
     }
     merge_records(*sys);

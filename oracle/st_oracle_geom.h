@@ -505,9 +505,12 @@ inline void fe_interp(const Element &e, double x, double y, double step, double 
         if (big == 0.0) throw FeFailure();
         vv[i] = 1.0 / big;
     }
+    // imax keeps its value from the previous column when no pivot candidate
+    // compares greater than zero (a zero or NaN column, as duplicate nodes
+    // give), as in the reference, which declares it once for the whole loop
+    int imax = 0;
     for (int k = 0; k < n; k++) {
         double big = 0.0, temp;
-        int imax = k;
         for (int i = k; i < n; i++) {
             temp = vv[i] * std::fabs(lu[i][k]);
             if (temp > big) { big = temp; imax = i; }

@@ -271,10 +271,14 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     }
     const SceneView sv = scene_view;
 
-    // total[0..1]: scan totals; total[3]: error flags, read with the totals
-    uint32_t *total = (uint32_t *)d_total.ensure(4 * sizeof(uint32_t));
+    // total[0..1]: scan totals; total[3]: error flags, read with the totals;
+    // total[4..5]: the least sun-ray index whose finite-element
+    // interpolation failed in a hit-mask launch (all ones: none)
+    uint32_t *total = (uint32_t *)d_total.ensure(6 * sizeof(uint32_t));
     uint32_t *err = total + 3;
+    unsigned long long *fe_fail = (unsigned long long *)(total + 4);
     launch_fill_u32(err, 1, 0u, s);
+    launch_fill_u32(total + 4, 2, 0xFFFFFFFFu, s);
     uint32_t *ovf_count = (uint32_t *)d_ovf_count.ensure(sizeof(uint32_t));
     const uint32_t ovf_cap = kMaxChunk / 2;
     d_ovf_rec.ensure((size_t)ovf_cap * sizeof(DevRecord));
@@ -341,6 +345,12 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         return b;
     };
     auto check_err = [&](uint32_t e) -> bool {
+        if (e & 16u) {
+            // the reference throws out of Trace (coretrace/vshot.cpp:177-178,
+            // raytrace.cpp:1108-1112)
+            errlog("trace error: Interpolation failed to find nearby nodes to ray hit in FEA surface.");
+            return false;
+        }
         if (e & 1u) {
             errlog("Bad optical interaction type (stage)");
             return false;
@@ -509,6 +519,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         ma.s0 = s0;
         ma.n = (uint32_t)batch;
         ma.mask = mask;
+        ma.fe_fail = fe_fail;
         ma.counters = ctrs;
         ma.generic = cs.stages[0].generic;
         // pairs pack the element index in 24 bits
@@ -614,6 +625,18 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         cur_b = next_b;
     }
     HC(hipStreamSynchronize(sm));
+    if (cs.stages[0].generic == 2) {
+        // hit-mask launches run ahead of the trace: a failed interpolation
+        // counts when the reference reaches its sun ray, i.e. before the
+        // last sun ray used
+        unsigned long long f = 0;
+        HC(hipMemcpyAsync(&f, fe_fail, sizeof f, hipMemcpyDeviceToHost, sm));
+        HC(hipStreamSynchronize(sm));
+        if (f < (unsigned long long)sun_ray_count) {
+            check_err(16u);
+            return -1;
+        }
+    }
     if (maxed && !cancelled) {
         errlog("generated sun rays reached maximum count: %d", raymax);
         return -1;

@@ -627,9 +627,176 @@ STD void vshot_slopes(const DevElem &e, const double *aux, double x, double y, d
     *zy = wy / s;
 }
 
+// Finite-element surface at (x, y) (coretrace/vshot.cpp:166-238): the data
+// nodes of the hash-tree zone holding the point and of its neighbours
+// (coretrace/treemesh.cpp:209-245, 511-537, 757-800), the seven nearest of
+// them, a kriging model over those with the power variogram
+// alpha r^1.999, alpha fitted by least squares over the node pairs
+// (coretrace/vshot.cpp:43-163), solved by LU with implicit partial pivoting,
+// then the model's value at the point and its forward differences one step
+// away in x and in y (the y step taken from the point itself).  Returns
+// false where the tree has no zone, or fewer than three nodes, for the
+// point (the reference throws there) or the kriging matrix has a zero row.
+//
+// aux + e.aux holds the flattened tree: {x0, x1, y0, y1, address bits,
+// step, zones, nodes}, five doubles per zone {child 0, child 1, terminal,
+// offset of its candidate list, candidates}, x y z per node, then the
+// candidate lists (node indices: the zone's own nodes, then its
+// neighbours').  Kept out of line: its arrays would otherwise weigh on the
+// registers of every kernel that can meet such a surface.
+__device__ __attribute__((noinline)) bool fe_interp(const DevElem &e, const double *aux, double x, double y,
+                                                    double *zr, double *dzdx, double *dzdy) {
+    const double *hd = aux + e.aux;
+    const int nc = (int)hd[4], nz = (int)hd[6];
+    const double step = hd[5];
+    const double *zones = hd + 8, *nodes = zones + 5 * nz;
+    // the zone: halve x and y alternately, down an address of nc digits,
+    // stopping at a terminal zone; an 'x' split has both children equal
+    double x0 = hd[0], x1 = hd[1], y0 = hd[2], y1 = hd[3];
+    int zn = 0;
+    for (int i = 0; i < nc; i++) {
+        const double *z = zones + 5 * zn;
+        if (z[2] != 0.0) break;
+        bool up;
+        if ((i & 1) == 0) {
+            const double c = (x0 + x1) * 0.5;
+            up = x > c;
+            if (up) x0 = c; else x1 = c;
+        } else {
+            const double c = (y0 + y1) * 0.5;
+            up = y > c;
+            if (up) y0 = c; else y1 = c;
+        }
+        const int next = (int)(up ? z[1] : z[0]);
+        if (next < 0) return false;
+        zn = next;
+    }
+    const double *cand = aux + (int64_t)zones[5 * zn + 3];
+    const int nn = (int)zones[5 * zn + 4];
+    if (nn < 3) return false;
+    // the seven nearest candidates, by distance, ties in list order
+    double rk[7];
+    int ik[7];
+    int m = 0;
+    for (int c = 0; c < nn; c++) {
+        const int i = (int)cand[c];
+        const double dx = x - nodes[3 * i], dy = y - nodes[3 * i + 1];
+        const double r = ::sqrt(dx * dx + dy * dy);
+        if (m == 7 && !(r < rk[6])) continue;
+        int k = m < 7 ? m++ : 6;
+        for (; k > 0 && rk[k - 1] > r; k--) {
+            rk[k] = rk[k - 1];
+            ik[k] = ik[k - 1];
+        }
+        rk[k] = r;
+        ik[k] = i;
+    }
+    double px[7], py[7], pz[8];
+    for (int k = 0; k < m; k++) {
+        px[k] = nodes[3 * ik[k]];
+        py[k] = nodes[3 * ik[k] + 1];
+        pz[k] = nodes[3 * ik[k] + 2];
+    }
+    const double bet = 1.999;
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < m; i++)
+        for (int j = i + 1; j < m; j++) {
+            double rb = 0.0;
+            rb += (px[i] - px[j]) * (px[i] - px[j]);
+            rb += (py[i] - py[j]) * (py[i] - py[j]);
+            rb = ::pow(rb, 0.5 * bet);
+            num += rb * (0.5 * ((pz[i] - pz[j]) * (pz[i] - pz[j])) - 0.0);
+            den += rb * rb;
+        }
+    const double alph = num / den;
+    const int n = m + 1;
+    double lu[8][8];
+    for (int i = 0; i < m; i++) {
+        for (int j = i; j < m; j++) {
+            double d = 0.0;
+            d += (px[i] - px[j]) * (px[i] - px[j]);
+            d += (py[i] - py[j]) * (py[i] - py[j]);
+            lu[i][j] = lu[j][i] = 0.0 + alph * ::pow(::sqrt(d), bet);
+        }
+        lu[i][m] = lu[m][i] = 1.0;
+    }
+    lu[m][m] = pz[m] = 0.0;
+    int indx[8];
+    double vv[8];
+    for (int i = 0; i < n; i++) {
+        double big = 0.0;
+        for (int j = 0; j < n; j++) big = ::fabs(lu[i][j]) > big ? ::fabs(lu[i][j]) : big;
+        if (big == 0.0) return false;
+        vv[i] = 1.0 / big;
+    }
+    // the pivot row carries over from the previous column when none in this
+    // one compares greater than zero (a zero or NaN column, as duplicate
+    // nodes give), as in the reference (coretrace/interpolate.cpp:5-50)
+    int imax = 0;
+    for (int k = 0; k < n; k++) {
+        double big = 0.0;
+        for (int i = k; i < n; i++) {
+            const double t = vv[i] * ::fabs(lu[i][k]);
+            if (t > big) { big = t; imax = i; }
+        }
+        if (k != imax) {
+            for (int j = 0; j < n; j++) {
+                const double t = lu[imax][j];
+                lu[imax][j] = lu[k][j];
+                lu[k][j] = t;
+            }
+            vv[imax] = vv[k];
+        }
+        indx[k] = imax;
+        if (lu[k][k] == 0.0) lu[k][k] = 1.0e-40;
+        for (int i = k + 1; i < n; i++) {
+            const double t = lu[i][k] /= lu[k][k];
+            for (int j = k + 1; j < n; j++) lu[i][j] -= t * lu[k][j];
+        }
+    }
+    double w[8];
+    for (int i = 0; i < n; i++) w[i] = pz[i];
+    int ii = 0;
+    for (int i = 0; i < n; i++) {
+        const int ip = indx[i];
+        double sum = w[ip];
+        w[ip] = w[i];
+        if (ii != 0)
+            for (int j = ii - 1; j < i; j++) sum -= lu[i][j] * w[j];
+        else if (sum != 0.0)
+            ii = i + 1;
+        w[i] = sum;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double sum = w[i];
+        for (int j = i + 1; j < n; j++) sum -= lu[i][j] * w[j];
+        w[i] = sum / lu[i][i];
+    }
+    const double qx[3] = {x, x + step, x + step - step}, qy[3] = {y, y, y + step};
+    double v[3];
+    for (int q = 0; q < 3; q++) {
+        double s = 0.0;
+        for (int i = 0; i < m; i++) {
+            double d = 0.0;
+            d += (qx[q] - px[i]) * (qx[q] - px[i]);
+            d += (qy[q] - py[i]) * (qy[q] - py[i]);
+            s += w[i] * (0.0 + alph * ::pow(::sqrt(d), bet));
+        }
+        s += w[m] * 1.0;
+        v[q] = s;
+    }
+    *zr = v[0];
+    *dzdx = (v[0] - v[1]) / step;
+    *dzdy = (v[0] - v[2]) / step;
+    return true;
+}
+
 // F and grad F for Newton (coretrace/surface.cpp:57-413).  X[2] is replaced
-// by the surface z for the tabulated surfaces, as in the reference.
-template <typename R>
+// by the surface z for the tabulated surfaces, as in the reference.  G = 2 is the
+// kernel variant for stages holding a finite-element surface: only there is
+// its interpolation compiled in (its scratch arrays would otherwise cost the
+// other generic kernels registers).
+template <typename R, int G>
 STD int surface_eval(const DevElem &e, const double *aux, R X[3], R *F, R DF[3]) {
     R x = X[0], y = X[1], z = X[2];
     switch (e.surf) {
@@ -691,6 +858,25 @@ STD int surface_eval(const DevElem &e, const double *aux, R X[3], R *F, R DF[3])
         *F = z - zz;
         DF[0] = (R)-::tan(zx); DF[1] = (R)-::tan(zy); DF[2] = 1;
         return 0;
+    }
+    case SURF_FE: {
+        // kriging over the finite-element nodes (coretrace/surface.cpp:218-242);
+        // 4 where the interpolation fails
+        if (x * x + y * y == R(0)) {
+            *F = z;
+            DF[0] = 0; DF[1] = 0; DF[2] = 1;
+            return 0;
+        }
+        if constexpr (G != 2) {
+            return 4;  // not in this variant (the host never sends it here)
+        } else {
+            double zz, dx, dy;
+            if (!fe_interp(e, aux, (double)x, (double)y, &zz, &dx, &dy)) return 4;
+            X[2] = (R)zz;
+            *F = z - (R)zz;
+            DF[0] = (R)dx; DF[1] = (R)dy; DF[2] = 1;
+            return 0;
+        }
     }
     default: {  // SURF_SPLINE
         R rho = M<R>::sqrt_(x * x + y * y);
@@ -857,10 +1043,11 @@ struct ElemHit {
 // Ray vs one element: Intersect + DetermineElementIntersectionNew
 // (coretrace/intersect.cpp:140-1011, determineelementintersectionnew.cpp:58-633).
 // P, D: the nudged ray in the element frame (double).  Returns true on an
-// intercept (before the caller's closest-hit / ZAperture rule).  G = false
+// intercept (before the caller's closest-hit / ZAperture rule); on a miss
+// where a finite-element interpolation failed, h.back is set to 2.  G = false
 // is the variant for stages whose elements are all closed-form quadrics or
 // flats: the torus and Newton code is left out of the kernel.
-template <typename R, bool G>
+template <typename R, int G>
 STD bool element_hit(const DevElem &e, const double *aux, const double P[3], const double D[3], ElemHit<R> &h) {
     const int s = e.surf;
     R X[3], DF[3];
@@ -955,7 +1142,11 @@ STD bool element_hit(const DevElem &e, const double *aux, const double P[3], con
             X[1] = Y1 + Dr[1] * SJ;
             X[2] = Z1 + Dr[2] * SJ;
             R F;
-            if (surface_eval<R>(e, aux, X, &F, DF) != 0) break;
+            const int code = surface_eval<R, G>(e, aux, X, &F, DF);
+            if (code != 0) {
+                if (code == 4) h.back = 2;  // finite-element interpolation failed
+                break;
+            }
             R dfd = dot3<R>(DF, Dr);
             if (M<R>::fabs_(F) <= eps * M<R>::fabs_(dfd)) { ok = true; break; }
             SJ1 = SJ - F / dfd;
@@ -987,9 +1178,11 @@ STD bool element_hit(const DevElem &e, const double *aux, const double P[3], con
 // heliostat normal (curvature radius ~1 km) by ~1e-9 rad; reflected towards
 // a receiver hundreds of metres away at grazing incidence that grows to
 // 1e-4 m.  P, D: the nudged ray in the element frame.
-template <bool G>
-STD void hit_finish_d(const DevElem &e, const double *aux, const double P[3], const double D[3], double path,
+// Returns false where a finite-element interpolation failed.
+template <int G>
+STD bool hit_finish_d(const DevElem &e, const double *aux, const double P[3], const double D[3], double path,
                       double X[3], double DF[3], int *back) {
+    bool ok = true;
     const int s = e.surf;
     if (s == SURF_FLAT || (!G && s > SURF_CYLINDER)) {
         // the reference's start plane z = 0, taken as the hit
@@ -1029,14 +1222,16 @@ STD void hit_finish_d(const DevElem &e, const double *aux, const double P[3], co
         // iterate, then the surface z and slopes at the final point
         double Xc[3], F;
         for (int i = 0; i < 3; i++) Xc[i] = P[i] + path * D[i];
-        surface_eval<double>(e, aux, Xc, &F, DF);
+        ok = surface_eval<double, G>(e, aux, Xc, &F, DF) != 4;
         const double dfd = DF[0] * D[0] + DF[1] * D[1] + DF[2] * D[2];
-        const double t = (dfd != 0.0) ? path - F / dfd : path;
+        const double t = (ok && dfd != 0.0) ? path - F / dfd : path;
         for (int i = 0; i < 3; i++) X[i] = P[i] + t * D[i];
-        surface_eval<double>(e, aux, X, &F, DF);
+        ok = ok && surface_eval<double, G>(e, aux, X, &F, DF) != 4;
+        if (!ok) { DF[0] = 0.0; DF[1] = 0.0; DF[2] = 1.0; }
     }
     *back = (dot3<double>(D, DF) < 0.0) ? 0 : 1;
     if (*back) { DF[0] = -DF[0]; DF[1] = -DF[1]; DF[2] = -DF[2]; }
+    return ok;
 }
 
 // ------------------------------------------------------------ optics ------

@@ -71,6 +71,7 @@ struct TraceArgs {
     DevRay *out_slots;           // n_rays
     uint32_t *out_flag;          // n_rays
     uint32_t *err_flags;         // bit 0: bad interaction, 1: phase mismatch, 2: overflow full
+                                 // 3: sunshape sampling, 4: finite-element interpolation
     FluxSpec flux;
     uint32_t *flux_bins;
     // event counters (nullptr: off): rays, element tests, BVH nodes,
@@ -86,6 +87,7 @@ struct MaskArgs {
     uint64_t s0;                 // first sun-ray index
     uint32_t n;                  // sun rays in this launch
     uint64_t *mask;              // one word per 64 sun rays
+    unsigned long long *fe_fail; // least sun-ray index whose finite-element interpolation failed
     unsigned long long *counters;  // as TraceArgs::counters (nullptr: off)
     int32_t generic;             // kernel variant: DevStage::generic of stage 0
     int32_t compact;             // screen grid candidates into per-wave pair queues

@@ -200,6 +200,24 @@ bool read_surface_file(HElement &e, const char *file, std::string &msg) {
         }
         e.surf = 'v';
         e.stype = 5;
+    } else if (ext == "fed") {
+        // coretrace/input.cpp:510-561: title, node count, x y z per node,
+        // then the spatial hash tree over the nodes
+        next();
+        next();
+        const int n = atoi(line);
+        e.fe_nodes.assign((size_t)std::max(n, 0) * 3, 0.0);
+        for (int i = 0; i < n; i++) {
+            double x = 0, y = 0, z = 0;
+            next();
+            sscanf(line, "%lg %lg %lg", &x, &y, &z);
+            e.fe_nodes[3 * (size_t)i] = x;
+            e.fe_nodes[3 * (size_t)i + 1] = y;
+            e.fe_nodes[3 * (size_t)i + 2] = z;
+        }
+        st_fe::build(e.fe_tree, e.fe_nodes);
+        e.surf = 'e';
+        e.stype = 4;
     } else if (ext == "ply") {
         next();
         int order = atoi(line);
@@ -347,6 +365,15 @@ static void aperture_plane(HElement &e) {
     case 'i':
         e.zap = e.sy.empty() ? 0.0 : e.sy.back();
         break;
+    case 'e': {
+        // the largest node *y* (column 1), at least 0, as the reference has it
+        // (coretrace/apertureplane.cpp:458-471)
+        double lastz = 0.0;
+        for (size_t i = 0; i < e.fe_nodes.size() / 3; i++)
+            if (e.fe_nodes[i * 3 + 1] > lastz) lastz = e.fe_nodes[i * 3 + 1];
+        e.zap = lastz;
+        break;
+    }
     case 'v': {
         // the largest fitted z over the data points, at least 0
         // (coretrace/apertureplane.cpp:474-488)
@@ -420,6 +447,7 @@ static int surf_kind(const HElement &e) {
     case 'f': return SURF_FLAT;
     case 'm': return SURF_ZERNIKE;
     case 'v': return SURF_VSHOT;
+    case 'e': return SURF_FE;
     case 'i': return SURF_SPLINE;
     case 'r': return SURF_POLY;
     default: return SURF_GENERAL;  // 'g', 'c'
@@ -488,7 +516,8 @@ static double bound_z(const HElement &e, double x, double y) {
 // hyperboloid -- and 0 for the torus, whose quartic solution
 // (coretrace/torusclosedform.cpp, root432.cpp) can land off the surface, far
 // outside the torus, so culling must always test it (box 0 is still its
-// nominal box).
+// nominal box); 0 also for finite-element surfaces, whose kriging
+// interpolant can overshoot its nodes.
 static int element_boxes(const HElement &e, double lo[2][3], double hi[2][3]) {
     const double A = e.p[0], B = e.p[1], C = e.p[2];
     double xl, xh, yl, yh;
@@ -582,7 +611,7 @@ static int element_boxes(const HElement &e, double lo[2][3], double hi[2][3]) {
         lo[b][1] = yl - pad; hi[b][1] = yh + pad;
         lo[b][2] = zl[src] - pad; hi[b][2] = zh[src] + pad;
     }
-    return sf == 'd' ? 0 : nb;
+    return (sf == 'd' || sf == 'e') ? 0 : nb;
 }
 
 // ------------------------------------------------------------------- BVH --
@@ -839,7 +868,33 @@ bool compile_scene(SceneModel &m, bool use_cull, CompiledScene &out, std::string
                         d.face_back = optic_face0[k] + 1;
                     }
             }
-            if (d.surf == SURF_ZERNIKE || d.surf == SURF_VSHOT) {
+            if (d.surf == SURF_FE) {
+                // the node tree flattened into aux (doubles): a header
+                // {x0, x1, y0, y1, address bits, derivative step, zones,
+                // nodes}, then per zone {m0, m1, terminal, first candidate,
+                // candidates}, then x y z per node, then the candidate lists
+                // (the zone's nodes, then its neighbours' nodes)
+                const st_fe::Tree &t = e.fe_tree;
+                const int nn = (int)(e.fe_nodes.size() / 3), nz = (int)t.zones.size();
+                const double density = (double)nn / aperture_area(e);
+                d.aux = (int)out.aux.size();
+                out.aux.insert(out.aux.end(), {t.xlim[0], t.xlim[1], t.ylim[0], t.ylim[1], (double)t.key_length(),
+                                               0.001 / sqrt(density), (double)nz, (double)nn});
+                const size_t zone0 = out.aux.size();
+                out.aux.resize(zone0 + 5 * (size_t)nz);
+                out.aux.insert(out.aux.end(), e.fe_nodes.begin(), e.fe_nodes.end());
+                for (int z = 0; z < nz; z++) {
+                    const st_fe::Zone &zn = t.zones[z];
+                    double *rec = &out.aux[zone0 + 5 * (size_t)z];
+                    rec[0] = zn.m0;
+                    rec[1] = zn.m1;
+                    rec[2] = zn.terminal ? 1.0 : 0.0;
+                    rec[3] = (double)out.aux.size();
+                    rec[4] = (double)(zn.data.size() + zn.neighbors.size());
+                    for (int i : zn.data) out.aux.push_back(i);
+                    for (int i : zn.neighbors) out.aux.push_back(i);
+                }
+            } else if (d.surf == SURF_ZERNIKE || d.surf == SURF_VSHOT) {
                 d.aux = (int)out.aux.size();
                 d.aux_n = e.fit_order + 1;
                 out.aux.insert(out.aux.end(), e.zern.begin(), e.zern.end());
@@ -876,7 +931,8 @@ bool compile_scene(SceneModel &m, bool use_cull, CompiledScene &out, std::string
                                   (hi[0][2] - lo[0][2]) * (hi[0][2] - lo[0][2]));
             for (int k = 0; k < 3; k++) d.bsphere_c[k] = c[k];
             d.bsphere_r = r;
-            if (d.surf != SURF_FLAT && d.surf > SURF_CYLINDER) ds.generic = 1;
+            if (d.surf != SURF_FLAT && d.surf > SURF_CYLINDER) ds.generic = std::max(ds.generic, 1);
+            if (d.surf == SURF_FE) ds.generic = 2;
             out.elems.push_back(d);
             for (int q = 0; e.enabled && q < std::max(nb, 1); q++) {
                 // stage-frame box from the 8 corners of an element-frame box

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "st_fetree.h"
 #include "st_types.h"
 
 namespace st {
@@ -45,6 +46,8 @@ struct HElement {
     std::vector<double> poly;            // .ply coefficients
     std::vector<double> sx, sy, sy2;     // cubic spline
     std::vector<double> vshot;           // VSHOT points: x, y, x slope, y slope
+    std::vector<double> fe_nodes;        // finite-element nodes: x, y, z
+    st_fe::Tree fe_tree;                 // their spatial hash tree
     int interaction = 0;
     std::string optic_name;
     // filled by compile()

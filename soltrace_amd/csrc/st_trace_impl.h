@@ -83,6 +83,7 @@ struct Best {
     int j;               // element index within the stage
     uint32_t tests;      // element tests (event counter)
     uint32_t nodes;      // BVH nodes visited (event counter)
+    uint32_t fefail;     // a finite-element interpolation failed on the way
 };
 
 // The ray in element j's frame, nudged 1e-5 m along itself
@@ -97,7 +98,7 @@ STD void element_ray(const DevElem &e, const double ps[3], const double cs[3], d
 // Test element j; keep it when it is the closest accepted hit so far
 // (coretrace/raytrace.cpp:740-768; ties go to the lower element index, the
 // order a brute-force scan meets them).
-template <typename R, bool G>
+template <typename R, int G>
 STD void consider(const SceneView &sv, const DevStage &st, int j, const double ps[3], const double cs[3], Best &b) {
     const DevElem &e = sv.elems[st.elem0 + j];
     if (!e.enabled) return;
@@ -105,7 +106,11 @@ STD void consider(const SceneView &sv, const DevStage &st, int j, const double p
     double pe[3], ce[3];
     element_ray(e, ps, cs, pe, ce);
     ElemHit<R> h;
-    if (!element_hit<R, G>(e, sv.aux, pe, ce, h)) return;
+    h.back = 0;
+    if (!element_hit<R, G>(e, sv.aux, pe, ce, h)) {
+        if (G == 2 && h.back == 2) b.fefail = 1;
+        return;
+    }
     if (h.path < b.path || (h.path == b.path && j < b.j)) {
         if (!e.zap_filter || (double)h.X[2] <= e.zap) {
             b.path = h.path;
@@ -136,11 +141,12 @@ STD bool foot_contains(const DevFoot *foot, int j, double sx, double sy) {
 // >= 0), screened by the elements' footprints at (sx, sy) when `screen`; the
 // stage BVH when present; every element otherwise.  One loop feeds them all
 // to a single consider() so the element code is instantiated once.
-template <typename R, bool G>
+template <typename R, int G>
 STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3], const double cs[3], int cell,
                      bool screen, double sx, double sy, Best &b) {
     b.path = 1e99;
     b.j = 0x7fffffff;
+    b.fefail = 0;
     const int32_t *items = nullptr;  // nullptr: the item is the element index
     int k = 0, kend = 0;
     bool bvh = false;
@@ -204,19 +210,20 @@ STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3]
 }
 
 // Any accepted hit at all (the first-hit test of a sun ray).
-template <typename R, bool G>
+template <typename R, int G>
 STD bool any_hit(const SceneView &sv, const DevStage &st, const double ps[3], const double cs[3], bool grid,
-                 double sx, double sy, uint32_t *tests) {
+                 double sx, double sy, uint32_t *tests, uint32_t *fefail) {
     Best b;
     b.tests = 0;
     b.nodes = 0;
     closest_hit<R, G>(sv, st, ps, cs, grid ? grid_cell(sv, sx, sy) : -1, grid, sx, sy, b);
     *tests += b.tests;
+    *fefail = b.fefail;
     return b.path < 1e99;
 }
 
 // ---------------------------------------------------------- sun mask ------
-template <typename R, bool G>
+template <typename R, int G>
 __global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     bool hit = false;
@@ -227,8 +234,10 @@ __global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
         double pg[3], cg[3], psun[2], ps[3], cs[3];
         gen_sun_ray(rng, a.sv.sun, st, pg, cg, psun);
         to_local(pg, cg, st.org, st.R, ps, cs);
-        uint32_t tests = 0;
-        hit = any_hit<R, G>(a.sv, st, ps, cs, a.sv.use_grid && !a.sv.sun.point_source, psun[0], psun[1], &tests);
+        uint32_t tests = 0, fefail = 0;
+        hit = any_hit<R, G>(a.sv, st, ps, cs, a.sv.use_grid && !a.sv.sun.point_source, psun[0], psun[1], &tests,
+                            &fefail);
+        if (G == 2 && fefail) atomicMin(a.fe_fail, (unsigned long long)(a.s0 + t));
         if (a.counters) {
             atomicAdd(&a.counters[0], 1ull);
             atomicAdd(&a.counters[1], (unsigned long long)tests);
@@ -257,7 +266,7 @@ static_assert(kMaskRays * 64 <= 256, "ray slot must fit in 8 bits");
 
 STD uint64_t lanes_below() { return (1ull << (threadIdx.x & 63)) - 1ull; }
 
-template <typename R, bool G>
+template <typename R, int G>
 __global__ __launch_bounds__(kBlock) void sun_mask_grid_kernel(MaskArgs a) {
     constexpr int W = kBlock / 64;
     __shared__ uint32_t ring[W][kMaskQueue];
@@ -305,8 +314,10 @@ __global__ __launch_bounds__(kBlock) void sun_mask_grid_kernel(MaskArgs a) {
             b.j = 0x7fffffff;
             b.tests = 0;
             b.nodes = 0;
+            b.fefail = 0;
             consider<R, G>(sv, st, j, ps, cs, b);
             if (b.path < 1e99) atomicOr(&hitw[wv][(slot >> 6) * 2 + ((slot >> 5) & 1)], 1u << (slot & 31));
+            if (G == 2 && b.fefail) atomicMin(a.fe_fail, (unsigned long long)(a.s0 + ray0 + (uint64_t)slot));
         }
         head += m;
         count -= m;
@@ -487,7 +498,7 @@ STD int flux_bin(const FluxSpec &fl, const double pos[3], const double cos[3]) {
     return (gx >= 0 && gx < fl.nx && gy >= 0 && gy < fl.ny) ? gx * fl.ny + gy : -1;
 }
 
-template <typename R, bool G>
+template <typename R, int G>
 // Waves per SIMD: the generic variant needs the whole register file; the
 // basic FP32 variant runs 3 (a few spills are cheaper than the latency they
 // hide), the basic FP64 variant 2.
@@ -566,6 +577,7 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
         b.nodes = 0;
         // rays here are known to hit, so footprints would screen out little
         closest_hit<R, G>(sv, st, ps, cs, mhc == 0 ? cell : -1, false, 0.0, 0.0, b);
+        if (G == 2 && b.fefail) atomicOr(a.err_flags, 16u);
         if (a.counters) {
             ev_lds[0][threadIdx.x] += b.tests;
             ev_lds[1][threadIdx.x] += b.nodes;
@@ -588,7 +600,7 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
         int back;
         element_ray(e, ps, cs, pe, ce);
         if constexpr (FAST) {
-            hit_finish_d<G>(e, sv.aux, pe, ce, b.path, xd, df, &back);
+            if (!hit_finish_d<G>(e, sv.aux, pe, ce, b.path, xd, df, &back)) atomicOr(a.err_flags, 16u);
         } else {
             // the winner again: the same arithmetic gives the same point
             ElemHit<R> h;
@@ -689,19 +701,22 @@ void launch_sun_mask(const MaskArgs &a, hipStream_t s) {
     if (a.compact) {
         const uint32_t per_block = kBlock * kMaskRays;
         const uint32_t blocks = (a.n + per_block - 1) / per_block;
-        if (a.generic) hipLaunchKernelGGL((sun_mask_grid_kernel<R, true>), dim3(blocks), dim3(kBlock), 0, s, a);
-        else hipLaunchKernelGGL((sun_mask_grid_kernel<R, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+        if (a.generic == 2) hipLaunchKernelGGL((sun_mask_grid_kernel<R, 2>), dim3(blocks), dim3(kBlock), 0, s, a);
+        else if (a.generic) hipLaunchKernelGGL((sun_mask_grid_kernel<R, 1>), dim3(blocks), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((sun_mask_grid_kernel<R, 0>), dim3(blocks), dim3(kBlock), 0, s, a);
         return;
     }
     const uint32_t blocks = (a.n + kBlock - 1) / kBlock;
-    if (a.generic) hipLaunchKernelGGL((sun_mask_kernel<R, true>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL((sun_mask_kernel<R, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+    if (a.generic == 2) hipLaunchKernelGGL((sun_mask_kernel<R, 2>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else if (a.generic) hipLaunchKernelGGL((sun_mask_kernel<R, 1>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((sun_mask_kernel<R, 0>), dim3(blocks), dim3(kBlock), 0, s, a);
 }
 template <typename R>
 void launch_trace_stage(const TraceArgs &a, hipStream_t s) {
     const int blocks = (a.n_rays + kBlock - 1) / kBlock;
-    if (a.generic) hipLaunchKernelGGL((trace_stage_kernel<R, true>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL((trace_stage_kernel<R, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+    if (a.generic == 2) hipLaunchKernelGGL((trace_stage_kernel<R, 2>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else if (a.generic) hipLaunchKernelGGL((trace_stage_kernel<R, 1>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((trace_stage_kernel<R, 0>), dim3(blocks), dim3(kBlock), 0, s, a);
 }
 
 }  // namespace st

@@ -22,6 +22,7 @@ enum SurfKind : int32_t {
     SURF_GENERAL = 9,     // 'g'/'c'/'o'(type 7) Newton on the conic (type 1/7)
     SURF_POLY = 10,       // 'r'  Newton      (type 8)
     SURF_VSHOT = 11,      // 'v'  Newton      (type 5): monomial z, measured slopes
+    SURF_FE = 12,         // 'e'  Newton      (type 4): kriging over finite-element nodes
 };
 
 enum ApKind : int32_t {
@@ -55,7 +56,8 @@ struct DevElem {
     int32_t aux, aux_n;     // Zernike B (n x n) or spline x,y,y2 (3 x n)
     int32_t fit_order, enabled;
     int32_t zap_filter;     // 0 for 'm' and 'r' (raytrace.cpp:747-751)
-    int32_t aux2, aux2_n;   // VSHOT points (x, y, x slope, y slope) and their count
+    int32_t aux2, aux2_n;   // VSHOT points (x, y, x slope, y slope) and their count;
+                            // finite element: aux is the flattened node tree (see st_scene.cpp)
     int32_t pad1;
     double roi2;            // VSHOT region of interest, squared radius (m^2)
 };
@@ -66,7 +68,8 @@ struct DevStage {
     int32_t elem0, nelem;   // slice of the element table
     int32_t virt, multihit, tracethrough;
     int32_t bvh0, nbvh;     // slice of the BVH node table (0 nodes: brute force)
-    int32_t generic;        // 1: an element needs the torus or a Newton surface other than flat
+    int32_t generic;        // 1: an element needs the torus or a Newton surface other than flat;
+                            // 2: and one is a finite-element surface
 };
 
 // Sun (coretrace/types.h:195-221) with the sun frame and sun-plane box.

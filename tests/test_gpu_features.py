@@ -13,7 +13,7 @@ import pytest
 
 from soltrace_amd import workloads
 from harness import (assert_records_match, counter_mode, hip_sun_index, oracle_sun_index, run_scene)
-from test_oracle_vs_ref import CASES, _mini, _write, torus_target_scene, vshot_scene
+from test_oracle_vs_ref import CASES, _mini, _write, fe_scene, torus_target_scene, vshot_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -91,6 +91,49 @@ def test_vshot_surface_fp32(hip_lib, oracle_lib):
             worst = max(worst, float(np.abs(r_o.pos[io] - r_g.pos[ig]).max()))
     assert same >= 0.995 * len(common), (same, len(common))
     assert worst <= 1e-4, worst
+
+
+@pytest.mark.parametrize("fed", ["dish.fed", "dish_un.fed"])
+def test_finite_element_surface_fp64(hip_lib, oracle_lib, fed):
+    # kriging over the hash-tree neighbourhood of the nearest mesh nodes
+    # (coretrace/surface.cpp:218-242, vshot.cpp:166-238); the device's pow
+    # differs from the host's in the last bit, which the kriging system can
+    # amplify, so points are held to 1e-6 m
+    sc = fe_scene(fed, n=3000)
+    n_o, r_o, _, _ = run_scene(oracle_lib, sc, 19, setup=counter_mode)
+    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 19, setup=_setup(64))
+    assert n_g == n_o and n_o > 0, (n_g, n_o, m_g)
+    assert_records_match(r_g, r_o, pos_tol=1e-6, cos_tol=1e-6)
+
+
+def test_finite_element_surface_fp32(hip_lib, oracle_lib):
+    sc = fe_scene("dish.fed", n=3000)
+    n_o, r_o, _, si_o = run_scene(oracle_lib, sc, 19, setup=counter_mode, after=oracle_sun_index)
+    n_g, r_g, m_g, si_g = run_scene(hip_lib, sc, 19, setup=_setup(32),
+                                    after=lambda lib, ctx: hip_sun_index(lib, ctx, 3000))
+    assert n_g > 0, m_g
+    so, sg = _by_ray(r_o, si_o), _by_ray(r_g, si_g)
+    common = set(so) & set(sg)
+    assert len(common) >= 0.995 * max(len(so), len(sg))
+    same, worst = 0, 0.0
+    for k in common:
+        io, ig = so[k], sg[k]
+        if len(io) == len(ig) and np.array_equal(r_o.element[io], r_g.element[ig]):
+            same += 1
+            worst = max(worst, float(np.abs(r_o.pos[io] - r_g.pos[ig]).max()))
+    assert same >= 0.995 * len(common), (same, len(common))
+    assert worst <= 1e-4, worst
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_finite_element_failure_fails_the_run(hip_lib, oracle_lib, bits):
+    # rays through the mesh's hole find no hash-tree zone: the run fails with
+    # the reference's message, as the oracle's does
+    sc = fe_scene("holed.fed", n=1000)
+    n_o, _, m_o, _ = run_scene(oracle_lib, sc, 3, setup=counter_mode)
+    n_g, _, m_g, _ = run_scene(hip_lib, sc, 3, setup=_setup(bits))
+    assert n_o == n_g == -1
+    assert m_g[-1] == m_o[-1] and "Interpolation failed" in m_g[-1]
 
 
 def _curved_field():

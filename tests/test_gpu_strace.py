@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from test_strace import SAMPLES, STRACE, STRACE_ORACLE, run_cli
+from test_strace import FE_FILES, FE_GOLDEN, SAMPLES, STRACE, STRACE_ORACLE, run_cli
 
 pytestmark = pytest.mark.gpu
 
@@ -41,6 +41,37 @@ def test_strace_hip_equals_oracle(hip_lib, tmp_path, name):
     np.testing.assert_array_equal(rg[:, 6:], ro[:, 6:])
     tol = 1e-6 + 1e-6 * np.abs(ro[:, :6])
     assert np.all(np.abs(rg[:, :6] - ro[:, :6]) <= tol), np.abs(rg[:, :6] - ro[:, :6]).max()
+
+
+@pytest.mark.skipif(not os.path.exists(STRACE_ORACLE), reason="oracle not built (make -C oracle)")
+def test_strace_finite_element_hip_equals_oracle(hip_lib, tmp_path):
+    # kriging over two meshed dish facets (tests/golden/make_fe_golden.py)
+    args = ["2000", "1000000", "13", "1", "1", "0"]
+    a, b = tmp_path / "hip", tmp_path / "oracle"
+    a.mkdir()
+    b.mkdir()
+    print("strace fe_dish " + " ".join(args), flush=True)
+    got, r1 = run_cli(STRACE, a, "fe_dish", args, src_dir=FE_GOLDEN, extra=FE_FILES)
+    want, r2 = run_cli(STRACE_ORACLE, b, "fe_dish", args, prefix=["--counter"], src_dir=FE_GOLDEN, extra=FE_FILES)
+    assert r1.returncode == 0 and got is not None, r1.stdout + r1.stderr
+    assert r2.returncode == 0 and want is not None, r2.stdout + r2.stderr
+    hg, rg = parse(got)
+    ho, ro = parse(want)
+    assert hg[4] == ho[4] and hg[5] == ho[5] and hg[5] > 0, (hg, ho)
+    np.testing.assert_array_equal(rg[:, 6:], ro[:, 6:])
+    tol = 1e-6 + 1e-6 * np.abs(ro[:, :6])
+    assert np.all(np.abs(rg[:, :6] - ro[:, :6]) <= tol), np.abs(rg[:, :6] - ro[:, :6]).max()
+
+
+def test_strace_finite_element_hole_fails_the_run(hip_lib, tmp_path):
+    # the interpolation finds no zone in the mesh's hole: the run fails with
+    # the reference's message and writes no .rays file
+    print("strace fe_holed", flush=True)
+    got, r = run_cli(STRACE, tmp_path, "fe_holed", ["300", "200000", "5", "1", "1", "0"], src_dir=FE_GOLDEN,
+                     extra=FE_FILES)
+    assert got is None, r.stdout
+    assert r.stdout.splitlines()[-1] == "error: trace error: Interpolation failed to find nearby nodes to ray hit " \
+                                        "in FEA surface."
 
 
 def test_read_stinput_and_ray_files_through_the_library(hip_lib, tmp_path):

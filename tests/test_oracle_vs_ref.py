@@ -174,6 +174,34 @@ def test_vshot_file_bitexact(ref_lib, oracle_lib):
     assert _compare(ref_lib, oracle_lib, vshot_scene(), 6) > 0
 
 
+FE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fe")
+
+
+def fe_scene(fed="dish.fed", n=1500):
+    # a rippled dish meshed by 961 nodes (tests/golden/make_fe_golden.py) on a
+    # 1 x 1 m facet: kriging over the hash-tree neighbourhood
+    # (coretrace/surface.cpp:218-242, vshot.cpp:166-238, treemesh.cpp)
+    return _mini(surface="e", sp=(), ap="r", app=(1.0, 1.0), surf_file=os.path.join(FE_DIR, fed), n=n)
+
+
+@pytest.mark.parametrize("fed", ["dish.fed", "dish_un.fed", "dish_dup.fed"])
+def test_finite_element_file_bitexact(ref_lib, oracle_lib, fed):
+    # dish_dup.fed repeats nodes, so some kriging systems are singular and
+    # their solution follows the last bit of pow: bit-exact here, where the
+    # oracle and the reference build share the host's pow
+    assert _compare(ref_lib, oracle_lib, fe_scene(fed), 6) > 0
+
+
+def test_finite_element_failure_matches(ref_lib, oracle_lib):
+    # rays through the mesh's hole reach hash-tree zones that do not exist:
+    # the reference throws out of Trace (coretrace/vshot.cpp:177-178)
+    sc = fe_scene("holed.fed")
+    n_r, _, m_r, _ = run_scene(ref_lib, sc, 3)
+    n_o, _, m_o, _ = run_scene(oracle_lib, sc, 3)
+    assert n_r == n_o == -1
+    assert m_r[-1] == m_o[-1] and "Interpolation failed" in m_o[-1]
+
+
 def test_error_paths_match(ref_lib, oracle_lib):
     sc = workloads.trough(raycount=1000)
     sc.maxcount = 50  # sun-ray cap reached (coretrace/raytrace.cpp:600-604)

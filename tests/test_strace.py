@@ -25,12 +25,19 @@ GOLDEN_ARGS = ["100", "100000", "7", "1", "1", "0"]
 needs_oracle = pytest.mark.skipif(not os.path.exists(STRACE_ORACLE), reason="oracle not built (make -C oracle)")
 
 
-def run_cli(exe, tmp_path, name, args, prefix=()):
-    """Run a strace command line on a fresh copy of a sample; return the
-    .rays text (None when none was written) and the process result."""
-    src = os.path.join(GOLDEN, name + ".stinput")
+FE_GOLDEN = os.path.join(ROOT, "tests", "golden", "fe")
+FE_FILES = ("dish.fed", "dish_un.fed", "dish_dup.fed", "holed.fed")
+
+
+def run_cli(exe, tmp_path, name, args, prefix=(), src_dir=GOLDEN, extra=()):
+    """Run a strace command line on a fresh copy of a sample (and of the
+    surface files it names); return the .rays text (None when none was
+    written) and the process result."""
+    src = os.path.join(src_dir, name + ".stinput")
     dst = tmp_path / (name + ".stinput")
     shutil.copy(src, dst)
+    for f in extra:
+        shutil.copy(os.path.join(src_dir, f), tmp_path / f)
     r = subprocess.run([exe, *prefix, str(dst), *args], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     rays = tmp_path / (name + ".stinput.rays")
     return (rays.read_text() if rays.exists() else None), r
@@ -154,3 +161,40 @@ def test_finite_element_sample_equals_reference_build(tmp_path, args):
         assert r.returncode == 0, r.stdout[-2000:]
         outs.append((d / (name + ".stinput.rays")).read_text())
     assert outs[0] == outs[1] and len(outs[0].splitlines()) > 300
+
+
+@needs_oracle
+def test_finite_element_fixture_equals_reference_golden(tmp_path):
+    # a rippled dish meshed twice (structured; random with duplicate corner
+    # nodes), golden .rays written by the reference build
+    # (tests/golden/make_fe_golden.py)
+    got, r = run_cli(STRACE_ORACLE, tmp_path, "fe_dish", ["300", "200000", "5", "1", "1", "0"], src_dir=FE_GOLDEN,
+                     extra=FE_FILES)
+    assert r.returncode == 0, r.stdout
+    assert got == open(os.path.join(FE_GOLDEN, "fe_dish.rays")).read()
+
+
+@needs_oracle
+@pytest.mark.skipif(not os.path.exists(STRACE_REF), reason="reference build absent (make -C oracle ref)")
+@pytest.mark.parametrize("name", ["fe_dish", "fe_holed"])
+def test_finite_element_fixture_equals_reference_build(tmp_path, name):
+    args = ["700", "400000", "11", "1", "1", "0"]
+    (tmp_path / "ours").mkdir()
+    (tmp_path / "ref").mkdir()
+    ours, r1 = run_cli(STRACE_ORACLE, tmp_path / "ours", name, args, src_dir=FE_GOLDEN, extra=FE_FILES)
+    ref, r2 = run_cli(STRACE_REF, tmp_path / "ref", name, args, src_dir=FE_GOLDEN, extra=FE_FILES)
+    assert r1.returncode == r2.returncode == 0
+    assert ours == ref
+    assert r1.stdout.splitlines()[-1] == r2.stdout.splitlines()[-1]
+
+
+@needs_oracle
+def test_finite_element_hole_fails_the_run(tmp_path):
+    # rays through the mesh's hole reach hash-tree zones that do not exist;
+    # the reference throws out of Trace (coretrace/vshot.cpp:177-178) and
+    # strace reports it and writes no .rays file
+    got, r = run_cli(STRACE_ORACLE, tmp_path, "fe_holed", ["300", "200000", "5", "1", "1", "0"], src_dir=FE_GOLDEN,
+                     extra=FE_FILES)
+    assert got is None
+    assert r.stdout.splitlines()[-1] == "error: trace error: Interpolation failed to find nearby nodes to ray hit " \
+                                        "in FEA surface."
