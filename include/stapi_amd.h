@@ -98,8 +98,53 @@ int st_amd_count_events(st_context_t pcxt, int on);
  * values; returns 104. */
 int st_amd_events(st_context_t pcxt, double *out, int n);
 
+/* Stage-data save/resume: the reference's st_sim_run_data
+ * (coretrace/stapi.cpp:680-728; raytrace.cpp:544-578, 1044-1078;
+ * generateray.cpp:165-219) with plain arrays in place of its std::vector
+ * arguments.
+ *   s1, n1   rows of stage-1 records to load, 8 doubles each: x, y, z,
+ *            cos x, cos y, cos z (stage frame), element, ray number
+ *   s2, n2   rows of rays entering stage 2 to load, 7 doubles each: x, y, z,
+ *            cos x, cos y, cos z (global frame), ray number
+ * With n1 > 0 and n2 > 0, stage 1 is not traced. Its records are the rows
+ * of s1, stage 2 starts from the rays of s2, and the sun-ray count is the
+ * last s1 row's ray number. Random draws of a loaded ray use the stream of
+ * sun-ray index (ray number - 1). n2 > 0 alone sets the ray count to n2, as
+ * in the reference. The flux map of st_amd_flux_request does not see loaded
+ * records.
+ * With save_stage_data, the run keeps the rows the reference appends to its
+ * two vectors; st_amd_saved_stage_data returns them.
+ * Returns the record count, or -1 (messages as st_sim_run). */
+int st_amd_sim_run_data(st_context_t pcxt, unsigned int seed, const double *s1, st_uint_t n1, const double *s2,
+                        st_uint_t n2, int save_stage_data,
+                        int (*callback)(st_uint_t ntracedtotal, st_uint_t ntraced, st_uint_t ntotrace,
+                                        st_uint_t curstage, st_uint_t nstages, void *data),
+                        void *data);
+
+/* Rows saved by the last st_amd_sim_run_data with save_stage_data.
+ *   which 1: every stage-1 record, in trace order (8 doubles a row, as s1);
+ *   which 2: the reference's stage-2 ray array as stage 2 leaves it (7
+ *            doubles a row, as s2). The reference reuses that array in
+ *            place, so its first rows are the rays leaving stage 2 and the
+ *            rest are the rays that entered stage 2 after them.
+ * Writes the rows to out when it is not NULL; returns the row count, or -1
+ * for another `which`. */
+int st_amd_saved_stage_data(st_context_t pcxt, int which, double *out);
+
 #ifdef __cplusplus
 }
+
+#include <vector>
+
+/* The reference's own entry point, with the C++ linkage it has there
+ * (coretrace/stapi.cpp:680-685; the declaration in stapi.h:159-162 is
+ * commented out). Loads from, or appends saved rows to, the two vectors as
+ * the reference does, by way of st_amd_sim_run_data. */
+int st_sim_run_data(st_context_t pcxt, unsigned int seed, std::vector<std::vector<double> > *data_s1,
+                    std::vector<std::vector<double> > *data_s2, bool save_stage_data,
+                    int (*callback)(st_uint_t ntracedtotal, st_uint_t ntraced, st_uint_t ntotrace,
+                                    st_uint_t curstage, st_uint_t nstages, void *data),
+                    void *data);
 #endif
 
 #endif

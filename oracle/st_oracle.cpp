@@ -139,6 +139,9 @@ static bool trace(System &sys, uint32_t seed, uint64_t N, uint64_t maxrays, bool
     double pos_sun_stage[3];
     if (!sun_to_primary_stage(sys, *sys.stages[0], pos_sun_stage)) return false;
 
+    // stage 1 from saved rows when both row sets are given
+    // (coretrace/raytrace.cpp:159-163)
+    const bool load = sys.load_n1 > 0 && sys.load_n2 > 0;
     std::vector<GlobalRay> incoming;  // rays passed from the previous stage
     bool prev_has_rays = false;
     uint64_t last_prev = N;  // LastRayNumberInPreviousStage
@@ -155,6 +158,32 @@ static bool trace(System &sys, uint32_t seed, uint64_t N, uint64_t maxrays, bool
         size_t read = 0;
         uint64_t ray_number = 1;
         bool stage_done = false;
+        if (si == 0 && load) {
+            // coretrace/raytrace.cpp:544-578, generateray.cpp:165-219: the
+            // records are the rows of s1, the rays entering stage 2 those of
+            // s2; the fields convert as the reference's casts do
+            for (uint64_t j = 0; j < sys.load_n1; j++) {
+                const double *v = sys.load_s1 + 8 * j;
+                Record r;
+                for (int k = 0; k < 3; k++) { r.pos[k] = v[k]; r.cos[k] = v[k + 3]; }
+                r.element = (int)(int64_t)v[6];  // double -> st_uint_t -> int
+                r.stage = 1;
+                r.raynum = (uint32_t)(uint64_t)(int64_t)v[7];
+                st.recs.push_back(r);
+            }
+            for (uint64_t j = 0; j < sys.load_n2; j++) {
+                const double *v = sys.load_s2 + 7 * j;
+                GlobalRay g;
+                for (int k = 0; k < 3; k++) { g.pos[k] = v[k]; g.cos[k] = v[k + 3]; }
+                g.num = (uint32_t)(int)v[6];
+                g.sun_index = (uint64_t)g.num - 1;  // the counter streams of a loaded ray
+                outgoing.push_back(g);
+            }
+            prev_has_rays = true;
+            prev_idx = (int64_t)sys.load_n2 - 1;
+            sys.sunraycount = st.recs.back().raynum;
+            stage_done = true;
+        }
 
         while (!stage_done) {
             // ---- Label_StartRayLoop ----
@@ -337,6 +366,22 @@ static bool trace(System &sys, uint32_t seed, uint64_t N, uint64_t maxrays, bool
         }
 
         // ---- Label_EndStageLoop ----
+        if (sys.save_data && si == 0) {
+            // every stage-1 record (coretrace/raytrace.cpp:1044-1063)
+            for (const Record &r : st.recs)
+                sys.saved_s1.insert(sys.saved_s1.end(), {r.pos[0], r.pos[1], r.pos[2], r.cos[0], r.cos[1], r.cos[2],
+                                                         (double)r.element, (double)r.raynum});
+        }
+        if (sys.save_data && si == 1) {
+            // the reference's one ray array as stage 2 leaves it
+            // (coretrace/raytrace.cpp:1065-1078): the first slots hold the
+            // rays written out of stage 2, the rest still the rays read in
+            for (size_t k = 0; k < read; k++) {
+                const GlobalRay &g = k < outgoing.size() ? outgoing[k] : incoming[k];
+                sys.saved_s2.insert(sys.saved_s2.end(),
+                                    {g.pos[0], g.pos[1], g.pos[2], g.cos[0], g.cos[1], g.cos[2], (double)g.num});
+            }
+        }
         if (!prev_has_rays) {
             last_prev = 0;
             continue;
@@ -344,7 +389,9 @@ static bool trace(System &sys, uint32_t seed, uint64_t N, uint64_t maxrays, bool
         if (prev_idx < (int64_t)outgoing.size()) {
             last_prev = outgoing[prev_idx].num;
             if (last_prev == 0) {
-                sys.errlog("LastRayNumberInPreviousStage=0, stage %d", (int)si + 1);
+                const uint32_t pp = prev_idx > 0 ? outgoing[prev_idx - 1].num : 0;
+                sys.errlog("LastRayNumberInPreviousStage=0, stage %d, PrevIdx=%d, CurIdx=%d, pp=%d", (int)si + 1,
+                           (int)prev_idx, (int)read, (int)pp);
                 return false;
             }
         } else {
@@ -631,7 +678,33 @@ int st_sim_params(st_context_t p, int raycount, int maxcount, int dyn) {
 int st_sim_errors(st_context_t p, int sun, int optics) {
     SYS(p, -1); sys->err_sunshape = sun != 0; sys->err_optical = optics != 0; return 1;
 }
-// coretrace/stapi.cpp:680-734
+int st_sim_run(st_context_t p, unsigned int seed, progress_cb cb, void *data);
+// coretrace/stapi.cpp:680-728, with plain arrays for the row vectors
+int st_amd_sim_run_data(st_context_t p, unsigned int seed, const double *s1, st_uint_t n1, const double *s2,
+                        st_uint_t n2, int save, progress_cb cb, void *data) {
+    SYS(p, -1);
+    if ((n1 && !s1) || (n2 && !s2)) return -1;
+    sys->load_s1 = s1;
+    sys->load_n1 = n1;
+    sys->load_s2 = s2;
+    sys->load_n2 = n2;
+    sys->save_data = save != 0;
+    sys->saved_s1.clear();
+    sys->saved_s2.clear();
+    const int r = st_sim_run(p, seed, cb, data);
+    sys->load_s1 = sys->load_s2 = nullptr;
+    sys->load_n1 = sys->load_n2 = 0;
+    sys->save_data = false;
+    return r;
+}
+int st_amd_saved_stage_data(st_context_t p, int which, double *out) {
+    SYS(p, -1);
+    if (which != 1 && which != 2) return -1;
+    const std::vector<double> &v = which == 1 ? sys->saved_s1 : sys->saved_s2;
+    if (out) std::copy(v.begin(), v.end(), out);
+    return (int)(v.size() / (which == 1 ? 8 : 7));
+}
+// coretrace/stapi.cpp:680-734 (st_sim_run is st_sim_run_data without rows)
 int st_sim_run(st_context_t p, unsigned int seed, progress_cb cb, void *data) {
     SYS(p, -1);
     sys->all.clear();
@@ -644,7 +717,10 @@ int st_sim_run(st_context_t p, unsigned int seed, progress_cb cb, void *data) {
         }
     bool cancelled = false;
     try {
-        if (!trace(*sys, seed, (uint64_t)(int64_t)sys->raycount, (uint64_t)(int64_t)sys->raymax, sys->err_sunshape,
+        // the ray count is the number of loaded stage-2 rays when there are
+        // any (coretrace/stapi.cpp:697-700)
+        const int rayct = sys->load_n2 > 0 ? (int)sys->load_n2 : sys->raycount;
+        if (!trace(*sys, seed, (uint64_t)(int64_t)rayct, (uint64_t)(int64_t)sys->raymax, sys->err_sunshape,
                    sys->err_optical, cb, data, &cancelled))
             return -1;
     } catch (const FeFailure &) {

@@ -124,6 +124,12 @@ struct System {
     bool torus_start = false;          // start torus solves at the bounding sphere
     std::vector<uint64_t> ray_sun_index;  // raynum-1 -> 0-based sun ray index
     uint64_t draws_total = 0;
+    // stage-data save/resume (coretrace/stapi.cpp:680-728): rows to load in
+    // the next run (8 and 7 doubles a row), and the rows the last run saved
+    const double *load_s1 = nullptr, *load_s2 = nullptr;
+    uint64_t load_n1 = 0, load_n2 = 0;
+    bool save_data = false;
+    std::vector<double> saved_s1, saved_s2;
 
     ~System() {
         for (Stage *s : stages) delete s;

@@ -124,6 +124,13 @@ struct Context {
     std::vector<uint32_t> flux_host;
     Timing timing;
 
+    // stage-data save/resume (coretrace/stapi.cpp:680-728): rows to load in
+    // the next run, and the rows the last run saved
+    const double *load_s1 = nullptr, *load_s2 = nullptr;
+    uint64_t load_n1 = 0, load_n2 = 0;
+    bool save_data = false;
+    std::vector<double> saved_s1, saved_s2;  // 8 and 7 doubles a row
+
     Context() {
         HC(hipGetDevice(&device));
         HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -230,10 +237,16 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         errlog("%s", msg.c_str());
         return -1;
     }
-    const uint64_t N = (uint64_t)(int64_t)raycount;
+    // rays to trace: the number of loaded stage-2 rays when there are any
+    // (coretrace/stapi.cpp:697-700)
+    const int rayct = load_n2 > 0 ? (int)load_n2 : raycount;
+    const bool load = load_n1 > 0 && load_n2 > 0;  // coretrace/raytrace.cpp:159-163
+    saved_s1.clear();
+    saved_s2.clear();
+    const uint64_t N = (uint64_t)(int64_t)rayct;
     const uint64_t maxrays = (uint64_t)(int64_t)raymax;
-    if ((int64_t)raycount < 1) {
-        errlog("invalid number of rays: %d", raycount);
+    if ((int64_t)rayct < 1) {
+        errlog("invalid number of rays: %d", rayct);
         return -1;
     }
     if (model.stages.empty()) {
@@ -489,157 +502,209 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     // The mask side has its own scan workspace and totals; the sunshape
     // angles alternate between two buffers.
     DevRay *next = (DevRay *)d_rays_a.ensure((size_t)N * sizeof(DevRay));
-    uint64_t *sun_index_all = (uint64_t *)d_hits.ensure((size_t)N * sizeof(uint64_t));
-    const uint64_t kMaxBatch = 1u << 25;
-    const size_t max_words = (size_t)kMaxBatch / 64;
-    uint64_t *mask = (uint64_t *)d_mask.ensure(max_words * sizeof(uint64_t));
-    uint32_t *popc = (uint32_t *)d_popc.ensure(max_words * sizeof(uint32_t));
-    uint32_t *offs = (uint32_t *)d_offs.ensure(max_words * sizeof(uint32_t));
-    uint32_t *mscan = (uint32_t *)d_mask_scan_tmp.ensure(scan_tmp_words(max_words) * sizeof(uint32_t));
-    uint32_t *mtotal = (uint32_t *)d_mask_total.ensure(4 * sizeof(uint32_t));
-    hipStream_t sm = mask_stream;
-    HC(hipEventRecord(ev_ready, s));  // counters and flags are cleared
-    HC(hipStreamWaitEvent(sm, ev_ready, 0));
     uint64_t n_next = 0;
-    uint64_t hits = 0, s0 = 0, batch = 0;
-    int parity = 0;
-    double rate = 0.0;  // hits per sun ray observed so far
-    auto plan = [&](uint64_t want) -> uint64_t {
-        uint64_t b = (rate > 0) ? (uint64_t)(want / rate * 1.05) + 4096 : std::min<uint64_t>(1u << 22, want * 4 + 4096);
-        b = std::min<uint64_t>(b, kMaxBatch);
-        b = std::max<uint64_t>(b, 4096);
-        if (s0 + b > maxrays) b = maxrays - s0;
-        return b;
-    };
-    auto launch_mask_batch = [&]() {
-        const size_t nwords = (batch + 63) / 64;
-        MaskArgs ma{};
-        ma.sv = sv;
-        ma.seed = seed;
-        ma.s0 = s0;
-        ma.n = (uint32_t)batch;
-        ma.mask = mask;
-        ma.fe_fail = fe_fail;
-        ma.counters = ctrs;
-        ma.generic = cs.stages[0].generic;
-        // pairs pack the element index in 24 bits
-        ma.compact = sv.use_grid && !sv.sun.point_source && cs.stages[0].nelem < (1 << 24);
-        timing.mask_launches++;
-        timed_on(sm, timing.mask_ms, [&]() { launch_sun_mask<R>(ma, sm); });
-        timing.sun_rays += batch;
-        launch_mask_popc(mask, popc, nwords, sm);
-        launch_exclusive_scan(popc, offs, nwords, mscan, mtotal, sm);
-    };
-    struct Ready {
-        uint64_t *hit_list;
-        uint32_t take;
-        uint64_t raynum_base;
-        SunPre *pre;
-        hipEvent_t ev;
-        bool more;           // another batch was launched behind this one
-        uint64_t sun_count;  // sun rays drawn up to this batch's last hit (or its end)
-    };
-    bool maxed = false;
-    // Read the pending mask batch's hit count, compact its hit list (and
-    // sample its sunshape angles), and launch the next mask batch if hits
-    // are still missing.
-    auto prepare = [&]() -> Ready {
-        Ready r{};
-        const uint64_t want = N - hits;
-        const size_t nwords = (batch + 63) / 64;
-        uint32_t h = 0;
-        HC(hipMemcpyAsync(&h, mtotal, sizeof h, hipMemcpyDeviceToHost, sm));
-        HC(hipStreamSynchronize(sm));
-        r.take = (uint32_t)std::min<uint64_t>(h, want);
-        r.hit_list = sun_index_all + hits;
-        r.raynum_base = hits;
-        if (r.take) launch_mask_compact(mask, offs, nwords, s0, r.take, r.hit_list, sm);
-        r.pre = nullptr;
-        if (presample && r.take) {
-            r.pre = (SunPre *)d_sunpre_batch[parity].ensure((size_t)r.take * sizeof(SunPre));
-            launch_presample(sm, r.hit_list, (int)r.take, r.pre);
-        }
-        if (r.take == want) {
-            uint64_t last = 0;
-            HC(hipMemcpyAsync(&last, r.hit_list + r.take - 1, sizeof last, hipMemcpyDeviceToHost, sm));
+    if (!load) {
+        uint64_t *sun_index_all = (uint64_t *)d_hits.ensure((size_t)N * sizeof(uint64_t));
+        const uint64_t kMaxBatch = 1u << 25;
+        const size_t max_words = (size_t)kMaxBatch / 64;
+        uint64_t *mask = (uint64_t *)d_mask.ensure(max_words * sizeof(uint64_t));
+        uint32_t *popc = (uint32_t *)d_popc.ensure(max_words * sizeof(uint32_t));
+        uint32_t *offs = (uint32_t *)d_offs.ensure(max_words * sizeof(uint32_t));
+        uint32_t *mscan = (uint32_t *)d_mask_scan_tmp.ensure(scan_tmp_words(max_words) * sizeof(uint32_t));
+        uint32_t *mtotal = (uint32_t *)d_mask_total.ensure(4 * sizeof(uint32_t));
+        hipStream_t sm = mask_stream;
+        HC(hipEventRecord(ev_ready, s));  // counters and flags are cleared
+        HC(hipStreamWaitEvent(sm, ev_ready, 0));
+        uint64_t hits = 0, s0 = 0, batch = 0;
+        int parity = 0;
+        double rate = 0.0;  // hits per sun ray observed so far
+        auto plan = [&](uint64_t want) -> uint64_t {
+            uint64_t b = (rate > 0) ? (uint64_t)(want / rate * 1.05) + 4096
+                                    : std::min<uint64_t>(1u << 22, want * 4 + 4096);
+            b = std::min<uint64_t>(b, kMaxBatch);
+            b = std::max<uint64_t>(b, 4096);
+            if (s0 + b > maxrays) b = maxrays - s0;
+            return b;
+        };
+        auto launch_mask_batch = [&]() {
+            const size_t nwords = (batch + 63) / 64;
+            MaskArgs ma{};
+            ma.sv = sv;
+            ma.seed = seed;
+            ma.s0 = s0;
+            ma.n = (uint32_t)batch;
+            ma.mask = mask;
+            ma.fe_fail = fe_fail;
+            ma.counters = ctrs;
+            ma.generic = cs.stages[0].generic;
+            // pairs pack the element index in 24 bits
+            ma.compact = sv.use_grid && !sv.sun.point_source && cs.stages[0].nelem < (1 << 24);
+            timing.mask_launches++;
+            timed_on(sm, timing.mask_ms, [&]() { launch_sun_mask<R>(ma, sm); });
+            timing.sun_rays += batch;
+            launch_mask_popc(mask, popc, nwords, sm);
+            launch_exclusive_scan(popc, offs, nwords, mscan, mtotal, sm);
+        };
+        struct Ready {
+            uint64_t *hit_list;
+            uint32_t take;
+            uint64_t raynum_base;
+            SunPre *pre;
+            hipEvent_t ev;
+            bool more;           // another batch was launched behind this one
+            uint64_t sun_count;  // sun rays drawn up to this batch's last hit (or its end)
+        };
+        bool maxed = false;
+        // Read the pending mask batch's hit count, compact its hit list (and
+        // sample its sunshape angles), and launch the next mask batch if hits
+        // are still missing.
+        auto prepare = [&]() -> Ready {
+            Ready r{};
+            const uint64_t want = N - hits;
+            const size_t nwords = (batch + 63) / 64;
+            uint32_t h = 0;
+            HC(hipMemcpyAsync(&h, mtotal, sizeof h, hipMemcpyDeviceToHost, sm));
             HC(hipStreamSynchronize(sm));
-            r.sun_count = last + 1;
-        } else {
-            r.sun_count = s0 + batch;
-        }
-        r.ev = ev_hits[parity];
-        HC(hipEventRecord(r.ev, sm));
-        parity ^= 1;
-        hits += r.take;
-        s0 += batch;
-        rate = (double)hits / (double)s0;
-        r.more = false;
-        if (hits < N) {
-            if (s0 >= maxrays) {
-                maxed = true;
-            } else {
-                batch = plan(N - hits);
-                launch_mask_batch();
-                r.more = true;
+            r.take = (uint32_t)std::min<uint64_t>(h, want);
+            r.hit_list = sun_index_all + hits;
+            r.raynum_base = hits;
+            if (r.take) launch_mask_compact(mask, offs, nwords, s0, r.take, r.hit_list, sm);
+            r.pre = nullptr;
+            if (presample && r.take) {
+                r.pre = (SunPre *)d_sunpre_batch[parity].ensure((size_t)r.take * sizeof(SunPre));
+                launch_presample(sm, r.hit_list, (int)r.take, r.pre);
             }
+            if (r.take == want) {
+                uint64_t last = 0;
+                HC(hipMemcpyAsync(&last, r.hit_list + r.take - 1, sizeof last, hipMemcpyDeviceToHost, sm));
+                HC(hipStreamSynchronize(sm));
+                r.sun_count = last + 1;
+            } else {
+                r.sun_count = s0 + batch;
+            }
+            r.ev = ev_hits[parity];
+            HC(hipEventRecord(r.ev, sm));
+            parity ^= 1;
+            hits += r.take;
+            s0 += batch;
+            rate = (double)hits / (double)s0;
+            r.more = false;
+            if (hits < N) {
+                if (s0 >= maxrays) {
+                    maxed = true;
+                } else {
+                    batch = plan(N - hits);
+                    launch_mask_batch();
+                    r.more = true;
+                }
+            }
+            return r;
+        };
+        batch = plan(N);
+        if (batch == 0) {
+            errlog("generated sun rays reached maximum count: %d", raymax);
+            return -1;
         }
-        return r;
-    };
-    batch = plan(N);
-    if (batch == 0) {
-        errlog("generated sun rays reached maximum count: %d", raymax);
-        return -1;
-    }
-    launch_mask_batch();
-    Ready cur_b = prepare();
-    for (;;) {
-        HC(hipStreamWaitEvent(s, cur_b.ev, 0));
-        bool ok = true;
-        Ready next_b{};
-        bool have_next = false;
-        for (uint32_t c = 0; c < cur_b.take && ok; c += kMaxChunk) {
-            const int n = (int)std::min<uint32_t>(kMaxChunk, cur_b.take - c);
-            Pending p = trace_launch(0, n, cur_b.hit_list + c, (uint32_t)(cur_b.raynum_base + c), nullptr, 0, next,
-                                     &n_next, cur_b.pre != nullptr ? cur_b.pre + c : nullptr);
-            if (c == 0 && cur_b.more) {
-                next_b = prepare();  // overlaps the trace just queued
+        launch_mask_batch();
+        Ready cur_b = prepare();
+        for (;;) {
+            HC(hipStreamWaitEvent(s, cur_b.ev, 0));
+            bool ok = true;
+            Ready next_b{};
+            bool have_next = false;
+            for (uint32_t c = 0; c < cur_b.take && ok; c += kMaxChunk) {
+                const int n = (int)std::min<uint32_t>(kMaxChunk, cur_b.take - c);
+                Pending p = trace_launch(0, n, cur_b.hit_list + c, (uint32_t)(cur_b.raynum_base + c), nullptr, 0, next,
+                                         &n_next, cur_b.pre != nullptr ? cur_b.pre + c : nullptr);
+                if (c == 0 && cur_b.more) {
+                    next_b = prepare();  // overlaps the trace just queued
+                    have_next = true;
+                }
+                ok = trace_finish(p);
+            }
+            if (!ok) {
+                HC(hipStreamSynchronize(sm));
+                return -1;
+            }
+            n_hits_done = (uint64_t)cur_b.raynum_base + cur_b.take;
+            sun_ray_count = cur_b.sun_count;
+            if (cb && !cb((st_uint_t)traced_total, (st_uint_t)n_hits_done, (st_uint_t)N, 1, (st_uint_t)nst, cbdata)) {
+                cancelled = true;
+                break;
+            }
+            if (!have_next && cur_b.more) {  // an empty batch: nothing was traced
+                next_b = prepare();
                 have_next = true;
             }
-            ok = trace_finish(p);
+            if (!have_next) break;
+            cur_b = next_b;
         }
-        if (!ok) {
-            HC(hipStreamSynchronize(sm));
-            return -1;
-        }
-        n_hits_done = (uint64_t)cur_b.raynum_base + cur_b.take;
-        sun_ray_count = cur_b.sun_count;
-        if (cb && !cb((st_uint_t)traced_total, (st_uint_t)n_hits_done, (st_uint_t)N, 1, (st_uint_t)nst, cbdata)) {
-            cancelled = true;
-            break;
-        }
-        if (!have_next && cur_b.more) {  // an empty batch: nothing was traced
-            next_b = prepare();
-            have_next = true;
-        }
-        if (!have_next) break;
-        cur_b = next_b;
-    }
-    HC(hipStreamSynchronize(sm));
-    if (cs.stages[0].generic == 2) {
-        // hit-mask launches run ahead of the trace: a failed interpolation
-        // counts when the reference reaches its sun ray, i.e. before the
-        // last sun ray used
-        unsigned long long f = 0;
-        HC(hipMemcpyAsync(&f, fe_fail, sizeof f, hipMemcpyDeviceToHost, sm));
         HC(hipStreamSynchronize(sm));
-        if (f < (unsigned long long)sun_ray_count) {
-            check_err(16u);
+        if (cs.stages[0].generic == 2) {
+            // hit-mask launches run ahead of the trace: a failed interpolation
+            // counts when the reference reaches its sun ray, i.e. before the
+            // last sun ray used
+            unsigned long long f = 0;
+            HC(hipMemcpyAsync(&f, fe_fail, sizeof f, hipMemcpyDeviceToHost, sm));
+            HC(hipStreamSynchronize(sm));
+            if (f < (unsigned long long)sun_ray_count) {
+                check_err(16u);
+                return -1;
+            }
+        }
+        if (maxed && !cancelled) {
+            errlog("generated sun rays reached maximum count: %d", raymax);
             return -1;
         }
+    } else {
+        // stage 1 from saved data (coretrace/raytrace.cpp:544-578,
+        // generateray.cpp:165-219): its records are the rows of s1, stage 2
+        // starts from the rays of s2; the row fields convert as the
+        // reference's casts do (element: double -> st_uint_t -> int)
+        std::vector<DevRecord> r0(load_n1);
+        for (uint64_t j = 0; j < load_n1; j++) {
+            const double *v = load_s1 + 8 * j;
+            DevRecord &r = r0[j];
+            for (int k = 0; k < 3; k++) { r.pos[k] = v[k]; r.cos[k] = v[k + 3]; }
+            r.element = (int32_t)(int64_t)v[6];
+            r.raynum = (uint32_t)(uint64_t)(int64_t)v[7];
+        }
+        std::vector<DevRay> in(load_n2);
+        for (uint64_t j = 0; j < load_n2; j++) {
+            const double *v = load_s2 + 7 * j;
+            DevRay &r = in[j];
+            for (int k = 0; k < 3; k++) { r.pos[k] = v[k]; r.cos[k] = v[k + 3]; }
+            r.raynum = (uint32_t)(int32_t)v[6];
+            r.pad = 0;
+            r.sun_index = (uint64_t)r.raynum - 1;  // the draws of a loaded ray
+        }
+        std::shared_ptr<DevBuf> chunk = take_buffer(r0.size() * sizeof(DevRecord));
+        HC(hipMemcpyAsync(chunk->p, r0.data(), r0.size() * sizeof(DevRecord), hipMemcpyHostToDevice, s));
+        HC(hipMemcpyAsync(next, in.data(), in.size() * sizeof(DevRay), hipMemcpyHostToDevice, s));
+        HC(hipStreamSynchronize(s));
+        recs[0].chunks.push_back(chunk);
+        recs[0].counts.push_back(r0.size());
+        recs[0].total += r0.size();
+        n_next = load_n2;
+        sun_ray_count = r0.back().raynum;
+        n_hits_done = N;
     }
-    if (maxed && !cancelled) {
-        errlog("generated sun rays reached maximum count: %d", raymax);
-        return -1;
+    if (save_data && !cancelled) {
+        // every stage-1 record in trace order (coretrace/raytrace.cpp:1044-1063)
+        std::vector<DevRecord> r0(recs[0].total);
+        size_t off = 0;
+        for (size_t c = 0; c < recs[0].chunks.size(); c++) {
+            if (recs[0].counts[c])
+                HC(hipMemcpyAsync(r0.data() + off, recs[0].chunks[c]->p, recs[0].counts[c] * sizeof(DevRecord),
+                                  hipMemcpyDeviceToHost, s));
+            off += recs[0].counts[c];
+        }
+        HC(hipStreamSynchronize(s));
+        saved_s1.reserve(r0.size() * 8);
+        for (const DevRecord &r : r0) {
+            saved_s1.insert(saved_s1.end(), {r.pos[0], r.pos[1], r.pos[2], r.cos[0], r.cos[1], r.cos[2],
+                                             (double)r.element, (double)r.raynum});
+        }
     }
 
     // stage-0 pass-on rule: the last ray's pass counts only when an earlier
@@ -654,7 +719,16 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     uint64_t n_cur = n_next;
     bool prev_has = false;
     uint32_t last_prev = 0;
-    if (!cancelled && n_cur > 0) {
+    if (load) {
+        prev_has = true;
+        last_prev = last_raynum_of(cur, n_cur);
+        if (last_prev == 0) {  // coretrace/raytrace.cpp:1087-1096
+            const uint32_t pp = n_cur > 1 ? last_raynum_of(cur, n_cur - 1) : 0;
+            errlog("LastRayNumberInPreviousStage=0, stage %d, PrevIdx=%d, CurIdx=%d, pp=%d", 1, (int)(n_cur - 1), 0,
+                   (int)pp);
+            return -1;
+        }
+    } else if (!cancelled && n_cur > 0) {
         last_prev = last_raynum_of(cur, n_cur);
         uint64_t below = (last_prev == N) ? n_cur - 1 : n_cur;
         prev_has = below > 0;
@@ -674,6 +748,23 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
                 cancelled = true;
                 break;
             }
+        }
+        if (si == 1 && save_data && !cancelled) {
+            // the reference's stage-2 ray array as stage 2 leaves it
+            // (coretrace/raytrace.cpp:1065-1078): the array is reused in
+            // place, so its first n_out rows are the rays leaving stage 2 and
+            // the rest the rays that entered it
+            std::vector<DevRay> rows(n_cur);
+            if (n_out)
+                HC(hipMemcpyAsync(rows.data(), nxt, n_out * sizeof(DevRay), hipMemcpyDeviceToHost, s));
+            if (n_cur > n_out)
+                HC(hipMemcpyAsync(rows.data() + n_out, cur + n_out, (n_cur - n_out) * sizeof(DevRay),
+                                  hipMemcpyDeviceToHost, s));
+            HC(hipStreamSynchronize(s));
+            saved_s2.reserve(rows.size() * 7);
+            for (const DevRay &r : rows)
+                saved_s2.insert(saved_s2.end(),
+                                {r.pos[0], r.pos[1], r.pos[2], r.cos[0], r.cos[1], r.cos[2], (double)r.raynum});
         }
         which ^= 1;
         cur = nxt;
@@ -1113,6 +1204,64 @@ STCORE_API int st_sim_errors(st_context_t p, int sun, int optics) {
 STCORE_API int st_sim_run(st_context_t p, unsigned int seed, st_callback_t cb, void *data) {
     CTX(p, -1);
     return c->run(seed, cb, data);
+}
+
+// coretrace/stapi.cpp:680-728 with plain arrays
+STCORE_API int st_amd_sim_run_data(st_context_t p, unsigned int seed, const double *s1, st_uint_t n1, const double *s2,
+                                   st_uint_t n2, int save_stage_data, st_callback_t cb, void *data) {
+    CTX(p, -1);
+    if ((n1 && !s1) || (n2 && !s2)) return -1;
+    c->load_s1 = s1;
+    c->load_n1 = n1;
+    c->load_s2 = s2;
+    c->load_n2 = n2;
+    c->save_data = save_stage_data != 0;
+    const int r = c->run(seed, cb, data);
+    c->load_s1 = c->load_s2 = nullptr;
+    c->load_n1 = c->load_n2 = 0;
+    c->save_data = false;
+    return r;
+}
+STCORE_API int st_amd_saved_stage_data(st_context_t p, int which, double *out) {
+    CTX(p, -1);
+    if (which != 1 && which != 2) return -1;
+    const std::vector<double> &v = which == 1 ? c->saved_s1 : c->saved_s2;
+    if (out) std::copy(v.begin(), v.end(), out);
+    return (int)(v.size() / (which == 1 ? 8 : 7));
+}
+
+// The reference's own entry point, with C++ linkage as there
+// (coretrace/stapi.cpp:680-685; its stapi.h declaration is commented out).
+// Rows are appended to the vectors, as the reference appends them.  A loaded
+// row of the wrong length keeps the previous row's values, as the reference's
+// loaders leave their outputs untouched then (generateray.cpp:165-219).
+__attribute__((visibility("default"))) int st_sim_run_data(st_context_t p, unsigned int seed,
+                                                           std::vector<std::vector<double>> *data_s1,
+                                                           std::vector<std::vector<double>> *data_s2,
+                                                           bool save_st_data, st_callback_t cb, void *cbdata) {
+    auto flat = [](const std::vector<std::vector<double>> *rows, size_t w) {
+        std::vector<double> f;
+        if (!rows) return f;
+        std::vector<double> prev(w, 0.0);
+        for (const auto &r : *rows) {
+            if (r.size() == w) prev.assign(r.begin(), r.end());
+            f.insert(f.end(), prev.begin(), prev.end());
+        }
+        return f;
+    };
+    const std::vector<double> f1 = flat(data_s1, 8), f2 = flat(data_s2, 7);
+    const int n = st_amd_sim_run_data(p, seed, f1.data(), f1.size() / 8, f2.data(), f2.size() / 7,
+                                      save_st_data ? 1 : 0, cb, cbdata);
+    Context *c = ctx_of(p);
+    if (n >= 0 && c && save_st_data) {
+        if (data_s1)
+            for (size_t i = 0; i < c->saved_s1.size(); i += 8)
+                data_s1->emplace_back(c->saved_s1.begin() + i, c->saved_s1.begin() + i + 8);
+        if (data_s2)
+            for (size_t i = 0; i < c->saved_s2.size(); i += 7)
+                data_s2->emplace_back(c->saved_s2.begin() + i, c->saved_s2.begin() + i + 7);
+    }
+    return n;
 }
 
 // ------------------------------------------------------------ utilities ---

@@ -177,3 +177,38 @@ def messages(lib: StLib, ctx) -> list[str]:
 def run(lib: StLib, ctx, seed: int, callback=None) -> int:
     cb = _NOCB if callback is None else CALLBACK(callback)
     return lib.st_sim_run(ctx, seed, cb, None)
+
+
+def run_data(lib: StLib, ctx, seed: int, s1=None, s2=None, save: bool = False, api=None, callback=None) -> int:
+    """st_amd_sim_run_data: a run that loads stage-1 records (rows of 8) and
+    stage-2 rays (rows of 7), and/or saves them (the reference's
+    st_sim_run_data, coretrace/stapi.cpp:680-728).  `api` is the library
+    exporting the call when it is not `lib` itself."""
+    d = api if api is not None else lib.dll
+    f = d.st_amd_sim_run_data
+    f.restype = C.c_int
+    f.argtypes = [st_context_t, C.c_uint, C.POINTER(C.c_double), st_uint_t, C.POINTER(C.c_double), st_uint_t,
+                  C.c_int, CALLBACK, C.c_void_p]
+    a1 = np.ascontiguousarray(np.zeros((0, 8)) if s1 is None else s1, dtype=np.float64).reshape(-1, 8)
+    a2 = np.ascontiguousarray(np.zeros((0, 7)) if s2 is None else s2, dtype=np.float64).reshape(-1, 7)
+    cb = _NOCB if callback is None else CALLBACK(callback)
+    return f(ctx, seed, _dp(a1), a1.shape[0], _dp(a2), a2.shape[0], int(save), cb, None)
+
+
+def saved_stage_data(lib: StLib, ctx, api=None):
+    """Rows the last run_data(save=True) saved: (stage-1 records (n, 8),
+    stage-2 ray array (m, 7))."""
+    d = api if api is not None else lib.dll
+    f = d.st_amd_saved_stage_data
+    f.restype = C.c_int
+    f.argtypes = [st_context_t, C.c_int, C.POINTER(C.c_double)]
+    out = []
+    for which, w in ((1, 8), (2, 7)):
+        n = f(ctx, which, None)
+        if n < 0:
+            raise RuntimeError("st_amd_saved_stage_data failed")
+        a = np.zeros((n, w))
+        if n:
+            f(ctx, which, _dp(a))
+        out.append(a)
+    return tuple(out)

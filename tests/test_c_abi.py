@@ -26,9 +26,13 @@ def declared(path):
 
 
 def defined_symbols(path, dynamic):
+    """Defined function names: C symbols as they are, C++ ones by name."""
     args = ["nm", "--defined-only"] + (["-D"] if dynamic else []) + [path]
     out = subprocess.run(args, capture_output=True, text=True, check=True).stdout
-    return {line.split()[-1] for line in out.splitlines() if re.search(r" T ", line)}
+    names = {line.split()[-1] for line in out.splitlines() if re.search(r" T ", line)}
+    out = subprocess.run(args[:1] + ["-C"] + args[1:], capture_output=True, text=True, check=True).stdout
+    names |= set(re.findall(r" T (st_\w+)\(", out))
+    return names
 
 
 STAPI = declared(os.path.join(ROOT, "include", "stapi.h"))
@@ -58,10 +62,40 @@ def test_static_library_exports_every_declaration():
     assert not missing, sorted(missing)
 
 
+# st_sim_run_data has C++ linkage, as in the reference (coretrace/stapi.cpp:680)
+CXX_ONLY = {"st_sim_run_data"}
+
+
 def test_shared_library_loads_and_resolves():
     lib = ctypes.CDLL(SO)
-    for name in sorted(STAPI | EXT):
+    for name in sorted((STAPI | EXT) - CXX_ONLY):
         assert getattr(lib, name) is not None
+
+
+CXX_APP = r"""
+#include <vector>
+#include "stapi.h"
+#include "stapi_amd.h"
+int main() {
+    st_context_t c = st_create_context();
+    std::vector<std::vector<double> > s1, s2;
+    int n = st_sim_run_data(c, 1, &s1, &s2, true, 0, 0);
+    st_free_context(c);
+    return n < 0;
+}
+"""
+
+
+def test_cpp_caller_of_st_sim_run_data_links(tmp_path):
+    # a C++ caller written against the reference's own prototype links
+    # unchanged (linking only: running needs a GPU)
+    import shutil
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src, exe = tmp_path / "app.cpp", tmp_path / "app"
+    src.write_text(CXX_APP)
+    subprocess.run(["g++", "-I", os.path.join(ROOT, "include"), str(src), SO, "-o", str(exe)], check=True)
+    assert exe.exists()
 
 
 APP = r"""
