@@ -131,6 +131,8 @@ void launch_mask_compact(const uint64_t *mask, const uint32_t *offsets, size_t n
                        nwords, s0, limit, hits);
 }
 
+// Slots are slot-major (record j of ray t at slots[j * n_rays + t]), so the
+// first records of a wave's rays, most of the data, are read contiguously.
 __global__ void compact_records_kernel(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets,
                                        int n_rays, int nslots, DevRecord *out) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -138,7 +140,7 @@ __global__ void compact_records_kernel(const DevRecord *slots, const uint32_t *c
     uint32_t c = count[t];
     uint32_t m = c < (uint32_t)nslots ? c : (uint32_t)nslots;
     uint32_t o = offsets[t];
-    for (uint32_t j = 0; j < m; ++j) out[o + j] = slots[(size_t)t * nslots + j];
+    for (uint32_t j = 0; j < m; ++j) out[o + j] = slots[(size_t)j * n_rays + t];
 }
 void launch_compact_records(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets, int n_rays,
                             int nslots, DevRecord *out, hipStream_t s) {

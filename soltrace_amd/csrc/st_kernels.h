@@ -62,7 +62,7 @@ struct TraceArgs {
     const DevRay *in_rays;
     uint32_t last_raynum;        // LastRayNumberInPreviousStage
     // outputs
-    DevRecord *rec_slots;        // n_rays * slots
+    DevRecord *rec_slots;        // slots * n_rays, slot-major: record j of ray t at [j * n_rays + t]
     uint32_t *rec_count;         // n_rays
     DevRecord *ovf_rec;          // overflow records (seq >= slots)
     uint32_t *ovf_meta;          // (ray, seq) pairs

@@ -370,9 +370,11 @@ __global__ __launch_bounds__(kBlock) void sun_mask_grid_kernel(MaskArgs a) {
 // Buie aureole, and a lane-per-ray loop runs a whole wave until its slowest
 // lane is done.  Here the 64 lanes of a wave share the wave's unresolved
 // rays: each round, lane l takes ray (l mod m) of the m unresolved ones and
-// iteration next_j + l/m of that ray's loop; each ray keeps the smallest
-// accepted iteration.  The result is exactly what the serial loop of
-// coretrace/errors.cpp:160-185 returns on the same stream.
+// groups next_g + 2(l/m) and next_g + 2(l/m) + 1 of that ray's loop (eight
+// tries: the round's shuffles and ballots are paid once per eight); each
+// ray keeps the smallest accepted iteration.  The result is exactly what
+// the serial loop of coretrace/errors.cpp:160-185 returns on the same
+// stream.
 // Position of set bit number n (from 0) of v, which has more than n set
 // bits: six halving steps on population counts.
 STD int nth_set_bit(uint64_t v, int n) {
@@ -405,11 +407,16 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
         const int m = __popcll(U);
         const int rank = lane % m, rep = lane / m;
         const int rl = nth_set_bit(U, rank);
-        const uint32_t g = (uint32_t)__shfl((int)next_g, rl) + (uint32_t)rep;
+        const uint32_t g = (uint32_t)__shfl((int)next_g, rl) + 2u * (uint32_t)rep;
         const uint64_t s = ((uint64_t)(uint32_t)__shfl((int)sid_hi, rl) << 32) | (uint32_t)__shfl((int)sid_lo, rl);
         double t2 = 0.0;
-        const int off = sunshape_d_group(a.seed, s, a.k0, g, a.sun, a.aux, &t2);
+        int off = sunshape_d_group(a.seed, s, a.k0, g, a.sun, a.aux, &t2);
         ev_eval += 4u;
+        if (off < 0) {
+            off = sunshape_d_group(a.seed, s, a.k0, g + 1u, a.sun, a.aux, &t2);
+            if (off >= 0) off += 4;
+            ev_eval += 4u;
+        }
         const bool acc = off >= 0;
         const uint64_t accm = __ballot(acc);
         // lanes that served this lane's ray: rank r_own, r_own + m, ...
@@ -425,7 +432,7 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
         bool resolved = false;
         if (owner) {
             if (hit) {
-                const uint32_t gg = next_g + (uint32_t)((first - r_own) / m);
+                const uint32_t gg = next_g + 2u * (uint32_t)((first - r_own) / m);
                 const uint32_t jj = 4u * gg + (uint32_t)off_first - sunshape_d_shift(a.k0);
                 SunPre p;
                 p.theta2 = t2_first / 1000000.0;
@@ -435,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
                 resolved = true;
                 ev_need += jj + 1u;
             } else {
-                next_g += (uint32_t)__popcll(mine);
+                next_g += 2u * (uint32_t)__popcll(mine);
                 if (next_g > (1u << 24)) {  // no acceptance in 2^26 tries: give up on this ray
                     SunPre p;
                     p.theta2 = 0.0;
@@ -500,9 +507,10 @@ STD int flux_bin(const FluxSpec &fl, const double pos[3], const double cos[3]) {
 
 template <typename R, int G>
 // Waves per SIMD: the generic variant needs the whole register file; the
-// basic FP32 variant runs 3 (a few spills are cheaper than the latency they
-// hide), the basic FP64 variant 2.
-__global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trace_stage_kernel(TraceArgs a) {
+// basic FP32 variant runs 4 (128 registers: its spills cost less than the
+// latency the fourth wave hides; 5 waves spill too much), the basic FP64
+// variant 2 (3 is slower).
+__global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 4 : 2)) void trace_stage_kernel(TraceArgs a) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= a.n_rays) return;
     const SceneView &sv = a.sv;
@@ -543,14 +551,14 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 3 : 2)) void trac
     bool passed = false;
     // records go out as they are made; an absorbed hit later rewrites the
     // element field of its own record (slot index, or -1 - overflow index)
-    int32_t last_loc = 0;  // < 2^24 slots and overflow entries per launch
+    int32_t last_loc = 0;  // < 2^31 slots and overflow entries per launch
     auto emit = [&](const double pos[3], const double cos[3], int element) {
         DevRecord r;
         for (int i = 0; i < 3; i++) { r.pos[i] = pos[i]; r.cos[i] = cos[i]; }
         r.element = element;
         r.raynum = raynum;
         if (nrec < (uint32_t)a.slots) {
-            last_loc = t * a.slots + (int32_t)nrec;
+            last_loc = (int32_t)nrec * a.n_rays + t;  // slot-major: a wave's first records are contiguous
             a.rec_slots[last_loc] = r;
         } else {
             uint32_t o = atomicAdd(a.ovf_count, 1u);
