@@ -1,6 +1,8 @@
 // Scans and compaction kernels shared by both precisions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "st_kernels.h"
 
 namespace st {
@@ -9,6 +11,7 @@ namespace st {
 static constexpr int kScanItems = 16;
 static constexpr int kScanBlock = 256;
 static constexpr int kScanTile = kScanBlock * kScanItems;  // 4096
+static_assert(kScanTile == (int)kScanTileElems, "tile size");
 
 // Block-wide exclusive scan of one value per thread (LDS, 64-wide waves).
 __device__ uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
@@ -32,23 +35,51 @@ __device__ uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
     return off + x - v;
 }
 
+// A thread's kScanItems values are 64 contiguous bytes: whole threads move
+// them as four 16-byte loads and stores (the arrays are 16-byte aligned),
+// the tail thread element by element.
+__device__ __forceinline__ void load_items(const uint32_t *in, size_t base, size_t n, uint32_t v[kScanItems]) {
+    if (base + kScanItems <= n) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+#pragma unroll
+        for (int k = 0; k < kScanItems / 4; ++k) {
+            const uint4 q = p[k];
+            v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) v[i] = (base + i < n) ? in[base + i] : 0u;
+    }
+}
+__device__ __forceinline__ void store_items(uint32_t *out, size_t base, size_t n, const uint32_t v[kScanItems]) {
+    if (base + kScanItems <= n) {
+        uint4 *p = reinterpret_cast<uint4 *>(out + base);
+#pragma unroll
+        for (int k = 0; k < kScanItems / 4; ++k) p[k] = make_uint4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i)
+            if (base + i < n) out[base + i] = v[i];
+    }
+}
+
 __global__ __launch_bounds__(kScanBlock) void scan_tiles(const uint32_t *in, uint32_t *out, size_t n,
                                                          uint32_t *tile_sums) {
     const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
+    load_items(in, base, n, v);
     uint32_t s = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        v[i] = (base + i < n) ? in[base + i] : 0u;
-        s += v[i];
-    }
+    for (int i = 0; i < kScanItems; ++i) s += v[i];
     uint32_t tot;
     uint32_t off = block_excl_scan(s, &tot);
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
-        if (base + i < n) out[base + i] = off;
-        off += v[i];
+        const uint32_t x = v[i];
+        v[i] = off;
+        off += x;
     }
+    store_items(out, base, n, v);
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
 }
 
@@ -72,13 +103,6 @@ __global__ __launch_bounds__(kScanBlock) void scan_sums(uint32_t *sums, size_t n
     if (threadIdx.x == 0) *total = tot;
 }
 
-__global__ __launch_bounds__(kScanBlock) void scan_add(uint32_t *out, size_t n, const uint32_t *tile_off) {
-    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
-    const uint32_t add = tile_off[blockIdx.x];
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i)
-        if (base + i < n) out[base + i] += add;
-}
 
 __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -91,16 +115,15 @@ void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
 
 size_t scan_tmp_words(size_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
-void launch_exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, uint32_t *total,
-                           hipStream_t s) {
+void launch_tile_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tile_off, uint32_t *total,
+                      hipStream_t s) {
     const size_t ntiles = (n + kScanTile - 1) / kScanTile;
     if (ntiles == 0) {
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, total, (size_t)1, 0u);
         return;
     }
-    hipLaunchKernelGGL(scan_tiles, dim3((unsigned)ntiles), dim3(kScanBlock), 0, s, in, out, n, tmp);
-    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanBlock), 0, s, tmp, ntiles, total);
-    hipLaunchKernelGGL(scan_add, dim3((unsigned)ntiles), dim3(kScanBlock), 0, s, out, n, tmp);
+    hipLaunchKernelGGL(scan_tiles, dim3((unsigned)ntiles), dim3(kScanBlock), 0, s, in, out, n, tile_off);
+    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanBlock), 0, s, tile_off, ntiles, total);
 }
 
 // -------------------------------------------------------- compaction ------
@@ -113,66 +136,69 @@ void launch_mask_popc(const uint64_t *mask, uint32_t *counts, size_t nwords, hip
                        nwords);
 }
 
-__global__ void mask_compact_kernel(const uint64_t *mask, const uint32_t *offsets, size_t nwords, uint64_t s0,
-                                    uint32_t limit, uint64_t *hits) {
+__global__ void mask_compact_kernel(const uint64_t *mask, const uint32_t *offsets, const uint32_t *tile_off,
+                                    size_t nwords, uint64_t s0, uint32_t limit, uint64_t *hits) {
     size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nwords) return;
     uint64_t m = mask[w];
-    uint32_t o = offsets[w];
+    uint32_t o = offsets[w] + tile_off[w / kScanTile];
     while (m != 0 && o < limit) {
         int b = __ffsll((unsigned long long)m) - 1;
         hits[o++] = s0 + w * 64 + (uint64_t)b;
         m &= m - 1;
     }
 }
-void launch_mask_compact(const uint64_t *mask, const uint32_t *offsets, size_t nwords, uint64_t s0, uint32_t limit,
-                         uint64_t *hits, hipStream_t s) {
+void launch_mask_compact(const uint64_t *mask, const uint32_t *offsets, const uint32_t *tile_off, size_t nwords,
+                         uint64_t s0, uint32_t limit, uint64_t *hits, hipStream_t s) {
     hipLaunchKernelGGL(mask_compact_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s, mask, offsets,
-                       nwords, s0, limit, hits);
+                       tile_off, nwords, s0, limit, hits);
 }
 
 // Slots are slot-major (record j of ray t at slots[j * n_rays + t]), so the
 // first records of a wave's rays, most of the data, are read contiguously.
 __global__ void compact_records_kernel(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets,
-                                       int n_rays, int nslots, DevRecord *out) {
+                                       const uint32_t *tile_off, int n_rays, int nslots, DevRecord *out) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_rays) return;
     uint32_t c = count[t];
     uint32_t m = c < (uint32_t)nslots ? c : (uint32_t)nslots;
-    uint32_t o = offsets[t];
+    uint32_t o = offsets[t] + tile_off[t / kScanTile];
     for (uint32_t j = 0; j < m; ++j) out[o + j] = slots[(size_t)j * n_rays + t];
 }
-void launch_compact_records(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets, int n_rays,
-                            int nslots, DevRecord *out, hipStream_t s) {
+void launch_compact_records(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets,
+                            const uint32_t *tile_off, int n_rays, int nslots, DevRecord *out, hipStream_t s) {
     hipLaunchKernelGGL(compact_records_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, s, slots, count, offsets,
-                       n_rays, nslots, out);
+                       tile_off, n_rays, nslots, out);
 }
 
+// Overflow records are rare: a small grid walks however many there are.
 __global__ void compact_overflow_kernel(const DevRecord *ovf, const uint32_t *meta, const uint32_t *n_ovf_dev,
-                                        uint32_t cap, const uint32_t *offsets, DevRecord *out) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                        uint32_t cap, const uint32_t *offsets, const uint32_t *tile_off,
+                                        DevRecord *out) {
     uint32_t n = *n_ovf_dev;
     if (n > cap) n = cap;
-    if (i >= n) return;
-    uint32_t t = meta[2 * i], seq = meta[2 * i + 1];
-    out[offsets[t] + seq] = ovf[i];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t t = meta[2 * i], seq = meta[2 * i + 1];
+        out[offsets[t] + tile_off[t / kScanTile] + seq] = ovf[i];
+    }
 }
 void launch_compact_overflow(const DevRecord *ovf, const uint32_t *meta, const uint32_t *n_ovf_dev, uint32_t cap,
-                             const uint32_t *offsets, DevRecord *out, hipStream_t s) {
-    hipLaunchKernelGGL(compact_overflow_kernel, dim3((cap + 255) / 256), dim3(256), 0, s, ovf, meta, n_ovf_dev, cap,
-                       offsets, out);
+                             const uint32_t *offsets, const uint32_t *tile_off, DevRecord *out, hipStream_t s) {
+    const unsigned blocks = std::min<unsigned>(256u, (cap + 255) / 256);
+    hipLaunchKernelGGL(compact_overflow_kernel, dim3(std::max(1u, blocks)), dim3(256), 0, s, ovf, meta, n_ovf_dev,
+                       cap, offsets, tile_off, out);
 }
 
-__global__ void compact_rays_kernel(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, int n_rays,
-                                    DevRay *out) {
+__global__ void compact_rays_kernel(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets,
+                                    const uint32_t *tile_off, int n_rays, DevRay *out) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_rays) return;
-    if (flag[t]) out[offsets[t]] = slots[t];
+    if (flag[t]) out[offsets[t] + tile_off[t / kScanTile]] = slots[t];
 }
-void launch_compact_rays(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, int n_rays, DevRay *out,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(compact_rays_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, s, slots, flag, offsets, n_rays,
-                       out);
+void launch_compact_rays(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, const uint32_t *tile_off,
+                         int n_rays, DevRay *out, hipStream_t s) {
+    hipLaunchKernelGGL(compact_rays_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, s, slots, flag, offsets,
+                       tile_off, n_rays, out);
 }
 
 }  // namespace st

@@ -105,7 +105,7 @@ struct Context {
     DevBuf d_elems, d_faces, d_aux, d_stages, d_bvh, d_bvh_items, d_grid_cells, d_grid_items, d_foot;
     // run scratch
     DevBuf d_mask_scan_tmp, d_mask_total;
-    DevBuf d_mask, d_popc, d_offs, d_scan_tmp, d_total, d_hits, d_rec_slots, d_rec_count, d_rec_offs, d_ovf_rec,
+    DevBuf d_mask, d_popc, d_offs, d_scan_tmp, d_scan_tmp2, d_total, d_hits, d_rec_slots, d_rec_count, d_rec_offs, d_ovf_rec,
         d_ovf_meta, d_ovf_count, d_out_slots, d_out_flag, d_out_offs, d_flux;
     DevBuf d_rays_a, d_rays_b, d_sunpre, d_sunpre_batch[2];
 
@@ -303,6 +303,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     d_out_flag.ensure((size_t)kMaxChunk * sizeof(uint32_t));
     d_out_offs.ensure((size_t)kMaxChunk * sizeof(uint32_t));
     d_scan_tmp.ensure(scan_tmp_words((size_t)1 << 26) * sizeof(uint32_t));
+    d_scan_tmp2.ensure(scan_tmp_words((size_t)1 << 26) * sizeof(uint32_t));
 
     unsigned long long *ctrs = nullptr;
     if (count_events) {
@@ -457,9 +458,8 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         launch_fill_u32(ovf_count, 1, 0u, s);
         timed(timing.trace_ms[si], [&]() { launch_trace_stage<R>(a, s); });
         timing.rays_in[si] += (uint64_t)n;
-        launch_exclusive_scan(a.rec_count, d_rec_offs.as<uint32_t>(), (size_t)n, d_scan_tmp.as<uint32_t>(), total, s);
-        launch_exclusive_scan(a.out_flag, d_out_offs.as<uint32_t>(), (size_t)n, d_scan_tmp.as<uint32_t>(), total + 1,
-                              s);
+        launch_tile_scan(a.rec_count, d_rec_offs.as<uint32_t>(), (size_t)n, d_scan_tmp.as<uint32_t>(), total, s);
+        launch_tile_scan(a.out_flag, d_out_offs.as<uint32_t>(), (size_t)n, d_scan_tmp2.as<uint32_t>(), total + 1, s);
         HC(hipMemcpyAsync(h_tot, total, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HC(hipEventRecord(ev_tot, s));
         return Pending{a, si, n, next_rays, n_next, std::chrono::steady_clock::now()};
@@ -471,12 +471,13 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         const TraceArgs &a = p.a;
         const int n = p.n, si = p.si;
         std::shared_ptr<DevBuf> chunk = take_buffer((size_t)std::max<uint32_t>(1, tot[0]) * sizeof(DevRecord));
-        launch_compact_records(a.rec_slots, a.rec_count, d_rec_offs.as<uint32_t>(), n, kSlots,
-                               chunk->as<DevRecord>(), s);
+        launch_compact_records(a.rec_slots, a.rec_count, d_rec_offs.as<uint32_t>(), d_scan_tmp.as<uint32_t>(), n,
+                               kSlots, chunk->as<DevRecord>(), s);
         launch_compact_overflow(a.ovf_rec, a.ovf_meta, ovf_count, ovf_cap, d_rec_offs.as<uint32_t>(),
-                                chunk->as<DevRecord>(), s);
+                                d_scan_tmp.as<uint32_t>(), chunk->as<DevRecord>(), s);
         if (tot[1] && p.next_rays)
-            launch_compact_rays(a.out_slots, a.out_flag, d_out_offs.as<uint32_t>(), n, p.next_rays + *p.n_next, s);
+            launch_compact_rays(a.out_slots, a.out_flag, d_out_offs.as<uint32_t>(), d_scan_tmp2.as<uint32_t>(), n,
+                                p.next_rays + *p.n_next, s);
         recs[si].chunks.push_back(chunk);
         recs[si].counts.push_back(tot[0]);
         recs[si].total += tot[0];
@@ -543,7 +544,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             timed_on(sm, timing.mask_ms, [&]() { launch_sun_mask<R>(ma, sm); });
             timing.sun_rays += batch;
             launch_mask_popc(mask, popc, nwords, sm);
-            launch_exclusive_scan(popc, offs, nwords, mscan, mtotal, sm);
+            launch_tile_scan(popc, offs, nwords, mscan, mtotal, sm);
         };
         struct Ready {
             uint64_t *hit_list;
@@ -568,7 +569,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             r.take = (uint32_t)std::min<uint64_t>(h, want);
             r.hit_list = sun_index_all + hits;
             r.raynum_base = hits;
-            if (r.take) launch_mask_compact(mask, offs, nwords, s0, r.take, r.hit_list, sm);
+            if (r.take) launch_mask_compact(mask, offs, mscan, nwords, s0, r.take, r.hit_list, sm);
             r.pre = nullptr;
             if (presample && r.take) {
                 r.pre = (SunPre *)d_sunpre_batch[parity].ensure((size_t)r.take * sizeof(SunPre));

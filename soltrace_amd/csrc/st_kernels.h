@@ -97,24 +97,29 @@ template <typename R> void launch_sun_mask(const MaskArgs &a, hipStream_t s);
 template <typename R> void launch_sunshape_d(const SunPreArgs &a, hipStream_t s);
 template <typename R> void launch_trace_stage(const TraceArgs &a, hipStream_t s);
 
-// Exclusive scan of n uint32 values into out (out may alias in); total via
-// *total (device).  tmp must hold scan_tmp_words(n) words.
+// Tile scan of n uint32 values: out[i] is the exclusive prefix sum of in[i]
+// within its tile of kScanTileElems values, tile_off[k] the exclusive prefix
+// of the tile sums, *total (device) the sum.  The exclusive prefix of i is
+// out[i] + tile_off[i / kScanTileElems]: consumers add the tile offset, which
+// saves a pass over out.  tile_off must hold scan_tmp_words(n) words.
+constexpr uint32_t kScanTileElems = 4096;
 size_t scan_tmp_words(size_t n);
-void launch_exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, uint32_t *total,
-                           hipStream_t s);
+void launch_tile_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tile_off, uint32_t *total,
+                      hipStream_t s);
 
 // Fill n words with v (a kernel, so it may sit inside a captured graph).
 void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 // Mask words -> per-word popcounts.
 void launch_mask_popc(const uint64_t *mask, uint32_t *counts, size_t nwords, hipStream_t s);
 // Hit list: the first `limit` set bits of the mask, as sun-ray indices.
-void launch_mask_compact(const uint64_t *mask, const uint32_t *offsets, size_t nwords, uint64_t s0, uint32_t limit,
-                         uint64_t *hits, hipStream_t s);
+// (offsets, tile_off: a tile scan of the counts, as launch_tile_scan leaves it)
+void launch_mask_compact(const uint64_t *mask, const uint32_t *offsets, const uint32_t *tile_off, size_t nwords,
+                         uint64_t s0, uint32_t limit, uint64_t *hits, hipStream_t s);
 // Record and ray compaction after a stage launch.
-void launch_compact_records(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets, int n_rays,
-                            int nslots, DevRecord *out, hipStream_t s);
+void launch_compact_records(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets,
+                            const uint32_t *tile_off, int n_rays, int nslots, DevRecord *out, hipStream_t s);
 void launch_compact_overflow(const DevRecord *ovf, const uint32_t *meta, const uint32_t *n_ovf_dev, uint32_t cap,
-                             const uint32_t *offsets, DevRecord *out, hipStream_t s);
+                             const uint32_t *offsets, const uint32_t *tile_off, DevRecord *out, hipStream_t s);
 // Flux statistics over resident records (st_stats.hip).  mm: min x, max x,
 // min y, max y as order keys; acc: centroid sums (um), considered, binned,
 // 1 + last considered index.
@@ -122,7 +127,7 @@ void launch_stats_bounds(const DevRecord *rec, uint32_t n, int32_t elem, const F
                          hipStream_t s);
 void launch_stats_bin(const DevRecord *rec, uint32_t n, int32_t elem, const FluxSpec &fl, uint64_t base,
                       uint32_t *bins, unsigned long long *acc, hipStream_t s);
-void launch_compact_rays(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, int n_rays, DevRay *out,
-                         hipStream_t s);
+void launch_compact_rays(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, const uint32_t *tile_off,
+                         int n_rays, DevRay *out, hipStream_t s);
 
 }  // namespace st
