@@ -254,11 +254,12 @@ __global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
 // Sun mask over the sun-plane grid, compacted per wave.  About one sun ray in
 // ten hits something, and those rays are spread over every wave, so a
 // lane-per-ray candidate loop runs the full element test in nearly every
-// wave for a few lanes.  Here each lane draws kMaskRays sun points and only
-// screens its candidates by their footprints; the (ray, element) pairs that
-// pass go to a per-wave queue in LDS, and the element tests run on the queue,
-// one pair per lane.  A ray hits when any of its pairs does, which is what
-// any_hit() answers for it.
+// wave for a few lanes.  Here each lane draws kMaskRays sun points; the
+// wave's (ray, grid item) candidates, 0.6 per sun ray on the tower field but
+// unevenly spread, are flattened and screened by their footprints 64 at a
+// time; the pairs that pass go to a per-wave queue in LDS, and the element
+// tests run on the queue, one pair per lane.  A ray hits when any of its
+// pairs does, which is what any_hit() answers for it.
 constexpr int kMaskRays = 4;
 constexpr int kMaskQueue = 128;      // ring of pairs per wave (holds up to 127)
 // a pair packs its ray slot (r * 64 + lane) in 8 bits above a 24-bit element index
@@ -272,6 +273,8 @@ __global__ __launch_bounds__(kBlock) void sun_mask_grid_kernel(MaskArgs a) {
     __shared__ uint32_t ring[W][kMaskQueue];
     __shared__ double2 pts[W][kMaskRays * 64];
     __shared__ uint32_t hitw[W][kMaskRays * 2];
+    __shared__ uint32_t cstart[W][kMaskRays * 64];  // candidate-list starts, lane-major (lane * kMaskRays + r)
+    __shared__ int32_t kbase[W][kMaskRays * 64];    // first grid item of each ray slot (r * 64 + lane)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t ray0 = ((uint64_t)blockIdx.x * W + wv) * (64 * kMaskRays);
     const SceneView &sv = a.sv;
@@ -323,30 +326,50 @@ __global__ __launch_bounds__(kBlock) void sun_mask_grid_kernel(MaskArgs a) {
         count -= m;
         tests += m;
     };
-    int cr = 0, k = kb[0], kend = ke[0];
-    double csx = pts[wv][lane].x, csy = pts[wv][lane].y;
-    for (;;) {
-        while (k >= kend && cr < kMaskRays - 1) {
-            ++cr;
-            k = kb[cr];
-            kend = ke[cr];
-            const double2 q = pts[wv][cr * 64 + lane];
-            csx = q.x;
-            csy = q.y;
+    // The candidates of the wave's rays, flattened: lane l's rays own
+    // positions cstart[l * kMaskRays + r] onwards; each pass hands the next
+    // 64 positions to the 64 lanes, so no lane idles while another walks a
+    // long cell list.
+    uint32_t mine = 0;
+#pragma unroll
+    for (int r = 0; r < kMaskRays; r++) mine += (uint32_t)(ke[r] - kb[r]);
+    uint32_t incl = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+    {
+        uint32_t c = incl - mine;
+#pragma unroll
+        for (int r = 0; r < kMaskRays; r++) {
+            cstart[wv][lane * kMaskRays + r] = c;
+            kbase[wv][r * 64 + lane] = kb[r];
+            c += (uint32_t)(ke[r] - kb[r]);
         }
-        const bool active = k < kend;
-        if (__ballot(active) == 0) break;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t f = base + (uint32_t)lane;
         bool pass = false;
-        int j = 0;
-        if (active) {
-            j = sv.grid_items[k];
-            ++k;
-            pass = foot_contains(sv.foot, j, csx, csy);
+        int j = 0, slot = 0;
+        if (f < total) {
+            // the last list start at or below f owns position f
+            int q = 0;
+#pragma unroll
+            for (int step = kMaskRays * 32; step > 0; step >>= 1)
+                if (cstart[wv][q + step] <= f) q += step;
+            slot = (q % kMaskRays) * 64 + q / kMaskRays;
+            j = sv.grid_items[kbase[wv][slot] + (int)(f - cstart[wv][q])];
+            const double2 p = pts[wv][slot];
+            pass = foot_contains(sv.foot, j, p.x, p.y);
         }
         const uint64_t bits = __ballot(pass);
         if (pass) {
             const uint32_t pos = (head + count + (uint32_t)__popcll(bits & lanes_below())) & (kMaskQueue - 1);
-            ring[wv][pos] = ((uint32_t)(cr * 64 + lane) << 24) | (uint32_t)j;
+            ring[wv][pos] = ((uint32_t)slot << 24) | (uint32_t)j;
         }
         count += (uint32_t)__popcll(bits);
         if (count >= 64) drain(64);
