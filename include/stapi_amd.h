@@ -110,8 +110,9 @@ int st_amd_events(st_context_t pcxt, double *out, int n);
  * of s1, stage 2 starts from the rays of s2, and the sun-ray count is the
  * last s1 row's ray number. Random draws of a loaded ray use the stream of
  * sun-ray index (ray number - 1). n2 > 0 alone sets the ray count to n2, as
- * in the reference. The flux map of st_amd_flux_request does not see loaded
- * records.
+ * in the reference. A flux map requested on stage 1 (st_amd_flux_request)
+ * bins the loaded records, each ray's last one on the element; a ray's rows
+ * are taken to be contiguous, as saved.
  * With save_stage_data, the run keeps the rows the reference appends to its
  * two vectors; st_amd_saved_stage_data returns them.
  * Returns the record count, or -1 (messages as st_sim_run). */

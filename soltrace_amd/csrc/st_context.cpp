@@ -686,6 +686,17 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         recs[0].chunks.push_back(chunk);
         recs[0].counts.push_back(r0.size());
         recs[0].total += r0.size();
+        if (fs.active && fs.stage == 0) {
+            // a traced stage is binned into the flux map at record time;
+            // loaded records are binned here, each ray's last record on the
+            // element (the rows of a ray are contiguous, as saved)
+            DevBuf acc;
+            acc.ensure(6 * sizeof(unsigned long long));
+            launch_fill_u32(acc.as<uint32_t>(), 12, 0u, s);
+            launch_stats_bin(chunk->as<DevRecord>(), (uint32_t)r0.size(), fs.elem, fs, 0, d_flux.as<uint32_t>(),
+                             acc.as<unsigned long long>(), s);
+            HC(hipStreamSynchronize(s));
+        }
         n_next = load_n2;
         sun_ray_count = r0.back().raynum;
         n_hits_done = N;

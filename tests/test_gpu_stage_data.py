@@ -57,6 +57,36 @@ def test_load_failure_matches_oracle(hip_lib, oracle_lib):
     assert g[0] == o[0] == -1 and g[3][-1] == o[3][-1]
 
 
+def test_resumed_flux_map_equals_saving_run(hip_lib):
+    # a flux map on a stage-1 facet: binned at record time while tracing,
+    # from the loaded rows when resuming; the same records give the same map
+    import ctypes as C
+    from soltrace_amd import stapi
+    d = hip_lib.dll
+    d.st_amd_flux_request.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                      C.c_double, C.c_double, C.c_double]
+    d.st_amd_flux_result.argtypes = [C.c_void_p, C.POINTER(C.c_uint), C.POINTER(C.c_double)]
+    sc = three_stage_scene(4000, flat=True)
+
+    def run(s1=None, s2=None, save=False):
+        ctx = hip_lib.st_create_context()
+        try:
+            sc.apply(hip_lib, ctx)
+            assert d.st_amd_flux_request(ctx, 0, 4, 20, 20, 0, -0.5, 0.5, -0.5, 0.5) == 1
+            n = stapi.run_data(hip_lib, ctx, 8, s1, s2, save)
+            buf = (C.c_uint * 400)()
+            assert d.st_amd_flux_result(ctx, buf, None) == 400
+            return n, np.frombuffer(buf, dtype=np.uint32).copy(), stapi.saved_stage_data(hip_lib, ctx)
+        finally:
+            hip_lib.st_free_context(ctx)
+
+    print("flux map: save, then resume", flush=True)
+    n, f_saved, (s1, s2) = run(save=True)
+    m, f_resumed, _ = run(s1, s2)
+    assert n > 0 and m > 0 and f_saved.sum() > 0
+    np.testing.assert_array_equal(f_resumed, f_saved)
+
+
 CXX = r"""
 #include <cstdio>
 #include <vector>

@@ -18,15 +18,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_SHIM = os.path.join(ROOT, "oracle", "_ref", "ref_run_data.so")
 
 
-def three_stage_scene(n=1500):
-    """Twelve parabolic facets, a partly reflecting secondary above them, and
-    an absorbing receiver between."""
+def three_stage_scene(n=1500, flat=False):
+    """Twelve parabolic (or flat) facets, a partly reflecting secondary above
+    them, and an absorbing receiver between."""
     s = Scene(sun_shape="p", sun_sigma=4.65, sun_xyz=(0.1, -0.2, 1.0), raycount=n, maxcount=10 ** 6)
     s.optics = [Optic("mirror", OpticFace(refl=0.9, slope=0.8, spec=0.3), OpticFace(refl=0.9, slope=0.8, spec=0.3)),
                 Optic("second", OpticFace(refl=0.6, slope=0.5), OpticFace(refl=0.6, slope=0.5)),
                 Optic("recv", OpticFace(refl=0.1), OpticFace(refl=0.1))]
     facets = [Element(xyz=(x, y, 0.0), aim=(-0.02 * x, -0.02 * y, 1.0), aperture="r", ap_params=(1.0, 1.0),
-                      surface="p", surf_params=(0.12, 0.12), optic="mirror")
+                      surface="f" if flat else "p", surf_params=() if flat else (0.12, 0.12), optic="mirror")
               for x in (-1.65, -0.55, 0.55, 1.65) for y in (-1.1, 0.0, 1.1)]
     second = Element(xyz=(0, 0, 4.0), aim=(0.05, 0, -1.0), aperture="c", ap_params=(2.2,), surface="f",
                      optic="second")
