@@ -233,7 +233,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the trace path shards by ray ranges, see DESIGN.md, but no multi-GPU run is built)")
+              "(the trace path would shard by ray ranges, but no multi-GPU run is built)")
         sys.exit(2)
 
     import torch
