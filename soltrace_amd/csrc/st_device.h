@@ -1358,20 +1358,23 @@ STD int table_index(const double *ang, int n, double theta, bool ascending) {
 // polar-angle bin (padded by more than the float error of the bin index),
 // so it only ever rejects what the exact test rejects; sunshape_d_exact is
 // the reference's table walk and test.  theta^2 (t2) is in mrad^2.
-STD bool sunshape_d_screen(uint32_t w0, uint32_t w1, uint32_t w2, const DevSun &sun, const double *aux, double *t2out,
-                           double *u2out) {
+// The draws enter unscaled: 2 ma (w 2^-32) is the exact product w (2 ma
+// 2^-32), since scaling by 2^-32 is exact, so tx and ty round as the
+// reference's 2 ma u - ma does; the envelope holds its bounds times 2^32, so
+// u2 > bound compares as w2 > bound 2^32, exactly.
+STD bool sunshape_d_screen(uint32_t w0, uint32_t w1, uint32_t w2, const DevSun &sun, const double *aux,
+                           double *t2out) {
     const double ma = sun.max_angle;
-    const double tx = 2.0 * ma * word_uniform(w0) - ma;
-    const double ty = 2.0 * ma * word_uniform(w1) - ma;
+    const double sc = 2.0 * ma * 2.3283064365386963e-10;
+    const double tx = (double)w0 * sc - ma;
+    const double ty = (double)w1 * sc - ma;
     const double t2 = tx * tx + ty * ty;
-    const double u2 = word_uniform(w2);
     *t2out = t2;
-    *u2out = u2;
     if (t2 > ma * ma) return false;
     if (sun.env_n > 0) {
         int b = (int)(::sqrtf((float)t2) * (float)sun.env_scale);
         b = b < 0 ? 0 : (b >= sun.env_n ? sun.env_n - 1 : b);
-        if (u2 > aux[sun.env + b]) return false;
+        if ((double)w2 > aux[sun.env + b]) return false;
     }
     return true;
 }
@@ -1391,8 +1394,7 @@ STD bool sunshape_d_exact(double t2, double u2, const DevSun &sun, const double 
 
 STD bool sunshape_d_try(uint32_t w0, uint32_t w1, uint32_t w2, const DevSun &sun, const double *aux,
                         double *t2out) {
-    double u2;
-    return sunshape_d_screen(w0, w1, w2, sun, aux, t2out, &u2) && sunshape_d_exact(*t2out, u2, sun, aux);
+    return sunshape_d_screen(w0, w1, w2, sun, aux, t2out) && sunshape_d_exact(*t2out, word_uniform(w2), sun, aux);
 }
 
 // Group q of the 'd' loop for the stream (seed, stage 0, sun_index) whose
@@ -1416,18 +1418,18 @@ STD int sunshape_d_group(uint32_t seed, uint64_t sun_index, uint32_t k0, uint32_
 #pragma unroll
         for (int i = 0; i < 4; ++i) w[4 * c + i] = b[i];
     }
-    double t2s[4], u2s[4];
+    double t2s[4];
     uint32_t cand = 0;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         if (j0 + t < 0) continue;
-        if (sunshape_d_screen(w[3 * t], w[3 * t + 1], w[3 * t + 2], sun, aux, &t2s[t], &u2s[t])) cand |= 1u << t;
+        if (sunshape_d_screen(w[3 * t], w[3 * t + 1], w[3 * t + 2], sun, aux, &t2s[t])) cand |= 1u << t;
     }
     while (cand != 0u) {
         const int t = __ffs(cand) - 1;
         cand &= cand - 1u;
         const double t2 = t == 0 ? t2s[0] : (t == 1 ? t2s[1] : (t == 2 ? t2s[2] : t2s[3]));
-        const double u2 = t == 0 ? u2s[0] : (t == 1 ? u2s[1] : (t == 2 ? u2s[2] : u2s[3]));
+        const double u2 = word_uniform(t == 0 ? w[2] : (t == 1 ? w[5] : (t == 2 ? w[8] : w[11])));
         if (sunshape_d_exact(t2, u2, sun, aux)) {
             *theta2 = t2;
             return t;

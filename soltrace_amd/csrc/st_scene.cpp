@@ -1138,7 +1138,7 @@ bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
                 for (int k = 0; k < ds.tab_n; k++)
                     if (sun.angle[k] > lo && sun.angle[k] < hi) m = std::max(m, sun.intensity[k]);
                 double v = m / sun.max_intensity;
-                out.aux.push_back(v * (1.0 + 1e-5) + 1e-7);
+                out.aux.push_back((v * (1.0 + 1e-5) + 1e-7) * 4294967296.0);  // in units of 2^-32
             }
             // bins past the last one (theta up to sqrt(2) * max angle) fall
             // to the exact test

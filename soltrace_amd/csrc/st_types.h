@@ -82,7 +82,7 @@ struct DevSun {
     int32_t shape, point_source;
     int32_t tab, tab_n;     // mirrored table (angles then intensities) in aux
     int32_t tab_ascending;  // 1: angles strictly ascending (binary search is exact)
-    int32_t env, env_n;     // upper envelope of intensity/max per polar-angle bin (aux)
+    int32_t env, env_n;     // upper envelope of intensity/max per polar-angle bin, times 2^32 (aux)
     int32_t pad;
     double env_scale;       // bins per mrad
 };
