@@ -79,3 +79,40 @@ def test_stats_grid_equals_flux_binning(oracle_lib, autoscale):
     assert list(bounds) == [fb[0], fb[1], fb[2], fb[3]]
     assert st["bx"] == fst[0] and st["by"] == fst[1] and st["binned"] == fst[2] > 0
     assert st["zval"] == pytest.approx(0.035, rel=1e-12)  # a cylinder's z value is its radius
+
+
+def test_cylinder_unroll_follows_projected_width(oracle_lib):
+    # a bare tube of radius R under an overhead sun: hits land on its upper
+    # half with a density per unit arc length proportional to the cosine of
+    # the angle from the top.  The unroll (app/src/project.cpp:1376-1386)
+    # maps a point at angle phi from the bottom to arc length R phi, so the
+    # bins of |x| < pi R / 2 stay empty and bin [a, b] of the upper half
+    # holds the fraction |sin(b / R) - sin(a / R)| / 2 of the hits.
+    R, L, n, nx = 0.5, 4.0, 200_000, 40
+    s = Scene(sun_shape="p", sun_sigma=0.01, sun_xyz=(0.0, 0.0, 100.0), raycount=n, maxcount=100 * n,
+              sunshape_err=False, optics_err=False)
+    s.optics = [Optic("abs", OpticFace(dist="g", refl=0.0), OpticFace(dist="g", refl=0.0))]
+    s.stages = [Stage(elements=[Element(xyz=(0, 0, 0), aim=(0, 0, 1), aperture="l", ap_params=(0.0, 0.0, L),
+                                        surface="t", surf_params=(1.0 / R,), optic="abs")])]
+    out = {}
+
+    def after(lib, ctx):
+        out["s"] = element_stats(lib, ctx, "st_oracle_element_stats", 0, 0, nx, 2, 0, 1, 1000.0,
+                                 (-1.0, 1.0, -L / 2, L / 2))
+
+    nrec, _, msgs, _ = run_scene(oracle_lib, s, 11, after=after)
+    assert nrec == n, msgs
+    bounds, grid, st = out["s"]
+    assert bounds[0] == pytest.approx(-math.pi * R) and bounds[1] == pytest.approx(math.pi * R)
+    counts = grid.sum(axis=1).astype(float)
+    assert counts.sum() == n
+    edges = np.linspace(-math.pi * R, math.pi * R, nx + 1)
+    lower = (np.abs(edges[1:]) <= math.pi * R / 2) & (np.abs(edges[:-1]) <= math.pi * R / 2)
+    assert counts[lower].sum() == 0
+    frac = np.abs(np.sin(edges[1:] / R) - np.sin(edges[:-1] / R)) / 2.0
+    frac[lower] = 0.0
+    assert frac.sum() == pytest.approx(1.0, rel=1e-12)
+    expect = n * frac
+    upper = ~lower
+    chi2 = float(np.sum((counts[upper] - expect[upper]) ** 2 / expect[upper]))
+    assert chi2 / upper.sum() < 2.0, chi2
