@@ -29,9 +29,14 @@ static constexpr int32_t kNoRecord = INT32_MIN;  // a record lost to a full over
 // ------------------------------------------------------ sun-ray source ----
 // coretrace/generateray.cpp:58-161 (the sun-plane draws are full 53-bit):
 // the point on the sun plane, then the ray it starts in the global frame.
+// x = MinX + (MaxX - MinX) U (coretrace/generateray.cpp:143-144) with the
+// uniform's 2^-32 folded into the width: the product is the same exact
+// number, so it rounds the same.
 STD void sun_plane_point(Rng &rng, const DevSun &sun, double psun[2]) {
-    psun[0] = sun.minx + (sun.maxx - sun.minx) * rng.uniform();
-    psun[1] = sun.miny + (sun.maxy - sun.miny) * rng.uniform();
+    const double wx = (sun.maxx - sun.minx) * 2.3283064365386963e-10;
+    const double wy = (sun.maxy - sun.miny) * 2.3283064365386963e-10;
+    psun[0] = sun.minx + (double)rng.next_word() * wx;
+    psun[1] = sun.miny + (double)rng.next_word() * wy;
 }
 
 STD void sun_ray_from_plane(const DevSun &sun, const DevStage &st, const double psun[2], double pg[3],
