@@ -61,7 +61,7 @@ FLOPS = {
 # by what the counters read for a plain 1 GiB device copy (FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0); profiles/hbm_traffic_tower1000.md has the runs.
 PMC_BYTES_PER_UNIT = {
-    "trace_stage0": 76.2 + 277.8,
+    "trace_stage0": 76.4 + 277.8,
     "sunshape": 8.6 + 16.7,
     "sun_mask": 0.14 + 0.13,
 }
