@@ -441,8 +441,8 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         a.ovf_meta = d_ovf_meta.as<uint32_t>();
         a.ovf_count = ovf_count;
         a.ovf_cap = ovf_cap;
-        a.out_slots = d_out_slots.as<DevRay>();
-        a.out_flag = d_out_flag.as<uint32_t>();
+        a.out_slots = next_rays ? d_out_slots.as<DevRay>() : nullptr;
+        a.out_flag = next_rays ? d_out_flag.as<uint32_t>() : nullptr;
         a.err_flags = err;
         a.flux = fs;
         a.flux_bins = fs.active ? d_flux.as<uint32_t>() : nullptr;
@@ -459,7 +459,11 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         timed(timing.trace_ms[si], [&]() { launch_trace_stage<R>(a, s); });
         timing.rays_in[si] += (uint64_t)n;
         launch_tile_scan(a.rec_count, d_rec_offs.as<uint32_t>(), (size_t)n, d_scan_tmp.as<uint32_t>(), total, s);
-        launch_tile_scan(a.out_flag, d_out_offs.as<uint32_t>(), (size_t)n, d_scan_tmp2.as<uint32_t>(), total + 1, s);
+        if (next_rays)
+            launch_tile_scan(a.out_flag, d_out_offs.as<uint32_t>(), (size_t)n, d_scan_tmp2.as<uint32_t>(), total + 1,
+                             s);
+        else
+            launch_fill_u32(total + 1, 1, 0u, s);
         HC(hipMemcpyAsync(h_tot, total, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HC(hipEventRecord(ev_tot, s));
         return Pending{a, si, n, next_rays, n_next, std::chrono::steady_clock::now()};
@@ -750,7 +754,10 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     DevBuf *bufs[2] = {&d_rays_a, &d_rays_b};
     int which = 0;
     for (size_t si = 1; si < nst && prev_has && !cancelled; si++) {
-        DevRay *nxt = (DevRay *)bufs[which ^ 1]->ensure(std::max<uint64_t>(n_cur, 1) * sizeof(DevRay));
+        // the last stage hands no rays on, unless its rays are saved
+        const bool hand_on = si + 1 < nst || (si == 1 && save_data);
+        DevRay *nxt = hand_on ? (DevRay *)bufs[which ^ 1]->ensure(std::max<uint64_t>(n_cur, 1) * sizeof(DevRay))
+                              : nullptr;
         uint64_t n_out = 0;
         for (uint64_t c = 0; c < n_cur; c += kMaxChunk) {
             int n = (int)std::min<uint64_t>(kMaxChunk, n_cur - c);

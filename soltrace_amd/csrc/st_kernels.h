@@ -68,8 +68,8 @@ struct TraceArgs {
     uint32_t *ovf_meta;          // (ray, seq) pairs
     uint32_t *ovf_count;
     uint32_t ovf_cap;
-    DevRay *out_slots;           // n_rays
-    uint32_t *out_flag;          // n_rays
+    DevRay *out_slots;           // n_rays (nullptr: no later stage takes the rays)
+    uint32_t *out_flag;          // n_rays (nullptr with out_slots)
     uint32_t *err_flags;         // bit 0: bad interaction, 1: phase mismatch, 2: overflow full
                                  // 3: sunshape sampling, 4: finite-element interpolation
     FluxSpec flux;

@@ -709,7 +709,7 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 4 : 2)) void trac
         }
     }
     a.rec_count[t] = nrec;
-    a.out_flag[t] = passed ? 1u : 0u;
+    if (a.out_flag != nullptr) a.out_flag[t] = passed ? 1u : 0u;
     if (a.counters) {
         atomicAdd(&a.counters[0], 1ull);
         atomicAdd(&a.counters[1], (unsigned long long)ev_lds[0][threadIdx.x]);
@@ -717,7 +717,7 @@ __global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 4 : 2)) void trac
         atomicAdd(&a.counters[3], (unsigned long long)(rng.k - ev_lds[2][threadIdx.x]));
         atomicAdd(&a.counters[4], (unsigned long long)nrec);
     }
-    if (passed) {
+    if (passed && a.out_slots != nullptr) {
         DevRay r;
         if (mhc == 0) {
             r = a.in_rays[t];  // passed through untouched: hand on the ray as it came
