@@ -533,12 +533,13 @@ STD int flux_bin(const FluxSpec &fl, const double pos[3], const double cos[3]) {
     return (gx >= 0 && gx < fl.nx && gy >= 0 && gy < fl.ny) ? gx * fl.ny + gy : -1;
 }
 
-template <typename R, int G>
-// Waves per SIMD: the generic variant needs the whole register file; the
-// basic FP32 variant runs 4 (128 registers: its spills cost less than the
-// latency the fourth wave hides; 5 waves spill too much), the basic FP64
-// variant 2 (3 is slower).
-__global__ __launch_bounds__(kBlock, G ? 1 : (sizeof(R) == 4 ? 4 : 2)) void trace_stage_kernel(TraceArgs a) {
+template <typename R, int G, int WV>
+// Waves per SIMD (WV): the generic variant needs the whole register file;
+// the basic FP32 variant runs 4 for the sun stage (128 registers: its spills
+// cost less than the latency the fourth wave hides; 5 waves spill too much)
+// and 3 for later stages, whose rays spend less time waiting on the BVH;
+// the basic FP64 variant 2 (3 is slower).
+__global__ __launch_bounds__(kBlock, G ? 1 : WV) void trace_stage_kernel(TraceArgs a) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= a.n_rays) return;
     const SceneView &sv = a.sv;
@@ -750,9 +751,11 @@ void launch_sun_mask(const MaskArgs &a, hipStream_t s) {
 template <typename R>
 void launch_trace_stage(const TraceArgs &a, hipStream_t s) {
     const int blocks = (a.n_rays + kBlock - 1) / kBlock;
-    if (a.generic == 2) hipLaunchKernelGGL((trace_stage_kernel<R, 2>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else if (a.generic) hipLaunchKernelGGL((trace_stage_kernel<R, 1>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL((trace_stage_kernel<R, 0>), dim3(blocks), dim3(kBlock), 0, s, a);
+    if (a.generic == 2) hipLaunchKernelGGL((trace_stage_kernel<R, 2, 1>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else if (a.generic) hipLaunchKernelGGL((trace_stage_kernel<R, 1, 1>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else if (sizeof(R) == 8) hipLaunchKernelGGL((trace_stage_kernel<R, 0, 2>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else if (a.si == 0) hipLaunchKernelGGL((trace_stage_kernel<R, 0, 4>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((trace_stage_kernel<R, 0, 3>), dim3(blocks), dim3(kBlock), 0, s, a);
 }
 
 }  // namespace st
