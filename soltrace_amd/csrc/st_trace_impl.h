@@ -272,10 +272,8 @@ static_assert(kMaskRays * 64 <= 256, "ray slot must fit in 8 bits");
 
 STD uint64_t lanes_below() { return (1ull << (threadIdx.x & 63)) - 1ull; }
 
-// 5 waves/SIMD for the basic FP32 variant (86 registers, no spills: 1% per
-// step over 4); the FP64 and generic variants keep the compiler's choice.
 template <typename R, int G>
-__global__ __launch_bounds__(kBlock, (!G && sizeof(R) == 4) ? 5 : 1) void sun_mask_grid_kernel(MaskArgs a) {
+__global__ __launch_bounds__(kBlock) void sun_mask_grid_kernel(MaskArgs a) {
     constexpr int W = kBlock / 64;
     __shared__ uint32_t ring[W][kMaskQueue];
     __shared__ double2 pts[W][kMaskRays * 64];
