@@ -739,13 +739,21 @@ void st_calc_euler_angles(double origin[3], double aimpoint[3], double zrot, dou
     if (euler_from_aim(origin, aimpoint, zrot, e))
         for (int i = 0; i < 3; i++) euler[i] = e[i];
 }
+// The exported transforms write their outputs in coretrace/mathproc.cpp:
+// 115-173's order (no copy of the direction), so aliased arguments give the
+// reference's result.
 void st_transform_to_local(double posref[3], double cosref[3], double origin[3], double R[3][3], double posloc[3],
                            double cosloc[3]) {
-    to_local(posref, cosref, origin, R, posloc, cosloc);
+    double d[3] = {posref[0] - origin[0], posref[1] - origin[1], posref[2] - origin[2]};
+    mat_vec(R, d, posloc);
+    mat_vec(R, cosref, cosloc);
 }
 void st_transform_to_reference(double posloc[3], double cosloc[3], double origin[3], double RT[3][3],
                                double posref[3], double cosref[3]) {
-    to_ref(posloc, cosloc, origin, RT, posref, cosref);
+    double t[3];
+    mat_vec(RT, posloc, t);
+    mat_vec(RT, cosloc, cosref);
+    for (int i = 0; i < 3; i++) posref[i] = t[i] + origin[i];
 }
 void st_matrix_vector_mult(double m[3][3], double v[3], double mxv[3]) { mat_vec(m, v, mxv); }
 void st_calc_transform_matrices(double euler[3], double R[3][3], double RT[3][3]) { euler_matrices(euler, R, RT); }

@@ -619,8 +619,10 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             bool have_next = false;
             for (uint32_t c = 0; c < cur_b.take && ok; c += kMaxChunk) {
                 const int n = (int)std::min<uint32_t>(kMaxChunk, cur_b.take - c);
-                Pending p = trace_launch(0, n, cur_b.hit_list + c, (uint32_t)(cur_b.raynum_base + c), nullptr, 0, next,
-                                         &n_next, cur_b.pre != nullptr ? cur_b.pre + c : nullptr);
+                // a single stage hands no rays on
+                Pending p = trace_launch(0, n, cur_b.hit_list + c, (uint32_t)(cur_b.raynum_base + c), nullptr, 0,
+                                         nst > 1 ? next : nullptr, &n_next,
+                                         cur_b.pre != nullptr ? cur_b.pre + c : nullptr);
                 if (c == 0 && cur_b.more) {
                     next_b = prepare();  // overlaps the trace just queued
                     have_next = true;
@@ -1290,41 +1292,40 @@ STCORE_API void st_calc_euler_angles(double origin[3], double aimpoint[3], doubl
     if (euler_from_aim(origin, aimpoint, zrot, e))
         for (int i = 0; i < 3; i++) euler[i] = e[i];
 }
+// The utilities write their outputs in the order coretrace/mathproc.cpp:58-173
+// does, so a caller passing one array as both input and output gets the
+// reference's result.
+static void matrix_vector_mult(const double m[3][3], const double *v, double *mxv) {
+    mxv[0] = m[0][0] * v[0] + m[0][1] * v[1] + m[0][2] * v[2];
+    mxv[1] = m[1][0] * v[0] + m[1][1] * v[1] + m[1][2] * v[2];
+    mxv[2] = m[2][0] * v[0] + m[2][1] * v[1] + m[2][2] * v[2];
+}
 STCORE_API void st_transform_to_local(double posref[3], double cosref[3], double origin[3], double R[3][3],
                                       double posloc[3], double cosloc[3]) {
-    double d[3] = {posref[0] - origin[0], posref[1] - origin[1], posref[2] - origin[2]};
-    double cc[3] = {cosref[0], cosref[1], cosref[2]};
-    for (int i = 0; i < 3; i++) {
-        posloc[i] = R[i][0] * d[0] + R[i][1] * d[1] + R[i][2] * d[2];
-        cosloc[i] = R[i][0] * cc[0] + R[i][1] * cc[1] + R[i][2] * cc[2];
-    }
+    double d[3];
+    for (int i = 0; i < 3; i++) d[i] = posref[i] - origin[i];
+    matrix_vector_mult(R, d, posloc);
+    matrix_vector_mult(R, cosref, cosloc);
 }
 STCORE_API void st_transform_to_reference(double posloc[3], double cosloc[3], double origin[3], double RT[3][3],
                                           double posref[3], double cosref[3]) {
-    double pl[3] = {posloc[0], posloc[1], posloc[2]}, cl[3] = {cosloc[0], cosloc[1], cosloc[2]};
-    for (int i = 0; i < 3; i++) {
-        posref[i] = RT[i][0] * pl[0] + RT[i][1] * pl[1] + RT[i][2] * pl[2] + origin[i];
-        cosref[i] = RT[i][0] * cl[0] + RT[i][1] * cl[1] + RT[i][2] * cl[2];
-    }
+    double d[3];
+    matrix_vector_mult(RT, posloc, d);
+    matrix_vector_mult(RT, cosloc, cosref);
+    for (int i = 0; i < 3; i++) posref[i] = d[i] + origin[i];
 }
-STCORE_API void st_matrix_vector_mult(double m[3][3], double v[3], double mxv[3]) {
-    double vv[3] = {v[0], v[1], v[2]};
-    for (int i = 0; i < 3; i++) mxv[i] = m[i][0] * vv[0] + m[i][1] * vv[1] + m[i][2] * vv[2];
-}
+STCORE_API void st_matrix_vector_mult(double m[3][3], double v[3], double mxv[3]) { matrix_vector_mult(m, v, mxv); }
 STCORE_API void st_calc_transform_matrices(double euler[3], double R[3][3], double RT[3][3]) {
     double m[9];
     euler_matrix(euler, m);
     for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-            R[i][j] = m[3 * i + j];
-            RT[j][i] = m[3 * i + j];
-        }
+        for (int j = 0; j < 3; j++) R[i][j] = m[3 * i + j];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) RT[i][j] = R[j][i];
 }
 STCORE_API void st_matrix_transpose(double in[3][3], double out[3][3]) {
-    double t[3][3];
     for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) t[i][j] = in[j][i];
-    memcpy(out, t, sizeof t);
+        for (int j = 0; j < 3; j++) out[i][j] = in[j][i];
 }
 
 // --------------------------------------------------- st_amd extensions ----

@@ -753,7 +753,8 @@ void launch_trace_stage(const TraceArgs &a, hipStream_t s) {
     const int blocks = (a.n_rays + kBlock - 1) / kBlock;
     if (a.generic == 2) hipLaunchKernelGGL((trace_stage_kernel<R, 2, 1>), dim3(blocks), dim3(kBlock), 0, s, a);
     else if (a.generic) hipLaunchKernelGGL((trace_stage_kernel<R, 1, 1>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else if (sizeof(R) == 8) hipLaunchKernelGGL((trace_stage_kernel<R, 0, 2>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else if constexpr (sizeof(R) == 8)
+        hipLaunchKernelGGL((trace_stage_kernel<R, 0, 2>), dim3(blocks), dim3(kBlock), 0, s, a);
     else if (a.si == 0) hipLaunchKernelGGL((trace_stage_kernel<R, 0, 4>), dim3(blocks), dim3(kBlock), 0, s, a);
     else hipLaunchKernelGGL((trace_stage_kernel<R, 0, 3>), dim3(blocks), dim3(kBlock), 0, s, a);
 }

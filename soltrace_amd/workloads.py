@@ -7,10 +7,12 @@ benchmarks and parity tests run, built only from their stated parameters.
 * :func:`tower` -- N spherically focused 10 x 10 m heliostats on a radially
   staggered layout within 1 km, flat 15 x 15 m receiver at 150 m, Buie CSR
   sun, sun at 45 deg elevation and 180 deg azimuth (aim = bisector of the sun
-  and receiver vectors, as app/deploy/api/examples/tower-demo.py:171-178).
+  and receiver vectors, as app/deploy/api/examples/tower-demo.py:49-54).
 * :func:`faceted_field` -- N heliostats x 4 flat 5 x 5 m facets plus a
   cylindrical external receiver (as the reference's tower sample's receiver:
   element on the wall, aimed along +y, 't' surface with an 'l' aperture).
+  Each facet is turned about its normal so its edges follow the heliostat's
+  rows: the four facets tile the heliostat with 5 cm gaps and never overlap.
 * :func:`buie_profile` -- the Buie et al. (2003) sunshape, tabulated for the
   'd' user sunshape (angles in mrad).
 """
@@ -129,6 +131,19 @@ def tower(n_helio: int = 1000, raycount: int = 10_000_000, helio_size: float = 1
     return s
 
 
+def facet_zrot(nrm, u) -> float:
+    """The z rotation (degrees) that turns an element aimed along unit
+    vector `nrm` so its local x axis lies along `u` (in the element's plane).
+    With zrot 0 the local x axis is (cos a, 0, -sin a) and y is
+    (-sin a sin b, cos b, -cos a sin b), a = atan2(nx, nz), b = asin(ny)
+    (coretrace/input.cpp:174-198, mathproc.cpp:175-217); zrot g turns x to
+    cos(g) x0 - sin(g) y0."""
+    a, b = math.atan2(nrm[0], nrm[2]), math.asin(nrm[1])
+    x0 = np.array([math.cos(a), 0.0, -math.sin(a)])
+    y0 = np.array([-math.sin(a) * math.sin(b), math.cos(b), -math.cos(a) * math.sin(b)])
+    return math.degrees(math.atan2(-float(u @ y0), float(u @ x0)))
+
+
 def faceted_field(n_helio: int = 10_000, raycount: int = 50_000_000, facet: float = 5.0,
                   rec_radius: float = 8.5, rec_height: float = 20.0, tower_h: float = 200.0,
                   r_max: float = 1500.0, sun_elev: float = 45.0, sun_azim: float = 180.0,
@@ -153,9 +168,10 @@ def faceted_field(n_helio: int = 10_000, raycount: int = 50_000_000, facet: floa
         u = np.cross([0.0, 0.0, 1.0], nrm)
         u /= np.linalg.norm(u)
         w = np.cross(nrm, u)
+        zrot = facet_zrot(nrm, u)
         for du, dw in ((-0.5, -0.5), (0.5, -0.5), (-0.5, 0.5), (0.5, 0.5)):
             c = p + (du * u + dw * w) * (facet + 0.05)
-            els.append(Element(xyz=tuple(c), aim=tuple(c + 100.0 * nrm), aperture="r",
+            els.append(Element(xyz=tuple(c), aim=tuple(c + 100.0 * nrm), zrot=zrot, aperture="r",
                                ap_params=(facet, facet), surface="f", optic="helio"))
     recel = Element(xyz=(0.0, -rec_radius, tower_h), aim=(0.0, 1000.0 - rec_radius, tower_h), aperture="l",
                     ap_params=(0.0, 0.0, rec_height), surface="t", surf_params=(1.0 / rec_radius,),

@@ -1,10 +1,9 @@
 """GPU parity over every in-scope surface, aperture and optics branch: the
-HIP library (FP64 path) against the oracle on the same per-ray draws, the
-exact-cull check (sun-plane grid + BVH against a full scan), and the FP32
-path against the oracle.  The FP32 path draws the same errors (every
-accept/reject test is double), so all but a handful of rays keep their hit
-sequence -- a float intersection can still land on the other side of an
-aperture edge -- and those that do must have hit points within 1e-4 m."""
+HIP library against the oracle on the same per-ray draws, in both
+precisions, and the exact-cull check (sun-plane grid + BVH against a full
+scan).  The FP32 path draws the same errors (every accept/reject test is
+double), so every ray must keep its hit sequence: stage, element and ray
+number equal, hit points and directions within 1e-4 (1e-6 for FP64)."""
 import os
 import tempfile
 
@@ -12,8 +11,9 @@ import numpy as np
 import pytest
 
 from soltrace_amd import workloads
-from harness import (assert_records_match, counter_mode, hip_sun_index, oracle_sun_index, run_scene)
+from harness import assert_records_match, counter_mode, run_scene
 from test_oracle_vs_ref import CASES, _mini, _write, fe_scene, torus_target_scene, vshot_scene
+from torus_checks import assert_within_reference_bounds, far_field_scene, first_hit_stats
 
 pytestmark = pytest.mark.gpu
 
@@ -30,16 +30,21 @@ def _setup(bits=64, cull=1):
 GPU_CASES = sorted(CASES)
 
 
+TOL = {64: 1e-6, 32: 1e-4}
+
+
+@pytest.mark.parametrize("bits", [64, 32])
 @pytest.mark.parametrize("name", GPU_CASES)
-def test_feature_scene_fp64(hip_lib, oracle_lib, name):
+def test_feature_scene(hip_lib, oracle_lib, name, bits):
     sc = _mini(**CASES[name])
     n_o, r_o, _, _ = run_scene(oracle_lib, sc, 31, setup=counter_mode)
-    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 31, setup=_setup(64))
+    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 31, setup=_setup(bits))
     assert n_g == n_o and n_o > 0, (n_g, n_o, m_g)
-    assert_records_match(r_g, r_o, pos_tol=1e-6, cos_tol=1e-6)
+    assert_records_match(r_g, r_o, pos_tol=TOL[bits], cos_tol=TOL[bits])
 
 
-def test_surface_files_fp64(hip_lib, oracle_lib):
+@pytest.mark.parametrize("bits", [64, 32])
+def test_surface_files(hip_lib, oracle_lib, bits):
     with tempfile.TemporaryDirectory() as d:
         mon = _write(d, "z.mon", "3\n0\n0\n0\n0.2\n0.001\n0.22\n0.0005\n0\n0.0003\n0.0001\n")
         rows = "".join(f"{x:.3f} {0.25 * x * x:.6f}\n" for x in np.linspace(0.0, 1.2, 13))
@@ -49,9 +54,9 @@ def test_surface_files_fp64(hip_lib, oracle_lib):
                    _mini(surface="i", sp=(), ap="a", app=(0.0, 1.2, 360.0), surf_file=csi),
                    _mini(surface="r", sp=(), ap="c", app=(2.0,), surf_file=ply)):
             n_o, r_o, _, _ = run_scene(oracle_lib, sc, 8, setup=counter_mode)
-            n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 8, setup=_setup(64))
+            n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 8, setup=_setup(bits))
             assert n_g == n_o and n_o > 0, (n_g, n_o, m_g)
-            assert_records_match(r_g, r_o, pos_tol=1e-6, cos_tol=1e-6)
+            assert_records_match(r_g, r_o, pos_tol=TOL[bits], cos_tol=TOL[bits])
 
 
 def test_cull_equals_full_scan(hip_lib):
@@ -76,21 +81,10 @@ def test_vshot_surface_fp64(hip_lib, oracle_lib):
 
 def test_vshot_surface_fp32(hip_lib, oracle_lib):
     sc = vshot_scene(n=3000)
-    n_o, r_o, _, si_o = run_scene(oracle_lib, sc, 17, setup=counter_mode, after=oracle_sun_index)
-    n_g, r_g, m_g, si_g = run_scene(hip_lib, sc, 17, setup=_setup(32),
-                                    after=lambda lib, ctx: hip_sun_index(lib, ctx, 3000))
-    assert n_g > 0, m_g
-    so, sg = _by_ray(r_o, si_o), _by_ray(r_g, si_g)
-    common = set(so) & set(sg)
-    assert len(common) >= 0.995 * max(len(so), len(sg))
-    same, worst = 0, 0.0
-    for k in common:
-        io, ig = so[k], sg[k]
-        if len(io) == len(ig) and np.array_equal(r_o.element[io], r_g.element[ig]):
-            same += 1
-            worst = max(worst, float(np.abs(r_o.pos[io] - r_g.pos[ig]).max()))
-    assert same >= 0.995 * len(common), (same, len(common))
-    assert worst <= 1e-4, worst
+    n_o, r_o, _, _ = run_scene(oracle_lib, sc, 17, setup=counter_mode)
+    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 17, setup=_setup(32))
+    assert n_g == n_o and n_o > 0, (n_g, n_o, m_g)
+    assert_records_match(r_g, r_o, pos_tol=1e-4, cos_tol=1e-4)
 
 
 @pytest.mark.parametrize("fed", ["dish.fed", "dish_un.fed"])
@@ -108,21 +102,10 @@ def test_finite_element_surface_fp64(hip_lib, oracle_lib, fed):
 
 def test_finite_element_surface_fp32(hip_lib, oracle_lib):
     sc = fe_scene("dish.fed", n=3000)
-    n_o, r_o, _, si_o = run_scene(oracle_lib, sc, 19, setup=counter_mode, after=oracle_sun_index)
-    n_g, r_g, m_g, si_g = run_scene(hip_lib, sc, 19, setup=_setup(32),
-                                    after=lambda lib, ctx: hip_sun_index(lib, ctx, 3000))
-    assert n_g > 0, m_g
-    so, sg = _by_ray(r_o, si_o), _by_ray(r_g, si_g)
-    common = set(so) & set(sg)
-    assert len(common) >= 0.995 * max(len(so), len(sg))
-    same, worst = 0, 0.0
-    for k in common:
-        io, ig = so[k], sg[k]
-        if len(io) == len(ig) and np.array_equal(r_o.element[io], r_g.element[ig]):
-            same += 1
-            worst = max(worst, float(np.abs(r_o.pos[io] - r_g.pos[ig]).max()))
-    assert same >= 0.995 * len(common), (same, len(common))
-    assert worst <= 1e-4, worst
+    n_o, r_o, _, _ = run_scene(oracle_lib, sc, 19, setup=counter_mode)
+    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 19, setup=_setup(32))
+    assert n_g == n_o and n_o > 0, (n_g, n_o, m_g)
+    assert_records_match(r_g, r_o, pos_tol=1e-4, cos_tol=1e-4)
 
 
 @pytest.mark.parametrize("bits", [64, 32])
@@ -180,33 +163,86 @@ def test_curved_field_fp64(hip_lib, oracle_lib):
     assert_records_match(r_g, r_o, pos_tol=1e-6, cos_tol=1e-6)
 
 
-def _by_ray(res, sun_idx):
-    """Group stage-1 records per sun ray: element sequence and positions."""
-    seqs = {}
-    order = np.lexsort((np.arange(len(res)), res.raynum, res.stage))
-    for i in order:
-        key = (int(sun_idx[res.raynum[i] - 1]), int(res.stage[i]))
-        seqs.setdefault(key, []).append(i)
-    return seqs
+def _by_ray(res):
+    out = {}
+    for i in np.lexsort((np.arange(len(res)), res.raynum)):
+        out.setdefault(int(res.raynum[i]), []).append(i)
+    return out
 
 
-def test_tower_fp32_close_to_oracle(hip_lib, oracle_lib):
-    sc = workloads.tower(n_helio=100, raycount=20000, r_max=500.0)
-    n_o, r_o, _, si_o = run_scene(oracle_lib, sc, 11, setup=counter_mode, after=oracle_sun_index)
-    n_g, r_g, m_g, si_g = run_scene(hip_lib, sc, 11, setup=_setup(32),
-                                    after=lambda lib, ctx: hip_sun_index(lib, ctx, 20000))
+def test_curved_field_fp32(hip_lib, oracle_lib):
+    # Every ray keeps its hit sequence except where the FP64 decision lies
+    # below float resolution: a hit on a cylinder's apex line within 1e-6 m
+    # of z = 2R, where the closest-hit search's z-aperture test
+    # (coretrace/raytrace.cpp:745-767) is decided by the hit point's last
+    # bits (one ray of 20,000 here).  Those rays are identified, not
+    # tolerated blindly.  All others keep their sequence; the first hit of
+    # each ray is within 1e-4 m, and later hits of a multi-bounce chain carry
+    # the earlier hits' differences through each curved surface's normal
+    # (a torus of 0.3 m tube radius turns a 2e-6 m difference into 1e-5 rad),
+    # so they are held to 1e-3 m, and 99.9% of all records to 1e-4 m.
+    from torus_checks import _rotation
+    sc = _curved_field()
+    n_o, r_o, _, _ = run_scene(oracle_lib, sc, 9, setup=counter_mode)
+    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 9, setup=_setup(32))
     assert n_g > 0, m_g
-    so, sg = _by_ray(r_o, si_o), _by_ray(r_g, si_g)
-    common = set(so) & set(sg)
-    assert len(common) >= 0.999 * max(len(so), len(sg))
-    same, worst = 0, 0.0
-    for k in common:
-        io, ig = so[k], sg[k]
-        if len(io) == len(ig) and np.array_equal(r_o.element[io], r_g.element[ig]):
-            same += 1
-            worst = max(worst, float(np.abs(r_o.pos[io] - r_g.pos[ig]).max()))
-    assert same >= 0.999 * len(common), (same, len(common))
-    assert worst <= 1e-4, worst
+    bo, bg = _by_ray(r_o), _by_ray(r_g)
+    assert set(bo) == set(bg)
+    els = sc.stages[0].elements
+    apex = n_rec = n_far = 0
+    for k, io in bo.items():
+        ig = bg[k]
+        so = [(int(r_o.stage[i]), int(r_o.element[i])) for i in io]
+        sg = [(int(r_g.stage[i]), int(r_g.element[i])) for i in ig]
+        if so == sg:
+            err = np.abs(r_g.pos[ig] - r_o.pos[io]).max(axis=1)
+            assert err[0] <= 1e-4 and err.max() <= 1e-3, (k, so, err)
+            n_rec += len(err)
+            n_far += int((err > 1e-4).sum())
+            continue
+        j = next(q for q in range(min(len(so), len(sg)) + 1) if q >= len(so) or q >= len(sg) or so[q] != sg[q])
+        rec = io[j] if j < len(io) else ig[j]
+        stage, el = (int(r_o.stage[rec]), abs(int(r_o.element[rec]))) if j < len(io) else \
+            (int(r_g.stage[rec]), abs(int(r_g.element[rec])))
+        e = els[el - 1] if stage == 1 and el > 0 else None
+        assert e is not None and e.surface == "t", (k, so, sg)
+        R = _rotation(np.subtract(e.aim, e.xyz))
+        pos = r_o.pos[rec] if j < len(io) else r_g.pos[rec]
+        z = (R @ (pos - np.asarray(e.xyz)))[2]
+        assert abs(z - 2.0 / e.surf_params[0]) < 1e-6, (k, so, sg, z)
+        apex += 1
+    assert apex <= 2, apex
+    assert n_far <= 1e-3 * n_rec, (n_far, n_rec)
+
+
+def test_tower_fp32_equals_oracle(hip_lib, oracle_lib):
+    sc = workloads.tower(n_helio=100, raycount=20000, r_max=500.0)
+    n_o, r_o, _, _ = run_scene(oracle_lib, sc, 11, setup=counter_mode)
+    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 11, setup=_setup(32))
+    assert n_g == n_o and n_o > 0, (n_g, n_o, m_g)
+    assert_records_match(r_g, r_o, pos_tol=1e-4, cos_tol=1e-4)
+
+
+_FACETED = {}
+
+
+def _faceted_oracle(oracle_lib):
+    # 500 heliostats x 4 flat 5 x 5 m facets (2,000 elements) on a surround
+    # field and the 't' cylinder receiver: the 40,000-facet workload's scene
+    # at a size the oracle's full scan finishes in about 20 s
+    if not _FACETED:
+        sc = workloads.faceted_field(n_helio=500, raycount=20000, r_max=450.0)
+        _FACETED["scene"] = sc
+        _FACETED["oracle"] = run_scene(oracle_lib, sc, 5, setup=counter_mode)
+    return _FACETED["scene"], _FACETED["oracle"]
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_faceted_field_equals_oracle(hip_lib, oracle_lib, bits):
+    sc, (n_o, r_o, _, _) = _faceted_oracle(oracle_lib)
+    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 5, setup=_setup(bits))
+    assert n_g == n_o and (np.abs(r_o.element[r_o.stage == 2]) == 1).sum() > 10000, (n_g, n_o, m_g)
+    assert_records_match(r_g, r_o, pos_tol=TOL[bits], cos_tol=TOL[bits])
 
 
 @pytest.mark.parametrize("receiver", ["flat", "cylinder"])
@@ -258,15 +294,31 @@ def test_torus_target(hip_lib, oracle_lib, bits):
 @pytest.mark.parametrize("bits", [64, 32])
 def test_torus_far_field(hip_lib, oracle_lib, bits):
     # sun rays from 10 km onto a torus whose axis is 30 degrees off the sun
-    from soltrace_amd.scene import Element, Optic, OpticFace, Scene, Stage
-    s = Scene(sun_shape="p", sun_sigma=4.65, sun_xyz=(0.2, 0.1, 10.0), raycount=5000, maxcount=10 ** 7)
-    s.optics = [Optic("m", OpticFace(refl=0.5), OpticFace(refl=0.5))]
-    tor = Element(xyz=(0, 0, 0), aim=(0.5, 0.2, 1), aperture="a", ap_params=(0, 0, 360.0), surface="d",
-                  surf_params=(0.8, 0.3), optic="m")
-    wide = Element(xyz=(0, 0, -0.5), aim=(0, 0, 1), aperture="r", ap_params=(3.0, 3.0), surface="f", optic="m")
-    s.stages = [Stage(elements=[tor, wide])]
-    n_o, r_o, _, _ = run_scene(oracle_lib, s, 3, setup=counter_mode)
-    n_g, r_g, m_g, _ = run_scene(hip_lib, s, 3, setup=_setup(bits))
+    sc = far_field_scene()
+    n_o, r_o, _, _ = run_scene(oracle_lib, sc, 3, setup=counter_mode)
+    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 3, setup=_setup(bits))
     assert n_g == n_o and (np.abs(r_o.element) == 1).sum() > 1000, (n_g, n_o, m_g)
-    tol = 1e-6 if bits == 64 else 1e-4
-    assert_records_match(r_g, r_o, pos_tol=tol, cos_tol=tol)
+    assert_records_match(r_g, r_o, pos_tol=TOL[bits], cos_tol=TOL[bits])
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_torus_far_field_pinned_by_the_quartic(hip_lib, bits):
+    # the library's far-field torus start point against the mathematics:
+    # first hits on a real root of the quartic, no nearer root skipped, no
+    # torus crossed on the way to the flat -- within the accuracy the
+    # reference's own solve reaches from 3 m
+    n, res, msgs, _ = run_scene(hip_lib, far_field_scene(n=20000), 3, setup=_setup(bits))
+    assert n > 0, msgs
+    assert_within_reference_bounds(first_hit_stats(res))
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_point_source_torus(hip_lib, oracle_lib, bits):
+    # a point source 3 m from the torus: rays in random directions
+    # (coretrace/generateray.cpp:101-133) met by the reference's near-field
+    # torus solve
+    sc = far_field_scene(n=5000, point_source=True)
+    n_o, r_o, _, _ = run_scene(oracle_lib, sc, 3, setup=counter_mode)
+    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 3, setup=_setup(bits))
+    assert n_g == n_o and (np.abs(r_o.element) == 1).sum() > 1000, (n_g, n_o, m_g)
+    assert_records_match(r_g, r_o, pos_tol=TOL[bits], cos_tol=TOL[bits])

@@ -23,10 +23,12 @@ def rows_match(a, b):
     assert np.all(np.abs(a[:, :6] - b[:, :6]) <= 1e-6 + 1e-6 * np.abs(b[:, :6])), np.abs(a[:, :6] - b[:, :6]).max()
 
 
-@pytest.mark.parametrize("seed", [3, 12])
-def test_save_hip_equals_oracle(hip_lib, oracle_lib, seed):
-    sc = three_stage_scene(4000)
-    print(f"save seed {seed}", flush=True)
+@pytest.mark.parametrize("seed,n_stages", [(3, 3), (12, 3), (3, 2)])
+def test_save_hip_equals_oracle(hip_lib, oracle_lib, seed, n_stages):
+    # n_stages 2: stage 2 is the last stage and still hands its rays on to
+    # the saved stage-2 array
+    sc = three_stage_scene(4000, n_stages=n_stages)
+    print(f"save seed {seed}, {n_stages} stages", flush=True)
     o = run_data(oracle_lib, sc, seed, save=True, setup=counter_mode)
     g = run_data(hip_lib, sc, seed, save=True)
     assert g[0] == o[0] > 0, (g[0], o[0], g[3])

@@ -75,3 +75,38 @@ def test_near_field_torus_is_well_conditioned(oracle_lib):
     tor = (a.stage == 2) & (np.abs(a.element) == 1)
     assert tor.sum() > 200
     assert np.abs(a.pos[tor] - b.pos[tor]).max() < 1e-9
+
+
+def _near_field_reference_stats(oracle_lib):
+    # the reference's own solve from 3 m (a point source; the oracle in the
+    # reference's stream mode, bit-exact with the reference build)
+    from torus_checks import far_field_scene, first_hit_stats
+    n, res, msgs, _ = run_scene(oracle_lib, far_field_scene(n=20000, point_source=True), 3)
+    assert n > 0, msgs
+    return first_hit_stats(res, src=(0.2, 0.1, 3.0))
+
+
+def test_near_field_reference_arithmetic_bounds(oracle_lib):
+    # how close the reference's own hits come to the quartic's roots where
+    # its solve is well-conditioned sets the bounds the far-field start point
+    # is held to
+    from torus_checks import assert_within_reference_bounds
+    assert_within_reference_bounds(_near_field_reference_stats(oracle_lib))
+
+
+def test_far_field_start_pinned_by_the_quartic(oracle_lib):
+    # the oracle's far-field start point (counter mode, as the HIP library
+    # solves) meets those bounds on sun rays from 10 km, and does no worse
+    # than the reference's arithmetic from 3 m; the reference's arithmetic
+    # run from 10 km does not come close
+    from harness import counter_mode
+    from torus_checks import (assert_no_worse_than, assert_within_reference_bounds, far_field_scene,
+                              first_hit_stats)
+    n, res, msgs, _ = run_scene(oracle_lib, far_field_scene(n=20000), 3, setup=counter_mode)
+    assert n > 0, msgs
+    st = first_hit_stats(res)
+    assert_within_reference_bounds(st)
+    assert_no_worse_than(st, _near_field_reference_stats(oracle_lib))
+    n, res, msgs, _ = run_scene(oracle_lib, far_field_scene(n=20000), 3)
+    st = first_hit_stats(res)
+    assert st["wild"] > 0.5 * st["torus"] and st["through"] > 0.1 * st["other"], st

@@ -54,9 +54,12 @@ def test_tower_dynamic_group_cull_is_exact(ref_lib, oracle_lib):
 
 
 def _mini(surface, sp, ap, app, sun="p", dist="g", interaction=2, extra_stage=True, surf_file=None,
-          refl=0.9, refr=(1.0, 1.5), stage_flags=(False, True, False), slope=1.0, spec=0.5, n=1500):
-    s = Scene(sun_shape=sun, sun_sigma=4.65 if sun != "g" else 2.0, sun_xyz=(0.2, 0.1, 10.0), raycount=n,
-              maxcount=400000)
+          refl=0.9, refr=(1.0, 1.5), stage_flags=(False, True, False), slope=1.0, spec=0.5, n=1500, point=False):
+    # point: rays leave a point source 2.5 m above the element in random
+    # directions (coretrace/generateray.cpp:101-133) instead of the sun plane
+    s = Scene(sun_shape=sun, sun_sigma=4.65 if sun != "g" else 2.0,
+              sun_xyz=(0.3, 0.2, 2.5) if point else (0.2, 0.1, 10.0), raycount=n, maxcount=400000,
+              point_source=point)
     front = OpticFace(dist=dist, refl=refl, trans=0.9, slope=slope, spec=spec, rreal=refr[0])
     back = OpticFace(dist=dist, refl=refl, trans=0.9, slope=slope, spec=spec, rreal=refr[1])
     s.optics = [Optic("m", front, back), Optic("abs", OpticFace(refl=0.2), OpticFace(refl=0.2))]
@@ -101,6 +104,8 @@ CASES = {
     "virtual_target": dict(surface="f", sp=(), ap="r", app=(2.0, 2.0), stage_flags=(True, True, False)),
     "tracethrough": dict(surface="f", sp=(), ap="r", app=(2.0, 2.0), stage_flags=(False, True, True)),
     "single_hit": dict(surface="s", sp=(0.6,), ap="c", app=(2.0,), stage_flags=(False, False, False)),
+    "point_source_flat": dict(surface="f", sp=(), ap="r", app=(2.0, 1.5), point=True),
+    "point_source_parabola": dict(surface="p", sp=(0.4, 0.4), ap="c", app=(2.0,), sun="g", point=True),
 }
 
 

@@ -18,9 +18,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_SHIM = os.path.join(ROOT, "oracle", "_ref", "ref_run_data.so")
 
 
-def three_stage_scene(n=1500, flat=False):
+def three_stage_scene(n=1500, flat=False, n_stages=3):
     """Twelve parabolic (or flat) facets, a partly reflecting secondary above
-    them, and an absorbing receiver between."""
+    them, and an absorbing receiver between (n_stages 2: no receiver, so
+    stage 2 is the last)."""
     s = Scene(sun_shape="p", sun_sigma=4.65, sun_xyz=(0.1, -0.2, 1.0), raycount=n, maxcount=10 ** 6)
     s.optics = [Optic("mirror", OpticFace(refl=0.9, slope=0.8, spec=0.3), OpticFace(refl=0.9, slope=0.8, spec=0.3)),
                 Optic("second", OpticFace(refl=0.6, slope=0.5), OpticFace(refl=0.6, slope=0.5)),
@@ -31,7 +32,7 @@ def three_stage_scene(n=1500, flat=False):
     second = Element(xyz=(0, 0, 4.0), aim=(0.05, 0, -1.0), aperture="c", ap_params=(2.2,), surface="f",
                      optic="second")
     recv = Element(xyz=(0, 0, 1.5), aim=(0, 0, 1.0), aperture="r", ap_params=(1.6, 1.6), surface="f", optic="recv")
-    s.stages = [Stage(elements=facets), Stage(elements=[second]), Stage(elements=[recv])]
+    s.stages = [Stage(elements=facets), Stage(elements=[second]), Stage(elements=[recv])][:n_stages]
     return s
 
 
@@ -67,6 +68,16 @@ def bitexact(a, b):
     assert ra.nsunrays == rb.nsunrays and ra.sun_box == rb.sun_box
     for x, y in zip(a[2], b[2]):
         assert x.shape == y.shape and np.array_equal(x.view(np.uint64), y.view(np.uint64))
+
+
+def test_save_with_stage2_last_equals_reference(ref_lib, ref_api, oracle_lib):
+    # stage 2 is the last stage: its rays go nowhere, but the saved stage-2
+    # array still holds them (coretrace/raytrace.cpp:1065-1078)
+    sc = three_stage_scene(n_stages=2)
+    r = run_data(ref_lib, sc, 3, save=True, api=ref_api)
+    o = run_data(oracle_lib, sc, 3, save=True)
+    bitexact(o, r)
+    assert len(o[2][1]) > 0
 
 
 @pytest.mark.parametrize("seed", [3, 11])
