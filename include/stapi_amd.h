@@ -62,6 +62,20 @@ int st_amd_flux_result(st_context_t pcxt, unsigned int *counts, double *bin_dims
 int st_amd_element_stats(st_context_t pcxt, int stage, int elem, int nx, int ny, int autoscale, int absorbed_only,
                          double dni, double bounds[4], unsigned int *grid, double *stats);
 
+/* st_amd_element_stats without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The bins and accumulators live in a fixed scratch array on
+ * each GPU, sized for at most 1,048,576 bins (nx*ny); larger grids return
+ * -1 with a message.  grid is required (the counts are copied straight into
+ * it).  Arguments and results are those of st_amd_element_stats.  Every call
+ * on one GPU uses that GPU's one scratch array: calls that overlap in time
+ * on the same GPU (from several threads) race and must be serialized by the
+ * caller.  Returns 1 or -1. */
+int st_amd_element_stats_noalloc(st_context_t pcxt, int stage, int elem, int nx, int ny, int autoscale,
+                                 int absorbed_only, double dni, double bounds[4], unsigned int *grid,
+                                 double *stats);
+
 /* Read a SolTrace .stinput system into the context with the headless strace
  * reader (coretrace/strace/main.cpp:358-397).  Returns 1, or -1 when the file
  * cannot be opened or is malformed. */

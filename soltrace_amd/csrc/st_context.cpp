@@ -68,6 +68,38 @@ static T *upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
     return b.as<T>();
 }
 
+// Scratch of st_amd_element_stats_noalloc, the allocation-free element
+// statistics: bin counts (at most kNoAllocMaxBins u32), then the centroid
+// accumulators and the autoscale min/max keys.  Each GPU holds its own copy
+// for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress gives on that device.
+static constexpr size_t kNoAllocMaxBins = (size_t)1 << 20;
+static constexpr size_t kNoAllocAccOff = kNoAllocMaxBins * sizeof(uint32_t);  // 6 x u64 accumulators
+static constexpr size_t kNoAllocMmOff = kNoAllocAccOff + 64;                 // 4 x u64 min/max keys
+static __device__ __align__(256) unsigned char scratch[kNoAllocMmOff + 64];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex scratch_mu;
+static void *scratch_dev[kMaxDevices] = {nullptr};
+
+// The device address of `scratch` on GPU `dev`, looked up once per device
+// with that device current.
+static void *scratch_on(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device ordinal out of range");
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[dev]) {
+        int cur = 0;
+        HC(hipGetDevice(&cur));
+        if (cur != dev) HC(hipSetDevice(dev));
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) HC(hipSetDevice(cur));
+        HC(e);
+        scratch_dev[dev] = ptr;
+    }
+    return scratch_dev[dev];
+}
+
 struct StageRecords {
     std::vector<std::shared_ptr<DevBuf>> chunks;
     std::vector<size_t> counts;
@@ -131,8 +163,11 @@ struct Context {
     bool save_data = false;
     std::vector<double> saved_s1, saved_s2;  // 8 and 7 doubles a row
 
+    void *scratch_ptr = nullptr;  // this context's GPU's copy of `scratch`
+
     Context() {
         HC(hipGetDevice(&device));
+        scratch_ptr = scratch_on(device);
         HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HC(hipStreamCreateWithFlags(&mask_stream, hipStreamNonBlocking));
         for (auto &e : ev_hits) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1402,136 +1437,177 @@ static unsigned long long host_order_key(double v) {
 }
 
 // The GUI's element statistics (app/src/project.cpp:1139-1436) over the last
-// run's records, binned on the GPU where the records are.
+// run's records, binned on the GPU where the records are.  d_bins (nx*ny
+// u32), d_acc (6 u64) and d_mm (4 u64) are device scratch; h_bins receives
+// the counts (nx*ny, host).  Makes no allocation of its own.
+static int element_stats_impl(Context *c, int stage, int elem, int nx, int ny, int autoscale, int absorbed_only,
+                              double dni, double bounds[4], double *stats, uint32_t *d_bins,
+                              unsigned long long *d_acc, unsigned long long *d_mm, unsigned int *h_bins) {
+    const HElement &e = c->model.stages[stage].elements[elem];
+    const char sf = (char)tolower(e.surf);
+    FluxSpec f{};
+    double eu[3];
+    if (!euler_from_aim(e.origin, e.aim, e.zrot, eu)) return -1;
+    euler_matrix(eu, f.R);
+    for (int k = 0; k < 3; k++) f.org[k] = e.origin[k];
+    const StageRecords &sr = c->recs[stage];
+    hipStream_t s = c->stream;
+    double minx = bounds[0], maxx = bounds[1], miny = bounds[2], maxy = bounds[3];
+    if (autoscale) {
+        const unsigned long long init[4] = {host_order_key(1e199), host_order_key(-1e199), host_order_key(1e199),
+                                            host_order_key(-1e199)};
+        HC(hipMemcpyAsync(d_mm, init, sizeof init, hipMemcpyHostToDevice, s));
+        for (size_t k = 0; k < sr.chunks.size(); k++)
+            launch_stats_bounds(sr.chunks[k]->as<DevRecord>(), (uint32_t)sr.counts[k], elem + 1, f, d_mm, s);
+        unsigned long long out[4];
+        HC(hipMemcpyAsync(out, d_mm, sizeof out, hipMemcpyDeviceToHost, s));
+        HC(hipStreamSynchronize(s));
+        minx = from_order_key(out[0]);
+        maxx = from_order_key(out[1]);
+        miny = from_order_key(out[2]);
+        maxy = from_order_key(out[3]);
+    }
+    if (maxx <= minx || maxy <= miny) return -1;
+    double gx = maxx - minx, gy = maxy - miny;
+    if (sf == 't' || sf == 's') {
+        const double Rr = 1.0 / (sf == 't' ? e.curv_rev : e.vcx);  // SurfaceParams[0]
+        f.radius = Rr;
+        if (sf == 't') {
+            f.kind = 1;
+            minx = -M_PI * Rr;
+            maxx = M_PI * Rr;
+            gx = 2.0 * M_PI * Rr;
+        } else {
+            f.kind = 2;
+            const double amin = asin(e.p[0] / Rr), amax = asin(e.p[1] / Rr);
+            minx = Rr * amin;
+            maxx = Rr * amax;
+            gx = (amax - amin) * Rr;
+        }
+    }
+    f.active = 1;
+    f.nx = nx;
+    f.ny = ny;
+    f.absorbed_only = absorbed_only != 0;
+    f.xmin = minx;
+    f.ymin = miny;
+    f.bx = gx / nx;
+    f.by = gy / ny;
+    const size_t nb = (size_t)nx * ny;
+    launch_fill_u32(d_bins, nb, 0u, s);
+    launch_fill_u32((uint32_t *)d_acc, 12, 0u, s);
+    uint64_t base = 0;
+    for (size_t k = 0; k < sr.chunks.size(); k++) {
+        launch_stats_bin(sr.chunks[k]->as<DevRecord>(), (uint32_t)sr.counts[k], elem + 1, f, base, d_bins, d_acc, s);
+        base += sr.counts[k];
+    }
+    unsigned long long a[6];
+    HC(hipMemcpyAsync(h_bins, d_bins, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HC(hipMemcpyAsync(a, d_acc, sizeof a, hipMemcpyDeviceToHost, s));
+    HC(hipStreamSynchronize(s));
+    // z value: the cylinder radius, or the z of the last ray considered
+    double zval = 0.0;
+    if (f.kind != 0) {
+        zval = f.radius;
+    } else if (a[5] > 0) {
+        uint64_t idx = a[5] - 1;
+        size_t k = 0;
+        while (k < sr.chunks.size() && idx >= sr.counts[k]) idx -= sr.counts[k++];
+        DevRecord r;
+        HC(hipMemcpyAsync(&r, sr.chunks[k]->as<DevRecord>() + idx, sizeof r, hipMemcpyDeviceToHost, s));
+        HC(hipStreamSynchronize(s));
+        // the third row of st_transform_to_local (coretrace/mathproc.cpp)
+        const double d[3] = {r.pos[0] - f.org[0], r.pos[1] - f.org[1], r.pos[2] - f.org[2]};
+        zval = f.R[6] * d[0] + f.R[7] * d[1] + f.R[8] * d[2];
+    }
+    // statistics in the GUI's order (app/src/project.cpp:1248-1288)
+    const double ppr = dni * (c->model.sun.maxx - c->model.sun.minx) * (c->model.sun.maxy - c->model.sun.miny) /
+                       (double)c->sun_ray_count;
+    const double zscale = ppr / (f.bx * f.by);
+    double peak = 0, sum = 0, sum2 = 0, minf = 1e99;
+    int npeak = -1;
+    for (int ix = 0; ix < nx; ix++)
+        for (int iy = 0; iy < ny; iy++) {
+            const double z = (double)h_bins[(size_t)ix * ny + iy] * zscale;
+            sum += z;
+            sum2 += z * z;
+            if (z > peak) {
+                peak = z;
+                npeak = (int)h_bins[(size_t)ix * ny + iy];
+            }
+            if (z < minf) minf = z;
+        }
+    const double nbins = (double)nx * (double)ny;
+    const double ave = sum / nbins;
+    const double sigma = sqrt((nbins * sum2 - sum * sum) / (nbins * nbins));
+    const double considered = (double)a[3];
+    auto centroid = [&](int q) { return considered > 0 ? (double)(int64_t)a[q] * 1e-6 / considered : 0.0; };
+    const double out[15] = {ppr, peak, minf, ave, sigma, sigma / ave, 100.0 / sqrt((double)npeak),
+                            100.0 / sqrt((double)c->total_records()), centroid(0), centroid(1), centroid(2),
+                            (double)a[4], zval, f.bx, f.by};
+    for (int q = 0; q < 15; q++) stats[q] = out[q];
+    bounds[0] = minx;
+    bounds[1] = maxx;
+    bounds[2] = miny;
+    bounds[3] = maxy;
+    return 1;
+}
+
+// Arguments every element-statistics entry point accepts: an element the GUI
+// can bin (flat, cylinder, partial cylinder), a run to bin, bins > 1.
+static bool element_stats_args_ok(Context *c, int stage, int elem, int nx, int ny, const double *bounds,
+                                  const double *stats) {
+    if (stage < 0 || (size_t)stage >= c->model.stages.size() || elem < 0 ||
+        (size_t)elem >= c->model.stages[stage].elements.size() || !bounds || !stats || nx <= 1 || ny <= 1)
+        return false;
+    const HElement &e = c->model.stages[stage].elements[elem];
+    const char sf = (char)tolower(e.surf), ap = (char)tolower(e.shape);
+    if (sf != 't' && sf != 'f' && !(ap == 'l' && sf == 's')) return false;
+    return (size_t)stage < c->recs.size();  // a run to bin
+}
+
 STCORE_API int st_amd_element_stats(st_context_t p, int stage, int elem, int nx, int ny, int autoscale,
                                     int absorbed_only, double dni, double bounds[4], unsigned int *grid,
                                     double *stats) {
     CTX(p, -1);
     try {
-        if (stage < 0 || (size_t)stage >= c->model.stages.size() || elem < 0 ||
-            (size_t)elem >= c->model.stages[stage].elements.size() || !bounds || !stats)
-            return -1;
-        const HElement &e = c->model.stages[stage].elements[elem];
-        const char sf = (char)tolower(e.surf), ap = (char)tolower(e.shape);
-        if (sf != 't' && sf != 'f' && !(ap == 'l' && sf == 's')) return -1;
-        if ((size_t)stage >= c->recs.size()) return -1;  // no run yet
-        FluxSpec f{};
-        double eu[3];
-        if (!euler_from_aim(e.origin, e.aim, e.zrot, eu)) return -1;
-        euler_matrix(eu, f.R);
-        for (int k = 0; k < 3; k++) f.org[k] = e.origin[k];
-        const StageRecords &sr = c->recs[stage];
-        hipStream_t s = c->stream;
-        double minx = bounds[0], maxx = bounds[1], miny = bounds[2], maxy = bounds[3];
-        if (autoscale) {
-            DevBuf mm;
-            mm.ensure(4 * sizeof(unsigned long long));
-            const unsigned long long init[4] = {host_order_key(1e199), host_order_key(-1e199), host_order_key(1e199),
-                                                host_order_key(-1e199)};
-            HC(hipMemcpyAsync(mm.p, init, sizeof init, hipMemcpyHostToDevice, s));
-            for (size_t k = 0; k < sr.chunks.size(); k++)
-                launch_stats_bounds(sr.chunks[k]->as<DevRecord>(), (uint32_t)sr.counts[k], elem + 1, f,
-                                    mm.as<unsigned long long>(), s);
-            unsigned long long out[4];
-            HC(hipMemcpyAsync(out, mm.p, sizeof out, hipMemcpyDeviceToHost, s));
-            HC(hipStreamSynchronize(s));
-            minx = from_order_key(out[0]);
-            maxx = from_order_key(out[1]);
-            miny = from_order_key(out[2]);
-            maxy = from_order_key(out[3]);
-        }
-        if (nx <= 1 || ny <= 1 || maxx <= minx || maxy <= miny) return -1;
-        double gx = maxx - minx, gy = maxy - miny;
-        if (sf == 't' || sf == 's') {
-            const double Rr = 1.0 / (sf == 't' ? e.curv_rev : e.vcx);  // SurfaceParams[0]
-            f.radius = Rr;
-            if (sf == 't') {
-                f.kind = 1;
-                minx = -M_PI * Rr;
-                maxx = M_PI * Rr;
-                gx = 2.0 * M_PI * Rr;
-            } else {
-                f.kind = 2;
-                const double amin = asin(e.p[0] / Rr), amax = asin(e.p[1] / Rr);
-                minx = Rr * amin;
-                maxx = Rr * amax;
-                gx = (amax - amin) * Rr;
-            }
-        }
-        f.active = 1;
-        f.nx = nx;
-        f.ny = ny;
-        f.absorbed_only = absorbed_only != 0;
-        f.xmin = minx;
-        f.ymin = miny;
-        f.bx = gx / nx;
-        f.by = gy / ny;
+        if (!element_stats_args_ok(c, stage, elem, nx, ny, bounds, stats)) return -1;
         const size_t nb = (size_t)nx * ny;
-        DevBuf bins, acc;
+        DevBuf bins, acc, mm;
         bins.ensure(nb * sizeof(uint32_t));
         acc.ensure(6 * sizeof(unsigned long long));
-        launch_fill_u32(bins.as<uint32_t>(), nb, 0u, s);
-        launch_fill_u32(acc.as<uint32_t>(), 12, 0u, s);
-        uint64_t base = 0;
-        for (size_t k = 0; k < sr.chunks.size(); k++) {
-            launch_stats_bin(sr.chunks[k]->as<DevRecord>(), (uint32_t)sr.counts[k], elem + 1, f, base,
-                             bins.as<uint32_t>(), acc.as<unsigned long long>(), s);
-            base += sr.counts[k];
-        }
+        mm.ensure(4 * sizeof(unsigned long long));
         std::vector<uint32_t> h(nb);
-        unsigned long long a[6];
-        HC(hipMemcpyAsync(h.data(), bins.p, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HC(hipMemcpyAsync(a, acc.p, sizeof a, hipMemcpyDeviceToHost, s));
-        HC(hipStreamSynchronize(s));
-        // z value: the cylinder radius, or the z of the last ray considered
-        double zval = 0.0;
-        if (f.kind != 0) {
-            zval = f.radius;
-        } else if (a[5] > 0) {
-            uint64_t idx = a[5] - 1;
-            size_t k = 0;
-            while (k < sr.chunks.size() && idx >= sr.counts[k]) idx -= sr.counts[k++];
-            DevRecord r;
-            HC(hipMemcpy(&r, sr.chunks[k]->as<DevRecord>() + idx, sizeof r, hipMemcpyDeviceToHost));
-            // the third row of st_transform_to_local (coretrace/mathproc.cpp)
-            const double d[3] = {r.pos[0] - f.org[0], r.pos[1] - f.org[1], r.pos[2] - f.org[2]};
-            zval = f.R[6] * d[0] + f.R[7] * d[1] + f.R[8] * d[2];
-        }
-        // statistics in the GUI's order (app/src/project.cpp:1248-1288)
-        const double ppr = dni * (c->model.sun.maxx - c->model.sun.minx) * (c->model.sun.maxy - c->model.sun.miny) /
-                           (double)c->sun_ray_count;
-        const double zscale = ppr / (f.bx * f.by);
-        double peak = 0, sum = 0, sum2 = 0, minf = 1e99;
-        int npeak = -1;
-        for (int ix = 0; ix < nx; ix++)
-            for (int iy = 0; iy < ny; iy++) {
-                const double z = (double)h[(size_t)ix * ny + iy] * zscale;
-                sum += z;
-                sum2 += z * z;
-                if (z > peak) {
-                    peak = z;
-                    npeak = (int)h[(size_t)ix * ny + iy];
-                }
-                if (z < minf) minf = z;
-            }
-        const double nbins = (double)nx * (double)ny;
-        const double ave = sum / nbins;
-        const double sigma = sqrt((nbins * sum2 - sum * sum) / (nbins * nbins));
-        const double considered = (double)a[3];
-        auto centroid = [&](int q) {
-            return considered > 0 ? (double)(int64_t)a[q] * 1e-6 / considered : 0.0;
-        };
-        const double out[15] = {ppr, peak, minf, ave, sigma, sigma / ave, 100.0 / sqrt((double)npeak),
-                                100.0 / sqrt((double)c->total_records()), centroid(0), centroid(1), centroid(2),
-                                (double)a[4], zval, f.bx, f.by};
-        for (int q = 0; q < 15; q++) stats[q] = out[q];
-        bounds[0] = minx;
-        bounds[1] = maxx;
-        bounds[2] = miny;
-        bounds[3] = maxy;
-        if (grid)
+        const int r = element_stats_impl(c, stage, elem, nx, ny, autoscale, absorbed_only, dni, bounds, stats,
+                                         bins.as<uint32_t>(), acc.as<unsigned long long>(),
+                                         mm.as<unsigned long long>(), h.data());
+        if (r == 1 && grid)
             for (size_t q = 0; q < nb; q++) grid[q] = h[q];
-        return 1;
+        return r;
+    } catch (const std::exception &ex) {
+        c->errlog("element statistics: %s", ex.what());
+        return -1;
+    }
+}
+
+// st_amd_element_stats without a device allocation: the bins and
+// accumulators live in this GPU's `scratch`, the counts go straight to the
+// caller's grid.  Calls overlapping on one GPU share that scratch and race.
+STCORE_API int st_amd_element_stats_noalloc(st_context_t p, int stage, int elem, int nx, int ny, int autoscale,
+                                            int absorbed_only, double dni, double bounds[4], unsigned int *grid,
+                                            double *stats) {
+    CTX(p, -1);
+    try {
+        if (!grid || !element_stats_args_ok(c, stage, elem, nx, ny, bounds, stats)) return -1;
+        if ((size_t)nx * (size_t)ny > kNoAllocMaxBins) {
+            c->errlog("element statistics: %d x %d bins exceed the %zu the allocation-free entry point holds", nx,
+                      ny, kNoAllocMaxBins);
+            return -1;
+        }
+        unsigned char *sp = (unsigned char *)c->scratch_ptr;
+        return element_stats_impl(c, stage, elem, nx, ny, autoscale, absorbed_only, dni, bounds, stats,
+                                  (uint32_t *)sp, (unsigned long long *)(sp + kNoAllocAccOff),
+                                  (unsigned long long *)(sp + kNoAllocMmOff), grid);
     } catch (const std::exception &ex) {
         c->errlog("element statistics: %s", ex.what());
         return -1;
