@@ -642,6 +642,7 @@ int st_sun_userdata(st_context_t p, st_uint_t n, double angle[], double intensit
         s.angle[i] = -angle[n - i - 1];
         s.intensity[i] = intensity[n - i - 1];
     }
+    sun_icdf_build(s);
     return (int)n;
 }
 

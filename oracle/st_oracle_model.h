@@ -108,6 +108,10 @@ struct Sun {
     double RT[3][3] = {{0}};
     double maxrad = 0, xcm = 0, ycm = 0;
     double minx = 0, maxx = 0, miny = 0, maxy = 0;
+    // counter mode: inverse-CDF segments of the table's polar-angle law, 8
+    // doubles each (lo, width, a, b, p0, p1, p2, mass before), then the
+    // total mass; empty when the table does not allow it
+    std::vector<double> icdf;
 };
 
 // coretrace/types.h:317-343, defaults types.cpp:439-448

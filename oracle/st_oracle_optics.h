@@ -37,6 +37,83 @@ inline void cone_direction(double theta2, double phi, const double RT[3][3], dou
     mat_vec(RT, loc, out);
 }
 
+// Counter-mode sampler of the 'd' sunshape's polar angle.  The law is the
+// reference's rejection loop's (coretrace/errors.cpp:160-185): (tx, ty)
+// uniform on the square of half-side max_angle, kept with probability
+// I(theta)/I_max inside the circle, I the linear interpolation of
+// errors.cpp:167-178, so theta has density I(theta) theta on [0, max_angle].
+// The counter streams draw it by inverting that law's CDF instead, from one
+// 53-bit uniform: on each table segment [lo, lo + w] the density is
+// (a + b x)(lo + x) in x = theta - lo and its CDF from lo the cubic
+// G(x) = x (p0 + x (p1 + x p2)).  The segment holding u * total comes from a
+// bisection over the masses before each segment, x from Newton steps kept in
+// a shrinking bracket (bisecting when a step leaves it) until a step changes
+// nothing.  Tables that are not ascending, or hold negative intensities,
+// keep the rejection loop.
+inline void sun_icdf_build(Sun &s) {
+    s.icdf.clear();
+    const size_t n = s.angle.size();
+    bool ok = s.max_angle > 0 && s.max_intensity > 0;
+    for (size_t i = 1; i < n; i++) ok = ok && s.angle[i] > s.angle[i - 1];
+    for (double v : s.intensity) ok = ok && v >= 0.0;
+    if (!ok) return;
+    double before = 0.0;
+    for (size_t i = 1; i < n; i++) {
+        const double t0 = s.angle[i - 1], t1 = s.angle[i];
+        const double lo = t0 > 0.0 ? t0 : 0.0;
+        const double hi = t1 < s.max_angle ? t1 : s.max_angle;
+        if (!(hi > lo)) continue;
+        const double slope = (s.intensity[i] - s.intensity[i - 1]) / (t1 - t0);
+        const double at_lo = s.intensity[i - 1] + (s.intensity[i] - s.intensity[i - 1]) * (lo - t0) / (t1 - t0);
+        const double w = hi - lo;
+        const double p0 = at_lo * lo, p1 = (at_lo + slope * lo) / 2.0, p2 = slope / 3.0;
+        const double seg[8] = {lo, w, at_lo, slope, p0, p1, p2, before};
+        s.icdf.insert(s.icdf.end(), seg, seg + 8);
+        before = before + w * (p0 + w * (p1 + w * p2));
+    }
+    if (s.icdf.empty() || !(before > 0.0)) {
+        s.icdf.clear();
+        return;
+    }
+    s.icdf.push_back(before);
+}
+
+// theta^2 (mrad^2) from the stream's next two 32-bit words w0, w1.
+inline double sun_icdf_theta2(const Sun &s, uint32_t w0, uint32_t w1) {
+    const size_t nseg = s.icdf.size() / 8;
+    const double *t = s.icdf.data();
+    const double total = t[8 * nseg];
+    const double u = ((double)w0 * 2097152.0 + (double)(w1 >> 11)) * 1.1102230246251565e-16;  // [0, 1), 2^-53 steps
+    const double m = u * total;
+    size_t k = 0;
+    for (size_t a = 0, b = nseg - 1; a < b;) {  // the last segment with mass-before <= m
+        const size_t mid = (a + b + 1) / 2;
+        if (t[8 * mid + 7] <= m) a = mid;
+        else b = mid - 1;
+        k = a;
+    }
+    const double *g = t + 8 * k;
+    const double lo = g[0], w = g[1], av = g[2], bv = g[3], p0 = g[4], p1 = g[5], p2 = g[6];
+    double r = m - g[7];
+    const double mass = w * (p0 + w * (p1 + w * p2));
+    if (!(r < mass)) r = mass;
+    if (!(r > 0.0)) r = 0.0;
+    double xl = 0.0, xh = w;
+    double x = mass > 0.0 ? w * (r / mass) : 0.0;
+    for (int it = 0; it < 60; ++it) {
+        const double f = x * (p0 + x * (p1 + x * p2)) - r;
+        if (f > 0.0) xh = x;
+        else xl = x;
+        const double d = (av + bv * x) * (lo + x);
+        double xn = d > 0.0 ? x - f / d : 0.5 * (xl + xh);
+        if (!(xn > xl && xn < xh)) xn = 0.5 * (xl + xh);
+        if (xn == x || !(xh > xl)) break;
+        x = xn;
+    }
+    const double th = lo + x;
+    return th * th;
+}
+
 // Sunshape (source 1) or specularity (source 2) perturbation,
 // coretrace/errors.cpp:60-226.
 inline void errors(Rng &rng, const double cin[3], int source, const Sun &sun, const Element &el, const OpticFace *opt,
@@ -73,6 +150,12 @@ inline void errors(Rng &rng, const double cin[3], int source, const Sun &sun, co
             break;
         }
         case 'd': case 'D': {
+            if (rng.mode() == 1 && !sun.icdf.empty()) {
+                const uint32_t w0 = (uint32_t)(rng() * 4294967296.0);
+                const uint32_t w1 = (uint32_t)(rng() * 4294967296.0);
+                theta2 = sun_icdf_theta2(sun, w0, w1) / 1000000.0;
+                break;
+            }
             const std::vector<double> &ang = sun.angle;
             const std::vector<double> &inten = sun.intensity;
             for (;;) {

@@ -1438,6 +1438,59 @@ STD int sunshape_d_group(uint32_t seed, uint64_t sun_index, uint32_t k0, uint32_
     return -1;
 }
 
+// The 'd' sunshape's polar angle by inversion, one 53-bit uniform per ray.
+// The reference's rejection loop (coretrace/errors.cpp:160-185) proposes
+// (tx, ty) uniform on the square of half-side max_angle and keeps it with
+// probability I(theta)/I_max inside the circle, I the table's linear
+// interpolation: the kept theta has density proportional to I(theta) theta
+// on [0, max_angle].  On each table segment that density is a quadratic in
+// x = theta - lo, (a + b x)(lo + x), and its integral from lo the cubic
+// G(x) = x (p0 + x (p1 + x p2)).  The host lays the segments out in aux, 8
+// doubles each: lo, width w, a, b, p0, p1, p2, and C, the mass of the
+// segments before it; the total mass follows the last segment.  The draw
+// picks the segment holding u * total by bisection over C and solves G(x) = r
+// by Newton steps kept inside a shrinking bracket, to the last bit.  Only
+// +, -, *, / are used, so an FP64 build without contraction gives the same
+// bits as the host.  Returns theta^2 in mrad^2.
+STD double sunshape_icdf_solve(const double *seg, double r) {
+    const double w = seg[1], p0 = seg[4], p1 = seg[5], p2 = seg[6];
+    const double a = seg[2], b = seg[3], lo = seg[0];
+    const double m = w * (p0 + w * (p1 + w * p2));
+    if (!(r < m)) r = m;
+    if (!(r > 0.0)) r = 0.0;
+    double xl = 0.0, xh = w;
+    double x = m > 0.0 ? w * (r / m) : 0.0;
+    for (int it = 0; it < 60; ++it) {
+        const double f = x * (p0 + x * (p1 + x * p2)) - r;
+        if (f > 0.0) xh = x;
+        else xl = x;
+        const double d = (a + b * x) * (lo + x);
+        double xn = d > 0.0 ? x - f / d : 0.5 * (xl + xh);
+        if (!(xn > xl && xn < xh)) xn = 0.5 * (xl + xh);
+        if (xn == x || !(xh > xl)) break;
+        x = xn;
+    }
+    return lo + x;
+}
+
+STD double sunshape_icdf(uint32_t w0, uint32_t w1, const DevSun &sun, const double *aux) {
+    const double *tab = aux + sun.icdf;
+    const int n = sun.icdf_n;
+    const double total = tab[kIcdfStride * n];
+    // 53-bit uniform in [0, 1): the high word then the top 21 bits of the next
+    const double u = ((double)w0 * 2097152.0 + (double)(w1 >> 11)) * 1.1102230246251565e-16;
+    const double m = u * total;
+    int lo = 0, hi = n - 1;  // the last segment whose mass-before is <= m
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[kIcdfStride * mid + 7] <= m) lo = mid;
+        else hi = mid - 1;
+    }
+    const double *seg = tab + kIcdfStride * lo;
+    const double th = sunshape_icdf_solve(seg, m - seg[7]);
+    return th * th;
+}
+
 // One angular perturbation of a direction: the sunshape error
 // (coretrace/errors.cpp:60-226, source 1), the slope error of the surface
 // normal (coretrace/surfacenormalerrors.cpp:58-167; slope = true, only 'g'
@@ -1470,9 +1523,14 @@ STD void perturb(Rng &rng, const double base[3], int dist, double delop, bool sl
                 theta2 = tx * tx + ty * ty;
             } while (theta2 > delop * delop);
         } else if (!slope && (dist == 'd' || dist == 'D')) {
-            for (;;) {
-                const uint32_t w0 = rng.next_word(), w1 = rng.next_word(), w2 = rng.next_word();
-                if (sunshape_d_try(w0, w1, w2, sun, aux, &theta2)) break;
+            if (sun.icdf_n > 0) {
+                const uint32_t w0 = rng.next_word(), w1 = rng.next_word();
+                theta2 = sunshape_icdf(w0, w1, sun, aux);
+            } else {
+                for (;;) {
+                    const uint32_t w0 = rng.next_word(), w1 = rng.next_word(), w2 = rng.next_word();
+                    if (sunshape_d_try(w0, w1, w2, sun, aux, &theta2)) break;
+                }
             }
             theta2 = theta2 / 1000000.0;
         } else if (!slope && (dist == 'f' || dist == 'F')) {

@@ -1143,6 +1143,34 @@ bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
             // bins past the last one (theta up to sqrt(2) * max angle) fall
             // to the exact test
         }
+        // Inverse-CDF segments of the same law (sunshape_icdf in
+        // st_device.h): density I(theta) theta on [0, max angle], I the
+        // interpolation of coretrace/errors.cpp:167-178 between mirrored
+        // entries i-1 and i.  Built for ascending tables of non-negative
+        // intensity; others keep the rejection loop.
+        bool nonneg = true;
+        for (double v : sun.intensity) nonneg = nonneg && v >= 0.0;
+        if (ds.tab_ascending && nonneg && sun.max_angle > 0 && sun.max_intensity > 0) {
+            std::vector<double> segs;
+            double cum = 0.0;
+            for (int i = 1; i < ds.tab_n; i++) {
+                const double s0 = sun.angle[i - 1], s1 = sun.angle[i];
+                const double lo = std::max(s0, 0.0), hi = std::min(s1, sun.max_angle);
+                if (!(hi > lo)) continue;
+                const double b = (sun.intensity[i] - sun.intensity[i - 1]) / (s1 - s0);
+                const double a = sun.intensity[i - 1] + (sun.intensity[i] - sun.intensity[i - 1]) * (lo - s0) / (s1 - s0);
+                const double w = hi - lo;
+                const double p0 = a * lo, p1 = (a + b * lo) / 2.0, p2 = b / 3.0;
+                segs.insert(segs.end(), {lo, w, a, b, p0, p1, p2, cum});
+                cum = cum + w * (p0 + w * (p1 + w * p2));
+            }
+            if (!segs.empty() && cum > 0.0) {
+                ds.icdf = (int)out.aux.size();
+                ds.icdf_n = (int)(segs.size() / kIcdfStride);
+                out.aux.insert(out.aux.end(), segs.begin(), segs.end());
+                out.aux.push_back(cum);
+            }
+        }
     }
 
     // Sun-plane footprints of the stage-1 elements: each hit box projected

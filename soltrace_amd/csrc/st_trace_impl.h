@@ -490,10 +490,36 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
     }
 }
 
+// The 'd' polar angle by inversion (sunshape_icdf): one lane per ray, two
+// draws at k0 and k0+1 of the ray's stage-0 stream.  One instantiation per
+// precision file, so each takes that file's contraction setting.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void sunshape_icdf_kernel(SunPreArgs a) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    Rng rng;
+    rng.begin(a.seed, 0, a.sun_index[i]);
+    rng.seek(a.k0);
+    const uint32_t w0 = rng.next_word(), w1 = rng.next_word();
+    SunPre p;
+    p.theta2 = sunshape_icdf(w0, w1, a.sun, a.aux) / 1000000.0;
+    p.kend = rng.k;
+    p.ok = 1u;
+    a.out[i] = p;
+    if (a.counters) {
+        const uint64_t m = __ballot(true);
+        if ((threadIdx.x & 63) == (uint32_t)(__ffsll((unsigned long long)m) - 1)) {
+            atomicAdd(&a.counters[0], (unsigned long long)__popcll(m));
+            atomicAdd(&a.counters[1], (unsigned long long)__popcll(m));
+        }
+    }
+}
+
 template <typename R>
 void launch_sunshape_d(const SunPreArgs &a, hipStream_t s) {
     const int blocks = (a.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(sunshape_d_kernel<R>, dim3(blocks), dim3(kBlock), 0, s, a);
+    if (a.sun.icdf_n > 0) hipLaunchKernelGGL(sunshape_icdf_kernel<R>, dim3(blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(sunshape_d_kernel<R>, dim3(blocks), dim3(kBlock), 0, s, a);
 }
 
 // ------------------------------------------------------- stage trace ------

@@ -72,6 +72,9 @@ struct DevStage {
                             // 2: and one is a finite-element surface
 };
 
+// Doubles per inverse-CDF segment of the 'd' sunshape (sunshape_icdf).
+constexpr int kIcdfStride = 8;
+
 // Sun (coretrace/types.h:195-221) with the sun frame and sun-plane box.
 struct DevSun {
     double origin[3];
@@ -83,7 +86,7 @@ struct DevSun {
     int32_t tab, tab_n;     // mirrored table (angles then intensities) in aux
     int32_t tab_ascending;  // 1: angles strictly ascending (binary search is exact)
     int32_t env, env_n;     // upper envelope of intensity/max per polar-angle bin, times 2^32 (aux)
-    int32_t pad;
+    int32_t icdf, icdf_n;   // inverse-CDF segments of the 'd' polar-angle law in aux (0 segments: none)
     double env_scale;       // bins per mrad
 };
 
