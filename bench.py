@@ -10,13 +10,28 @@ soltrace_amd.workloads.tower.
 
 The headline value is the FP32-bulk path.  The same job is also run on the
 FP64 path with the same seed (same counter-based draws) to report the FP32 /
-FP64 speed ratio and the flux-map NRMSE = RMS(F32 - F64) / max(F64).
+FP64 speed ratio and the flux-map NRMSE = RMS(F32 - F64) / max(F64).  The
+other two single-GPU workloads run beside it ("workloads"): the parabolic
+trough (1e6 hits, FP64, the reference's plain path) and the 40,000-facet
+field (5e7 hits, FP32).  "end_to_end" adds the result getters' copy of every
+record to the host (st_locations .. st_raynumbers), which st_sim_run itself
+leaves in HBM.
+
+Roofline: the dominant kernel's achieved rate is its ALGORITHMIC floating-
+point work per launch over its average launch time (HIP events on its own
+stream, summed over the timed steps).  The work per unit comes from the
+oracle built with operation counting (oracle/_build/st_oracle_count.so, run
+on a bounded sample of the same workload on the per-ray streams): flops per
+sun ray, per hitting ray, per ray entering a later stage, per tabulated-
+sunshape draw, and per element test of each stage; the element tests a
+kernel makes are the GPU's own event counts (its cull is exact, so these are
+the tests the search needs); BVH node visits are counted as no flops.
 
 `cpu_baseline` times the reference's own coretrace, compiled from source in
 this repository's oracle/ recipe (oracle/_ref/coretrace_ref.so), on the
 host's cores with the GUI's multithreaded scheme (one context per thread,
 rays split evenly, seeds seed, seed+123, seed+369, ...), on a bounded sample
-of the same scene.
+of the same scene, and on one thread.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -38,33 +53,24 @@ sys.path.insert(0, ROOT)
 from soltrace_amd import stapi, workloads  # noqa: E402
 
 METRIC = "ray hits/sec on 1 MI355X (FP32 vs FP64 vs CPU); flux-map NRMSE vs FP64"
-PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak
-PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector peak
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (published; needs packed FP32)
+PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector peak; also the non-packed FP32 rate
 N_HITS = 10_000_000
 NX = NY = 100
-
-# Hand-counted FP work per kernel event (FMA = 2 flops, add/mul/div/sqrt = 1,
-# transcendentals 1 each), from the device code in soltrace_amd/csrc.
-FLOPS = {
-    "sun_ray": 110,        # sun-plane point, sun->stage->global->stage transforms, grid cell
-    "stage_ray": 33,       # global -> stage transform of an entering ray
-    "test_sphere": 115,    # element transform + nudge + rebase + float quadric roots + aperture
-    "test_flat": 50,       # element transform + nudge + start-plane point + aperture
-    "bvh_node": 12,        # float slab test
-    "uniform": 5,          # conversion + rejection-loop arithmetic per uniform drawn in the kernel
-    "record": 420,         # double hit finish, record transform, efficiency, 2-3 perturbations, reflection
-    "sampler_try": 14,     # sunshape screen: proposal, radius, bin, envelope test (Philox is integer work)
-}
+COUNT_SO = os.path.join(ROOT, "oracle", "_build", "st_oracle_count.so")
+# CPU threads the box gives one GPU's job
+BOX_CPU_SHARE = 16
 
 # HBM bytes per unit (ray) of the dominant kernels on this workload, from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script, scaled
 # by what the counters read for a plain 1 GiB device copy (FETCH_SIZE x 2.0,
-# WRITE_SIZE x 1.0); profiles/hbm_traffic_tower1000.md has the runs.
+# WRITE_SIZE x 1.0); profiles/hbm_traffic_tower1000.md has the runs.  Not
+# measured in this run: a PMC run cannot time the kernels.
 PMC_BYTES_PER_UNIT = {
     "trace_stage0": 76.4 + 277.8,
-    "sunshape": 8.6 + 16.7,
     "sun_mask": 0.14 + 0.13,
 }
+PMC_SOURCE = "profiles/hbm_traffic_tower1000.md (calibrated FETCH_SIZE + WRITE_SIZE of an earlier run of this bench)"
 
 
 def _lib():
@@ -112,45 +118,117 @@ def _arr(lib, fn, ctx, n):
 
 
 def _timed_steps(lib, ctx, steps, warmup, seed, sync):
+    """Mean wall time of `steps` runs, and the mean of each run's
+    st_amd_timing (kernel times from HIP events on each kernel's stream)."""
     for _ in range(warmup):
         _run(lib, ctx, seed)
     sync()
+    acc = np.zeros(53)
     t0 = time.perf_counter()
     for _ in range(steps):
         _run(lib, ctx, seed)
+        acc += np.array(_arr(lib, "st_amd_timing", ctx, 53))
     sync()
-    return (time.perf_counter() - t0) / steps
+    return (time.perf_counter() - t0) / steps, list(acc / steps)
 
 
-def _flop_model(scene, ev, sampler_units):
-    """Algorithmic flops of each kernel of one run from its event counts."""
-    launches_mask = ev[0]
-    launches_trace = ev[1:17]
+def _workload_rate(lib, scene, bits, steps, seed, sync):
+    """ray hits/s and ms per run of one workload (no flux map)."""
+    ctx = lib.st_create_context()
+    try:
+        scene.apply(lib, ctx)
+        lib.dll.st_amd_set_precision(ctx, bits)
+        t, tm = _timed_steps(lib, ctx, steps, 1, seed, sync)
+        return {"value": scene.raycount / t, "ms_per_step": t * 1e3, "precision": f"fp{bits}",
+                "sun_rays": int(tm[19]), "records": int(sum(tm[36:52]))}
+    finally:
+        lib.st_free_context(ctx)
+
+
+def _end_to_end(lib, ctx, seed, sync):
+    """One run plus the getters' copy of every record to the host."""
+    sync()
+    t0 = time.perf_counter()
+    n = _run(lib, ctx, seed)
+    t1 = time.perf_counter()
+    stapi.fetch_results(lib, ctx)
+    t2 = time.perf_counter()
+    return {"ms_per_step": (t2 - t0) * 1e3, "trace_ms": (t1 - t0) * 1e3, "getters_ms": (t2 - t1) * 1e3,
+            "records": n, "value": N_HITS / (t2 - t0)}
+
+
+def counted_work(scene_fn, hits, seed):
+    """Algorithmic flops per unit from the operation-counting oracle on a
+    bounded sample of the workload (per-ray streams, the GPU's draws).
+    Returns per-unit figures: flops (add/sub, mul, div, sqrt) and
+    transcendentals, or None when the counting oracle is not built."""
+    if not os.path.exists(COUNT_SO):
+        return None
+    lib = stapi.StLib(COUNT_SO)
+    for n in ("st_oracle_set_rng", "st_oracle_set_torus_start"):
+        getattr(lib.dll, n).argtypes = [C.c_void_p, C.c_int]
+    lib.dll.st_opcount_read.argtypes = [C.POINTER(C.c_ulonglong), C.c_int]
+    sc = scene_fn(hits)
+    ctx = lib.st_create_context()
+    try:
+        sc.apply(lib, ctx)
+        lib.dll.st_oracle_set_rng(ctx, 1)
+        lib.dll.st_oracle_set_torus_start(ctx, 1)
+        lib.dll.st_opcount_reset()
+        t0 = time.perf_counter()
+        if stapi.run(lib, ctx, seed) < 0:
+            return None
+        dt = time.perf_counter() - t0
+        res = stapi.fetch_results(lib, ctx)
+    finally:
+        lib.st_free_context(ctx)
+    out = (C.c_ulonglong * 83)()
+    lib.dll.st_opcount_read(out, 83)
+    a = np.frombuffer(out, dtype=np.uint64).astype(np.float64)
+    ph, tests = a[:80].reshape(16, 5), a[80:]
+    fl = ph[:, :4].sum(axis=1)
+    tr = ph[:, 4]
+    sun = float(res.nsunrays)
+    rays2 = float(len(np.unique(res.raynum[res.stage == 2]))) if (res.stage == 2).any() else 0.0
+    def per(p, units):
+        return {"flops": fl[p] / units if units else 0.0, "trans": tr[p] / units if units else 0.0}
+    return {
+        "sample": f"{hits} ray hits ({int(sun)} sun rays) on the counting oracle, {dt:.1f} s",
+        "sun_ray": {"flops": (fl[0] + fl[1]) / sun, "trans": (tr[0] + tr[1]) / sun},
+        "hit": {"flops": (fl[1] + fl[2]) / hits, "trans": (tr[1] + tr[2]) / hits},
+        "sunshape": per(3, hits),
+        "stage2_ray": per(5, rays2),
+        "test_stage1": per(12, tests[0]),
+        "test_stage2": per(13, tests[1]),
+    }
+
+
+def kernel_work(cw, ev, hits_per_run, tm):
+    """Algorithmic flops of each kernel in one run: the counted per-unit
+    figures times the GPU's units and element tests (st_amd_events)."""
     base = 17
     mask = ev[base:base + 5]
-    per_stage = [ev[base + 5 * (1 + s): base + 5 * (2 + s)] for s in range(16)]
-    kinds = [s.elements[0].surface if s.elements else "f" for s in scene.stages]
-    def test_cost(k):
-        return FLOPS["test_sphere"] if k in "spot" else FLOPS["test_flat"]
-    out = {}
-    out["sun_mask"] = dict(launches=launches_mask,
-                           flops=mask[0] * FLOPS["sun_ray"] + mask[1] * test_cost(kinds[0]) +
-                           mask[3] * FLOPS["uniform"], units=mask[0])
-    for s in range(len(scene.stages)):
-        r, tests, nodes, unif, recs = per_stage[s]
-        fl = (r * (FLOPS["sun_ray"] if s == 0 else FLOPS["stage_ray"]) + tests * test_cost(kinds[s]) +
-              nodes * FLOPS["bvh_node"] + unif * FLOPS["uniform"] + recs * FLOPS["record"])
-        out[f"trace_stage{s}"] = dict(launches=launches_trace[s], flops=fl, units=r, tests=tests,
-                                      nodes=nodes, uniforms=unif, records=recs)
-    if len(ev) >= 104 and ev[102] > 0:
-        out["sunshape"] = dict(launches=launches_trace[0], flops=ev[102] * FLOPS["sampler_try"],
-                               units=sampler_units, tries_evaluated=ev[102], tries_needed=ev[103])
+    st = [ev[base + 5 * (1 + s): base + 5 * (2 + s)] for s in range(2)]
+    out = {
+        "sun_mask": {"launches": ev[0], "units": mask[0],
+                     "flops": mask[0] * cw["sun_ray"]["flops"] + mask[1] * cw["test_stage1"]["flops"]},
+        "trace_stage0": {"launches": ev[1], "units": st[0][0],
+                         "flops": st[0][0] * cw["hit"]["flops"] + st[0][1] * cw["test_stage1"]["flops"]},
+        "trace_stage1": {"launches": ev[2], "units": st[1][0],
+                         "flops": st[1][0] * cw["stage2_ray"]["flops"] + st[1][1] * cw["test_stage2"]["flops"]},
+        "sunshape": {"launches": ev[1], "units": hits_per_run, "flops": hits_per_run * cw["sunshape"]["flops"]},
+    }
+    for k, s_ in (("trace_stage0", 0), ("trace_stage1", 1)):
+        out[k]["tests"] = st[s_][1]
+        out[k]["bvh_nodes"] = st[s_][2]
+    out["sun_mask"]["tests"] = mask[1]
     return out
 
 
 def cpu_baseline(scene_fn, seed, target_s=12.0):
     """The reference build on the host cores, GUI threading scheme
-    (app/src/trace.cpp:709-729), on a bounded sample of the same scene."""
+    (app/src/trace.cpp:709-729), on a bounded sample of the same scene, and
+    its single-thread rate."""
     so = os.path.join(ROOT, "oracle", "_ref", "coretrace_ref.so")
     kind = "reference"
     if not os.path.exists(so):
@@ -159,8 +237,9 @@ def cpu_baseline(scene_fn, seed, target_s=12.0):
     if not os.path.exists(so):
         return None
     ref = stapi.StLib(so)
-    threads = max(1, min(16, os.cpu_count() or 1))
-    # calibrate on one thread
+    host_cpus = len(os.sched_getaffinity(0))
+    threads = max(1, min(BOX_CPU_SHARE, host_cpus))
+    # single thread, on a sample of about a second
     cal = 1500
     sc = scene_fn(cal)
     ctx = ref.st_create_context()
@@ -171,8 +250,7 @@ def cpu_baseline(scene_fn, seed, target_s=12.0):
     ref.st_free_context(ctx)
     per_thread = int(max(1000, min(2_000_000, rate1 * target_s)))
     total = per_thread * threads
-    ctxs, scenes = [], []
-    seeds = []
+    ctxs, seeds = [], []
     sd = seed
     for i in range(threads):
         sd = sd + i * 123  # cumulative, as the GUI does
@@ -198,8 +276,11 @@ def cpu_baseline(scene_fn, seed, target_s=12.0):
     if min(res) < 0:
         return None
     return {"value": total / dt, "unit": "ray hits/s", "cores": threads, "kind": kind,
+            "host_cpus": host_cpus, "value_1_thread": rate1,
             "sample": f"{threads} contexts x {per_thread} ray hits of the same 1,000-heliostat scene "
-                      f"(dynamic grouping on), {dt:.1f} s wall; single-thread calibration {rate1:.0f} hits/s"}
+                      f"(dynamic grouping on) on {threads} threads ({host_cpus} CPUs visible, "
+                      f"{BOX_CPU_SHARE} per GPU on the box), {dt:.1f} s wall; one thread: {rate1:.0f} hits/s "
+                      f"on {cal} hits"}
 
 
 def copy_calibration(mib):
@@ -225,6 +306,7 @@ def main():
     ap.add_argument("--seed", type=int, default=123)
     ap.add_argument("--hits", type=int, default=N_HITS)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true", help="only the tower workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (scale for the HBM byte counters)")
     args = ap.parse_args()
@@ -246,11 +328,9 @@ def main():
     c32 = _context(lib, scene, 32)
     c64 = _context(lib, scene, 64)
 
-    t32 = _timed_steps(lib, c32, args.steps, args.warmup, args.seed, sync)
-    tm32 = _arr(lib, "st_amd_timing", c32, 53)
+    t32, tm32 = _timed_steps(lib, c32, args.steps, args.warmup, args.seed, sync)
     f32 = _flux(lib, c32)
-    t64 = _timed_steps(lib, c64, max(1, args.steps // 2), 1, args.seed, sync)
-    tm64 = _arr(lib, "st_amd_timing", c64, 53)
+    t64, tm64 = _timed_steps(lib, c64, max(1, args.steps // 2), 1, args.seed, sync)
     f64 = _flux(lib, c64)
     nrmse = float(np.sqrt(np.mean((f32 - f64) ** 2)) / max(1.0, f64.max()))
 
@@ -259,18 +339,44 @@ def main():
     _run(lib, c32, args.seed)
     ev = _arr(lib, "st_amd_events", c32, 104)
     lib.dll.st_amd_count_events(c32, 0)
-    model = _flop_model(scene, ev, sampler_units=tm32[20])
-    # dominant kernel of the timed FP32 runs
+    # kernel times of the timed FP32 runs (mean per run)
     kms = {"sun_mask": tm32[0], "sunshape": tm32[52]}
     for s in range(len(scene.stages)):
         kms[f"trace_stage{s}"] = tm32[1 + s]
-    dom = max((k for k in kms if k in model), key=kms.get)
-    launches = max(1.0, model[dom]["launches"])
-    avg_ms = kms[dom] / launches
-    flops_per_launch = model[dom]["flops"] / launches
-    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
-    units_per_launch = model[dom]["units"] / launches
-    traffic = PMC_BYTES_PER_UNIT[dom] * units_per_launch if dom in PMC_BYTES_PER_UNIT else None
+    dom = max(kms, key=kms.get)
+    cw = counted_work(lambda n: workloads.tower(n_helio=1000, raycount=n), 4000, args.seed)
+    roof = None
+    if cw is not None:
+        work = kernel_work(cw, ev, args.hits, tm32)
+        w = work[dom]
+        launches = max(1.0, w["launches"])
+        avg_ms = kms[dom] / launches
+        flops_per_launch = w["flops"] / launches
+        achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+        units_per_launch = w["units"] / launches
+        traffic = PMC_BYTES_PER_UNIT[dom] * units_per_launch if dom in PMC_BYTES_PER_UNIT else None
+        roof = {
+            "bound": "valu",
+            "pipe": "vector ALU, FP32 and FP64 (no contraction, so no MFMA; HBM traffic is a few percent of "
+                    "8 TB/s). peak is the published FP32 vector rate, which needs packed FP32; the kernels "
+                    "issue non-packed FP32 and FP64, whose rate is the 78.6 TF FP64 peak (frac_nonpacked)",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "frac_nonpacked": achieved / PEAK_FP64_TFLOPS,
+            "traffic": traffic,
+            "traffic_source": PMC_SOURCE,
+            "flops_per_launch": flops_per_launch,
+            "avg_launch_ms": avg_ms,
+            "flops_per_unit": {k: round(v["flops"], 1) for k, v in cw.items() if isinstance(v, dict)},
+            "transcendentals_per_unit": {k: round(v["trans"], 2) for k, v in cw.items() if isinstance(v, dict)},
+            "work_sample": cw["sample"],
+            "kernels": {k: {"ms": round(kms.get(k, 0.0), 3), "launches": v["launches"], "units": v["units"],
+                            "tflops": (v["flops"] / (kms[k] * 1e-3) / 1e12) if kms.get(k) else None}
+                        for k, v in work.items()},
+        }
 
     hits_s = args.hits / t32
     line = {
@@ -299,27 +405,28 @@ def main():
         "fp32_over_fp64": t64 / t32,
         "flux_nrmse_fp32_vs_fp64": nrmse,
         "kernel_ms": {k: round(v, 3) for k, v in kms.items()},
-        "roofline": {
-            "bound": "mfma",
-            "pipe": "fp32 vector ALU (the ray algebra is not a contraction; packed FP32 is not used, "
-                    "so the reachable FP32 FMA rate equals the 78.6 TF FP64 rate)",
-            "kernel": dom,
-            "achieved": achieved,
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": traffic,
-            "traffic_gbps": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
-            "flops_per_launch": flops_per_launch,
-            "avg_launch_ms": avg_ms,
-            "events_per_run": {k: {kk: vv for kk, vv in v.items() if kk != "flops"} for k, v in model.items()},
-        },
+        "roofline": roof,
         "cpu_baseline": None,
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(lambda n: workloads.tower(n_helio=1000, raycount=n), args.seed)
+    if not args.no_workloads:
+        line["end_to_end"] = _end_to_end(lib, c32, args.seed, sync)
     lib.st_free_context(c32)
     lib.st_free_context(c64)
+    if not args.no_workloads:
+        line["workloads"] = {
+            "trough_fp64": dict(_workload_rate(lib, workloads.trough(raycount=1_000_000), 64, 5, args.seed, sync),
+                                workload="parabolic trough 5.77 x 12 m, f 1.71 m, 70 mm tube, pillbox 4.65 mrad, "
+                                         "1e6 ray hits"),
+            "trough_fp32": _workload_rate(lib, workloads.trough(raycount=1_000_000), 32, 5, args.seed, sync),
+            "faceted_fp32": dict(_workload_rate(lib, workloads.faceted_field(n_helio=10_000, raycount=50_000_000),
+                                                32, 3, args.seed, sync),
+                                 workload="10,000 heliostats x 4 flat 5 x 5 m facets, cylindrical receiver "
+                                          "R 8.5 m x 20 m, Gaussian sun 2.5 mrad, 5e7 ray hits"),
+            "faceted_fp64": _workload_rate(lib, workloads.faceted_field(n_helio=10_000, raycount=50_000_000), 64,
+                                           2, args.seed, sync),
+        }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(lambda n: workloads.tower(n_helio=1000, raycount=n), args.seed)
     print(json.dumps(line))
 
 

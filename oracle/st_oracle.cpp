@@ -48,35 +48,43 @@ struct HitState {
     int back = 0;
 };
 
-static void closest_hit(const Stage &st, const double ps[3], const double cs[3], HitState &h) {
+// Test element j; keep it when it is the closest accepted hit so far
+// (coretrace/raytrace.cpp:722-768).
+static void test_element(const Element &e, size_t j, const double ps[3], const double cs[3], HitState &h) {
+    double pe[3], ce[3];
+    to_local(ps, cs, e.origin, e.R, pe, ce);
+    pe[0] = pe[0] + 1.0e-5 * ce[0];
+    pe[1] = pe[1] + 1.0e-5 * ce[1];
+    pe[2] = pe[2] + 1.0e-5 * ce[2];
+    double X[3], C[3], DF[3], path = 0.0;
+    int err = 0, hit = 0, back = 0;
+    element_intersection(e, pe, ce, X, C, DF, &path, &err, &hit, &back);
+    if (!hit) return;
+    if (path < h.last_path) {
+        if (X[2] <= e.zap || e.surf == 'm' || e.surf == 'M' || e.surf == 'r' || e.surf == 'R') {
+            h.hit = true;
+            h.last_path = path;
+            for (int k = 0; k < 3; k++) {
+                h.pos_el[k] = X[k];
+                h.cos_el[k] = C[k];
+                h.df[k] = DF[k];
+            }
+            h.elem = (int)j + 1;
+            to_ref(X, C, e.origin, e.RT, h.pos_st, h.cos_st);
+            h.back = back;
+        }
+    }
+}
+
+static void closest_hit(const Stage &st, int si, const double ps[3], const double cs[3], HitState &h) {
     h.last_path = 1e99;
     h.hit = false;
     for (size_t j = 0; j < st.elements.size(); j++) {
         const Element &e = *st.elements[j];
         if (!e.enabled) continue;
-        double pe[3], ce[3];
-        to_local(ps, cs, e.origin, e.R, pe, ce);
-        pe[0] = pe[0] + 1.0e-5 * ce[0];
-        pe[1] = pe[1] + 1.0e-5 * ce[1];
-        pe[2] = pe[2] + 1.0e-5 * ce[2];
-        double X[3], C[3], DF[3], path = 0.0;
-        int err = 0, hit = 0, back = 0;
-        element_intersection(e, pe, ce, X, C, DF, &path, &err, &hit, &back);
-        if (!hit) continue;
-        if (path < h.last_path) {
-            if (X[2] <= e.zap || e.surf == 'm' || e.surf == 'M' || e.surf == 'r' || e.surf == 'R') {
-                h.hit = true;
-                h.last_path = path;
-                for (int k = 0; k < 3; k++) {
-                    h.pos_el[k] = X[k];
-                    h.cos_el[k] = C[k];
-                    h.df[k] = DF[k];
-                }
-                h.elem = (int)j + 1;
-                to_ref(X, C, e.origin, e.RT, h.pos_st, h.cos_st);
-                h.back = back;
-            }
-        }
+        ST_OPS_TEST(si);
+        test_element(e, j, ps, cs, h);
+        ST_OPS_POP();
     }
 }
 
@@ -190,6 +198,7 @@ static bool trace(System &sys, uint32_t seed, uint64_t N, uint64_t maxrays, bool
             uint64_t mhc = 0;
             uint64_t sun_index = 0;
             if (si == 0) {
+                ST_OPS_PHASE(15);  // the search of this sun ray, until it is settled
                 sun_index = sys.sunraycount;
                 rng.begin_ray(0, sun_index);
                 double psun[3];
@@ -200,6 +209,7 @@ static bool trace(System &sys, uint32_t seed, uint64_t N, uint64_t maxrays, bool
                     return false;
                 }
             } else {
+                ST_OPS_PHASE(4 + (int)si);
                 const GlobalRay &g = incoming[read++];
                 ray_number = g.num;
                 sun_index = g.sun_index;
@@ -214,7 +224,8 @@ static bool trace(System &sys, uint32_t seed, uint64_t N, uint64_t maxrays, bool
 
             if (cb != nullptr && traced_total++ % per_cb == 0) {
                 if (traced_total > 1) {
-                    double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                    double ms = 1e-3 * (double)std::chrono::duration_cast<std::chrono::microseconds>(
+                        std::chrono::steady_clock::now() - t0).count();
                     double msper = ms / (double)traced_total;
                     per_cb = msper > 0 ? (uint64_t)(200.0 / msper) : 1000000;
                     if (per_cb < 5) per_cb = 5;
@@ -228,7 +239,8 @@ static bool trace(System &sys, uint32_t seed, uint64_t N, uint64_t maxrays, bool
             // ---- Label_MultiHitLoop ----
             for (;;) {
                 HitState h;
-                closest_hit(st, rs.ps, rs.cs, h);
+                closest_hit(st, (int)si, rs.ps, rs.cs, h);
+                if (si == 0 && mhc == 0) ST_OPS_SETTLE(h.hit ? 1 : 0, 2);  // missed or hit; the rest is phase 2
                 bool flag_miss = false;
                 if (!h.hit) {
                     if (si == 0) {

@@ -19,6 +19,15 @@
 // with the scene compiler (pinned against the reference build)
 #include "../soltrace_amd/csrc/st_fetree.h"
 
+// Phase hooks of the operation-counting build (st_count.h); no-ops here.
+#ifndef ST_OPS_PHASE
+#define ST_OPS_PHASE(p) ((void)0)
+#define ST_OPS_SETTLE(to, next) ((void)0)
+#define ST_OPS_PUSH(p) ((void)0)
+#define ST_OPS_POP() ((void)0)
+#define ST_OPS_TEST(si) ((void)0)
+#endif
+
 namespace oracle {
 
 static const double kDegToRad = 0.017453292519943295;  // types.h:64

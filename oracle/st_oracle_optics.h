@@ -47,8 +47,8 @@ inline void cone_direction(double theta2, double phi, const double RT[3][3], dou
 // (a + b x)(lo + x) in x = theta - lo and its CDF from lo the cubic
 // G(x) = x (p0 + x (p1 + x p2)).  The segment holding u * total comes from a
 // bisection over the masses before each segment, x from Newton steps kept in
-// a shrinking bracket (bisecting when a step leaves it) until a step changes
-// nothing.  Tables that are not ascending, or hold negative intensities,
+// a shrinking bracket (bisecting when a step leaves it) until a step moves x
+// by at most 1e-15 of the segment's far end.  Tables that are not ascending, or hold negative intensities,
 // keep the rejection loop.
 inline void sun_icdf_build(Sun &s) {
     s.icdf.clear();
@@ -100,14 +100,24 @@ inline double sun_icdf_theta2(const Sun &s, uint32_t w0, uint32_t w1) {
     if (!(r > 0.0)) r = 0.0;
     double xl = 0.0, xh = w;
     double x = mass > 0.0 ? w * (r / mass) : 0.0;
+    const double tol = 1e-15 * (lo + w);
     for (int it = 0; it < 60; ++it) {
         const double f = x * (p0 + x * (p1 + x * p2)) - r;
         if (f > 0.0) xh = x;
         else xl = x;
         const double d = (av + bv * x) * (lo + x);
-        double xn = d > 0.0 ? x - f / d : 0.5 * (xl + xh);
+        double xn;
+        if (d > 0.0) {
+            xn = x - f / d;
+            if (fabs(xn - x) <= tol) {
+                x = xn < 0.0 ? 0.0 : (xn > w ? w : xn);
+                break;
+            }
+        } else {
+            xn = 0.5 * (xl + xh);
+        }
         if (!(xn > xl && xn < xh)) xn = 0.5 * (xl + xh);
-        if (xn == x || !(xh > xl)) break;
+        if (!(xh > xl)) break;
         x = xn;
     }
     const double th = lo + x;
@@ -150,10 +160,12 @@ inline void errors(Rng &rng, const double cin[3], int source, const Sun &sun, co
             break;
         }
         case 'd': case 'D': {
+            ST_OPS_PUSH(3);
             if (rng.mode() == 1 && !sun.icdf.empty()) {
                 const uint32_t w0 = (uint32_t)(rng() * 4294967296.0);
                 const uint32_t w1 = (uint32_t)(rng() * 4294967296.0);
                 theta2 = sun_icdf_theta2(sun, w0, w1) / 1000000.0;
+                ST_OPS_POP();
                 break;
             }
             const std::vector<double> &ang = sun.angle;
@@ -175,6 +187,7 @@ inline void errors(Rng &rng, const double cin[3], int source, const Sun &sun, co
                 break;
             }
             theta2 = theta2 / 1000000.0;
+            ST_OPS_POP();
             break;
         }
         case 'f': case 'F':

@@ -31,6 +31,12 @@
 #include <cmath>
 #include <cstdint>
 
+// long double, named so the operation-counting build (st_count.h) can keep it
+#ifndef ST_HAVE_LONG_DOUBLE_T
+#define ST_HAVE_LONG_DOUBLE_T
+typedef long double st_long_double;
+#endif
+
 namespace oracle {
 
 // ---------------------------------------------------------------- Philox ---
@@ -82,12 +88,12 @@ public:
         return x_;
     }
     double uniform() {
-        const long double r = 2147483646.0L;  // max - min + 1
+        const st_long_double r = (st_long_double)2147483646.0;  // max - min + 1
         double sum = 0.0;
         double tmp = 1.0;
         for (int k = 0; k < 2; ++k) {
             sum += (double)(engine() - 1ull) * tmp;
-            tmp = (double)((long double)tmp * r);
+            tmp = (double)((st_long_double)tmp * r);
         }
         double ret = sum / tmp;
         if (ret >= 1.0) ret = std::nextafter(1.0, 0.0);

@@ -1449,7 +1449,8 @@ STD int sunshape_d_group(uint32_t seed, uint64_t sun_index, uint32_t k0, uint32_
 // doubles each: lo, width w, a, b, p0, p1, p2, and C, the mass of the
 // segments before it; the total mass follows the last segment.  The draw
 // picks the segment holding u * total by bisection over C and solves G(x) = r
-// by Newton steps kept inside a shrinking bracket, to the last bit.  Only
+// by Newton steps kept inside a shrinking bracket, until a step is within a
+// few ulps of theta (about 3 steps).  Only
 // +, -, *, / are used, so an FP64 build without contraction gives the same
 // bits as the host.  Returns theta^2 in mrad^2.
 STD double sunshape_icdf_solve(const double *seg, double r) {
@@ -1460,14 +1461,24 @@ STD double sunshape_icdf_solve(const double *seg, double r) {
     if (!(r > 0.0)) r = 0.0;
     double xl = 0.0, xh = w;
     double x = m > 0.0 ? w * (r / m) : 0.0;
+    const double tol = 1e-15 * (lo + w);  // a few ulps of theta
     for (int it = 0; it < 60; ++it) {
         const double f = x * (p0 + x * (p1 + x * p2)) - r;
         if (f > 0.0) xh = x;
         else xl = x;
         const double d = (a + b * x) * (lo + x);
-        double xn = d > 0.0 ? x - f / d : 0.5 * (xl + xh);
+        double xn;
+        if (d > 0.0) {
+            xn = x - f / d;
+            if (::fabs(xn - x) <= tol) {  // converged: take the last step
+                x = xn < 0.0 ? 0.0 : (xn > w ? w : xn);
+                break;
+            }
+        } else {
+            xn = 0.5 * (xl + xh);
+        }
         if (!(xn > xl && xn < xh)) xn = 0.5 * (xl + xh);
-        if (xn == x || !(xh > xl)) break;
+        if (!(xh > xl)) break;
         x = xn;
     }
     return lo + x;
