@@ -76,6 +76,36 @@ int st_amd_element_stats_noalloc(st_context_t pcxt, int stage, int elem, int nx,
                                  int absorbed_only, double dni, double bounds[4], unsigned int *grid,
                                  double *stats);
 
+/* Trace n contexts' scenes together, context k with seed seeds[k], each with
+ * the records, ray numbers and sun-ray count its own st_sim_run(ctxs[k],
+ * seeds[k], NULL, NULL) gives (coretrace/stapi.cpp:680-734); the results
+ * are read with the usual getters of each context.  No counterpart in the
+ * reference, whose callers trace contexts one by one or on host threads
+ * (app/src/trace.cpp:709-730).  Every kernel launch covers all the scenes:
+ * persistent sun-mask and trace launches drain a device queue of per-wave
+ * tiles drawn from every scene, so scenes of uneven size fill the GPU
+ * together.  The trace launches' tile size and waves per SIMD are picked
+ * per scene class by timing the candidates on the first launch of that
+ * class and kept for the process.  All contexts must be on one device with
+ * one precision, without flux requests or stage data; callbacks are not
+ * called.  results[k] (may be NULL) receives each scene's record count or
+ * -1 (messages on that context).  Returns 1, or -1 when any scene failed or
+ * the arguments are invalid. */
+int st_amd_sim_run_batch(st_context_t *ctxs, int n, const unsigned int *seeds, int *results);
+
+/* Times and counts of the last batched run whose first context is pcxt:
+ * [0] sun-mask ms, [1] sunshape ms, [2] wall ms, [3] sun rays, [4] mask
+ * launches, [5] sunshape launches, [6..21] trace ms per stage, [22..37]
+ * trace launches per stage, [38..53] rays per stage, [54..69] records per
+ * stage.  Writes min(n, 70) values; returns 70, or -1 before any batch. */
+int st_amd_batch_timing(st_context_t pcxt, double *out, int n);
+
+/* The batch autotuner's choices so far, three ints each: class key
+ * (precision bits * 100 + kernel variant * 10 + 1 for stages after the
+ * first), rays per tile, waves per SIMD.  Writes min(n/3) entries; returns
+ * the number of classes tuned. */
+int st_amd_batch_tuning(int *out, int n);
+
 /* Read a SolTrace .stinput system into the context with the headless strace
  * reader (coretrace/strace/main.cpp:358-397).  Returns 1, or -1 when the file
  * cannot be opened or is malformed. */

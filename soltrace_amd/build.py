@@ -18,7 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("SOLTRACE_AMD_ARCH", "gfx950")
 
-SOURCES = ["st_trace_f64.hip", "st_trace_f32.hip", "st_compact.hip", "st_stats.hip", "st_scene.cpp", "st_context.cpp"]
+SOURCES = ["st_trace_f64.hip", "st_trace_f32.hip", "st_compact.hip", "st_stats.hip", "st_scene.cpp", "st_context.cpp",
+           "st_batch.cpp"]
 # The FP64 trace keeps separate multiply/add roundings (no FMA contraction),
 # the arithmetic the reference's CPU build performs.
 EXTRA = {"st_trace_f64.hip": ["-ffp-contract=off"], "st_trace_f32.hip": ["-ffp-contract=fast"],

@@ -201,4 +201,76 @@ void launch_compact_rays(const DevRay *slots, const uint32_t *flag, const uint32
                        tile_off, n_rays, out);
 }
 
+// ---------------------------------------------------- batched scenes ----
+__device__ __forceinline__ int batch_item_of(const BatchQueue &q, uint32_t x) {
+    int lo = 0, hi = q.n - 1;  // the last item whose first tile is <= x
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (q.prefix[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void prefix_at_kernel(const uint32_t *offsets, const uint32_t *tile_off, size_t n, const uint32_t *total,
+                                 const uint32_t *pos, int m, uint32_t *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const size_t p = pos[i];
+    out[i] = p < n ? offsets[p] + tile_off[p / kScanTile] : *total;
+}
+void launch_prefix_at(const uint32_t *offsets, const uint32_t *tile_off, size_t n, const uint32_t *total,
+                      const uint32_t *pos, int m, uint32_t *out, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(prefix_at_kernel, dim3((m + 255) / 256), dim3(256), 0, s, offsets, tile_off, n, total, pos, m,
+                       out);
+}
+
+__global__ void mask_compact_batch_kernel(const uint64_t *mask, const uint32_t *offsets, const uint32_t *tile_off,
+                                          size_t nwords, const MaskCompactItem *items, BatchQueue q) {
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwords) return;
+    const int k = batch_item_of(q, (uint32_t)w);
+    const uint32_t w0 = q.prefix[k];
+    const MaskCompactItem it = items[k];
+    uint64_t m = mask[w];
+    // rank of the word's first hit within its scene
+    uint32_t o = (offsets[w] + tile_off[w / kScanTile]) - (offsets[w0] + tile_off[w0 / kScanTile]);
+    while (m != 0 && o < it.take) {
+        const int b = __ffsll((unsigned long long)m) - 1;
+        it.dst[o++] = it.s0 + (uint64_t)(w - w0) * 64 + (uint64_t)b;
+        m &= m - 1;
+    }
+}
+void launch_mask_compact_batch(const uint64_t *mask, const uint32_t *offsets, const uint32_t *tile_off,
+                               size_t nwords, const MaskCompactItem *items, const BatchQueue &q, hipStream_t s) {
+    if (nwords == 0) return;
+    hipLaunchKernelGGL(mask_compact_batch_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s, mask,
+                       offsets, tile_off, nwords, items, q);
+}
+
+__global__ void copy_rays_batch_kernel(const RayCopyItem *items, BatchQueue q, uint32_t total) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int k = batch_item_of(q, i);
+    const RayCopyItem it = items[k];
+    const uint32_t j = i - q.prefix[k];
+    it.dst[j] = it.src[j];
+}
+void launch_copy_rays_batch(const RayCopyItem *items, const BatchQueue &q, uint32_t total, hipStream_t s) {
+    if (total == 0) return;
+    hipLaunchKernelGGL(copy_rays_batch_kernel, dim3((total + 255) / 256), dim3(256), 0, s, items, q, total);
+}
+
+__global__ void last_raynum_kernel(const LastRayItem *items, int m, uint32_t *out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const LastRayItem it = items[k];
+    out[k] = it.n ? it.rays[it.n - 1].raynum : 0u;
+}
+void launch_last_raynum(const LastRayItem *items, int m, uint32_t *out, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(last_raynum_kernel, dim3((m + 255) / 256), dim3(256), 0, s, items, m, out);
+}
+
 }  // namespace st

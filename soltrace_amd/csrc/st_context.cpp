@@ -8,65 +8,9 @@
 // traces every stage on the GPU and leaves the records in HBM; the result
 // getters copy them out in the reference's TRayData::Merge order
 // (coretrace/types.cpp:311-355).
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <chrono>
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <memory>
-#include <mutex>
-#include <string>
-#include <vector>
-
-#include "st_kernels.h"
-#include "st_scene.h"
-
-#define STCORE_API extern "C" __attribute__((visibility("default")))
-
-typedef unsigned long st_uint_t;
-typedef void *st_context_t;
-typedef int (*st_callback_t)(st_uint_t, st_uint_t, st_uint_t, st_uint_t, st_uint_t, void *);
+#include "st_context.h"
 
 namespace st {
-
-static void hip_check(hipError_t e, const char *what) {
-    if (e != hipSuccess) {
-        char b[256];
-        snprintf(b, sizeof b, "HIP error in %s: %s", what, hipGetErrorString(e));
-        throw std::runtime_error(b);
-    }
-}
-#define HC(x) hip_check((x), #x)
-
-// Device buffer that grows on demand and keeps its capacity.
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    void *ensure(size_t bytes) {
-        if (bytes <= cap && p) return p;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        size_t want = std::max<size_t>(bytes, 256);
-        HC(hipMalloc(&p, want));
-        cap = want;
-        return p;
-    }
-    template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
-};
-
-template <typename T>
-static T *upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
-    b.ensure(std::max<size_t>(1, v.size()) * sizeof(T));
-    if (!v.empty()) HC(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
-    return b.as<T>();
-}
 
 // Scratch of st_amd_element_stats_noalloc, the allocation-free element
 // statistics: bin counts (at most kNoAllocMaxBins u32), then the centroid
@@ -84,7 +28,7 @@ static void *scratch_dev[kMaxDevices] = {nullptr};
 
 // The device address of `scratch` on GPU `dev`, looked up once per device
 // with that device current.
-static void *scratch_on(int dev) {
+void *scratch_on(int dev) {
     if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device ordinal out of range");
     std::lock_guard<std::mutex> lk(scratch_mu);
     if (!scratch_dev[dev]) {
@@ -100,122 +44,6 @@ static void *scratch_on(int dev) {
     return scratch_dev[dev];
 }
 
-struct StageRecords {
-    std::vector<std::shared_ptr<DevBuf>> chunks;
-    std::vector<size_t> counts;
-    size_t total = 0;
-};
-
-struct Timing {
-    double mask_ms = 0, trace_ms[kMaxStages] = {0}, other_ms = 0, total_ms = 0, sunpre_ms = 0;
-    uint64_t sun_rays = 0, rays_in[kMaxStages] = {0}, records[kMaxStages] = {0};
-    uint64_t mask_launches = 0, trace_launches[kMaxStages] = {0};
-    // event counters per kernel: [0] sun mask, [1 + stage] trace
-    unsigned long long ev[1 + kMaxStages][5] = {{0}};
-    unsigned long long ev_sun[2] = {0, 0};  // sampler: tries evaluated, tries needed
-};
-
-struct Context {
-    SceneModel model;
-    int raycount = 1000, raymax = 100000;
-    bool dyn_group = false, err_sun = true, err_opt = true;
-    int precision = 64;
-    bool cull = true;
-    bool count_events = false;
-    DevBuf d_counters;
-    std::vector<std::string> messages;
-    hipStream_t stream = nullptr;
-    hipStream_t mask_stream = nullptr;  // next batch's sun mask, beside the current batch's trace
-    hipEvent_t ev_hits[2] = {nullptr, nullptr}, ev_ready = nullptr, ev_tot = nullptr;
-    uint32_t *h_tot = nullptr;  // pinned: per-launch totals and error flags
-    int device = 0;
-
-    CompiledScene cs;
-    bool dirty = true;            // a setter ran since the last compile
-    bool compiled_cull = true;
-    SceneView scene_view{};
-    DevBuf d_elems, d_faces, d_aux, d_stages, d_bvh, d_bvh_items, d_grid_cells, d_grid_items, d_foot;
-    // run scratch
-    DevBuf d_mask_scan_tmp, d_mask_total;
-    DevBuf d_mask, d_popc, d_offs, d_scan_tmp, d_scan_tmp2, d_total, d_hits, d_rec_slots, d_rec_count, d_rec_offs, d_ovf_rec,
-        d_ovf_meta, d_ovf_count, d_out_slots, d_out_flag, d_out_offs, d_flux;
-    DevBuf d_rays_a, d_rays_b, d_sunpre, d_sunpre_batch[2];
-
-    // results
-    std::vector<StageRecords> recs;
-    // record buffers of earlier runs, reused so a run allocates no device
-    // memory once the buffers have grown to the scene's size
-    std::vector<std::shared_ptr<DevBuf>> rec_pool;
-    std::vector<hipEvent_t> ev_pool;  // kernel timing events, reused
-    uint64_t sun_ray_count = 0;
-    bool host_valid = false;
-    std::vector<DevRecord> host_recs;  // merged order
-    uint64_t n_hits_done = 0;
-
-    FluxSpec flux{};
-    std::vector<uint32_t> flux_host;
-    Timing timing;
-
-    // stage-data save/resume (coretrace/stapi.cpp:680-728): rows to load in
-    // the next run, and the rows the last run saved
-    const double *load_s1 = nullptr, *load_s2 = nullptr;
-    uint64_t load_n1 = 0, load_n2 = 0;
-    bool save_data = false;
-    std::vector<double> saved_s1, saved_s2;  // 8 and 7 doubles a row
-
-    void *scratch_ptr = nullptr;  // this context's GPU's copy of `scratch`
-
-    Context() {
-        HC(hipGetDevice(&device));
-        scratch_ptr = scratch_on(device);
-        HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        HC(hipStreamCreateWithFlags(&mask_stream, hipStreamNonBlocking));
-        for (auto &e : ev_hits) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        HC(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
-        HC(hipEventCreateWithFlags(&ev_tot, hipEventDisableTiming));
-        HC(hipHostMalloc((void **)&h_tot, 16 * sizeof(uint32_t), hipHostMallocDefault));
-    }
-    ~Context() {
-        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
-        for (auto e : ev_hits)
-            if (e) (void)hipEventDestroy(e);
-        if (ev_ready) (void)hipEventDestroy(ev_ready);
-        if (ev_tot) (void)hipEventDestroy(ev_tot);
-        if (h_tot) (void)hipHostFree(h_tot);
-        if (mask_stream) (void)hipStreamDestroy(mask_stream);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-    void errlog(const char *fmt, ...) {
-        char b[513];
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(b, 512, fmt, ap);
-        va_end(ap);
-        messages.push_back(b);
-    }
-    void clear_results() {
-        for (auto &st : recs)
-            for (auto &c : st.chunks) rec_pool.push_back(c);
-        recs.clear();
-        host_valid = false;
-        host_recs.clear();
-        n_hits_done = 0;
-        sun_ray_count = 0;
-    }
-    int run(uint32_t seed, st_callback_t cb, void *cbdata);
-    template <typename R> int run_t(uint32_t seed, st_callback_t cb, void *cbdata);
-    void materialize();
-    size_t total_records() const {
-        size_t n = 0;
-        for (const auto &s : recs) n += s.total;
-        return n;
-    }
-};
-
-static Context *ctx_of(st_context_t p) { return reinterpret_cast<Context *>(p); }
-
-static const uint32_t kMaxChunk = 1u << 22;   // rays per trace launch
-static const int kSlots = 4;                  // record slots per ray per launch
 
 void Context::materialize() {
     if (host_valid) return;
@@ -228,7 +56,7 @@ void Context::materialize() {
         size_t off = 0;
         for (size_t c = 0; c < recs[s].chunks.size(); c++) {
             if (recs[s].counts[c])
-                HC(hipMemcpyAsync(per[s].data() + off, recs[s].chunks[c]->p, recs[s].counts[c] * sizeof(DevRecord),
+                HC(hipMemcpyAsync(per[s].data() + off, recs[s].ptr(c), recs[s].counts[c] * sizeof(DevRecord),
                                   hipMemcpyDeviceToHost, stream));
             off += recs[s].counts[c];
         }
@@ -250,53 +78,35 @@ void Context::materialize() {
     host_valid = true;
 }
 
-int Context::run(uint32_t seed, st_callback_t cb, void *cbdata) {
-    try {
-        return precision == 32 ? run_t<float>(seed, cb, cbdata) : run_t<double>(seed, cb, cbdata);
-    } catch (const std::exception &e) {
-        errlog("trace error: %s", e.what());
-        return -1;
-    }
-}
-
-template <typename R>
-int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
-    auto t_start = std::chrono::steady_clock::now();
-    clear_results();
-    timing = Timing();
+// Compile the scene when a setter changed it (geometry set-up, cull
+// structures), check the run's inputs, and upload the device tables: the
+// reference's InitGeometries, then Trace's checks and SunToPrimaryStage
+// (coretrace/stapi.cpp:699-712, raytrace.cpp:137-250), in its order.
+bool Context::prepare() {
     std::string msg;
-    // The scene is compiled (geometry set-up, cull structures, device tables)
-    // only when a setter changed it since the last successful compile.
     const bool recompile = dirty || compiled_cull != cull;
     if (recompile && !compile_scene(model, cull, cs, msg)) {
         errlog("%s", msg.c_str());
-        return -1;
+        return false;
     }
-    // rays to trace: the number of loaded stage-2 rays when there are any
-    // (coretrace/stapi.cpp:697-700)
     const int rayct = load_n2 > 0 ? (int)load_n2 : raycount;
-    const bool load = load_n1 > 0 && load_n2 > 0;  // coretrace/raytrace.cpp:159-163
-    saved_s1.clear();
-    saved_s2.clear();
-    const uint64_t N = (uint64_t)(int64_t)rayct;
-    const uint64_t maxrays = (uint64_t)(int64_t)raymax;
     if ((int64_t)rayct < 1) {
         errlog("invalid number of rays: %d", rayct);
-        return -1;
+        return false;
     }
     if (model.stages.empty()) {
         errlog("no stages defined.");
-        return -1;
+        return false;
     }
     if ((int)model.stages.size() > kMaxStages) {
         errlog("at most %d stages are supported", (int)kMaxStages);
-        return -1;
+        return false;
     }
     hipStream_t s = stream;
     if (recompile) {
         if (!compile_sun(model, cs, msg)) {
             errlog("%s", msg.c_str());
-            return -1;
+            return false;
         }
         SceneView &v = scene_view;
         v = SceneView{};
@@ -317,6 +127,36 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         dirty = false;
         compiled_cull = cull;
     }
+    return true;
+}
+
+int Context::run(uint32_t seed, st_callback_t cb, void *cbdata) {
+    try {
+        return precision == 32 ? run_t<float>(seed, cb, cbdata) : run_t<double>(seed, cb, cbdata);
+    } catch (const std::exception &e) {
+        errlog("trace error: %s", e.what());
+        return -1;
+    }
+}
+
+template <typename R>
+int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
+    auto t_start = std::chrono::steady_clock::now();
+    clear_results();
+    timing = Timing();
+    std::string msg;
+    // The scene is compiled (geometry set-up, cull structures, device tables)
+    // only when a setter changed it since the last successful compile.
+    if (!prepare()) return -1;
+    // rays to trace: the number of loaded stage-2 rays when there are any
+    // (coretrace/stapi.cpp:697-700)
+    const int rayct = load_n2 > 0 ? (int)load_n2 : raycount;
+    const bool load = load_n1 > 0 && load_n2 > 0;  // coretrace/raytrace.cpp:159-163
+    saved_s1.clear();
+    saved_s2.clear();
+    const uint64_t N = (uint64_t)(int64_t)rayct;
+    const uint64_t maxrays = (uint64_t)(int64_t)raymax;
+    hipStream_t s = stream;
     const SceneView sv = scene_view;
 
     // total[0..1]: scan totals; total[3]: error flags, read with the totals;
@@ -328,15 +168,19 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     launch_fill_u32(err, 1, 0u, s);
     launch_fill_u32(total + 4, 2, 0xFFFFFFFFu, s);
     uint32_t *ovf_count = (uint32_t *)d_ovf_count.ensure(sizeof(uint32_t));
-    const uint32_t ovf_cap = kMaxChunk / 2;
+    // Per-launch scratch sized for this run: no launch takes more than N
+    // rays (stage-1 hits or rays entering a later stage), so a context of a
+    // small scene holds little device memory, however many contexts exist.
+    const uint32_t launch_cap = (uint32_t)std::min<uint64_t>(kMaxChunk, std::max<uint64_t>(N, 1));
+    const uint32_t ovf_cap = std::max<uint32_t>(launch_cap / 2, std::min<uint32_t>(kMaxChunk / 2, 65536u));
     d_ovf_rec.ensure((size_t)ovf_cap * sizeof(DevRecord));
     d_ovf_meta.ensure((size_t)ovf_cap * 2 * sizeof(uint32_t));
-    d_rec_slots.ensure((size_t)kMaxChunk * kSlots * sizeof(DevRecord));
-    d_rec_count.ensure((size_t)kMaxChunk * sizeof(uint32_t));
-    d_rec_offs.ensure((size_t)kMaxChunk * sizeof(uint32_t));
-    d_out_slots.ensure((size_t)kMaxChunk * sizeof(DevRay));
-    d_out_flag.ensure((size_t)kMaxChunk * sizeof(uint32_t));
-    d_out_offs.ensure((size_t)kMaxChunk * sizeof(uint32_t));
+    d_rec_slots.ensure((size_t)launch_cap * kSlots * sizeof(DevRecord));
+    d_rec_count.ensure((size_t)launch_cap * sizeof(uint32_t));
+    d_rec_offs.ensure((size_t)launch_cap * sizeof(uint32_t));
+    d_out_slots.ensure((size_t)launch_cap * sizeof(DevRay));
+    d_out_flag.ensure((size_t)launch_cap * sizeof(uint32_t));
+    d_out_offs.ensure((size_t)launch_cap * sizeof(uint32_t));
     d_scan_tmp.ensure(scan_tmp_words((size_t)1 << 26) * sizeof(uint32_t));
     d_scan_tmp2.ensure(scan_tmp_words((size_t)1 << 26) * sizeof(uint32_t));
 
@@ -465,6 +309,8 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         a.inc_sun = err_sun;
         a.inc_err = err_opt;
         a.n_rays = n;
+        a.ray_base = 0;
+        a.slot_stride = (uint32_t)n;
         a.slots = kSlots;
         a.sun_index = sun_idx;
         a.raynum_base = raynum_base;
@@ -485,7 +331,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         a.generic = cs.stages[si].generic;
         a.sun_pre = pre;
         if (si == 0 && presample && pre == nullptr) {
-            SunPre *out = (SunPre *)d_sunpre.ensure((size_t)kMaxChunk * sizeof(SunPre));
+            SunPre *out = (SunPre *)d_sunpre.ensure((size_t)launch_cap * sizeof(SunPre));
             launch_presample(s, sun_idx, n, out);
             a.sun_pre = out;
         }
@@ -517,9 +363,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         if (tot[1] && p.next_rays)
             launch_compact_rays(a.out_slots, a.out_flag, d_out_offs.as<uint32_t>(), d_scan_tmp2.as<uint32_t>(), n,
                                 p.next_rays + *p.n_next, s);
-        recs[si].chunks.push_back(chunk);
-        recs[si].counts.push_back(tot[0]);
-        recs[si].total += tot[0];
+        recs[si].add(chunk, 0, tot[0], true);
         timing.records[si] += tot[0];
         if (p.n_next) *p.n_next += tot[1];
         timing.other_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p.t0).count();
@@ -652,8 +496,8 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             bool ok = true;
             Ready next_b{};
             bool have_next = false;
-            for (uint32_t c = 0; c < cur_b.take && ok; c += kMaxChunk) {
-                const int n = (int)std::min<uint32_t>(kMaxChunk, cur_b.take - c);
+            for (uint32_t c = 0; c < cur_b.take && ok; c += launch_cap) {
+                const int n = (int)std::min<uint32_t>(launch_cap, cur_b.take - c);
                 // a single stage hands no rays on
                 Pending p = trace_launch(0, n, cur_b.hit_list + c, (uint32_t)(cur_b.raynum_base + c), nullptr, 0,
                                          nst > 1 ? next : nullptr, &n_next,
@@ -724,9 +568,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         HC(hipMemcpyAsync(chunk->p, r0.data(), r0.size() * sizeof(DevRecord), hipMemcpyHostToDevice, s));
         HC(hipMemcpyAsync(next, in.data(), in.size() * sizeof(DevRay), hipMemcpyHostToDevice, s));
         HC(hipStreamSynchronize(s));
-        recs[0].chunks.push_back(chunk);
-        recs[0].counts.push_back(r0.size());
-        recs[0].total += r0.size();
+        recs[0].add(chunk, 0, r0.size(), true);
         if (fs.active && fs.stage == 0) {
             // a traced stage is binned into the flux map at record time;
             // loaded records are binned here, each ray's last record on the
@@ -748,7 +590,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         size_t off = 0;
         for (size_t c = 0; c < recs[0].chunks.size(); c++) {
             if (recs[0].counts[c])
-                HC(hipMemcpyAsync(r0.data() + off, recs[0].chunks[c]->p, recs[0].counts[c] * sizeof(DevRecord),
+                HC(hipMemcpyAsync(r0.data() + off, recs[0].ptr(c), recs[0].counts[c] * sizeof(DevRecord),
                                   hipMemcpyDeviceToHost, s));
             off += recs[0].counts[c];
         }
@@ -796,8 +638,8 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         DevRay *nxt = hand_on ? (DevRay *)bufs[which ^ 1]->ensure(std::max<uint64_t>(n_cur, 1) * sizeof(DevRay))
                               : nullptr;
         uint64_t n_out = 0;
-        for (uint64_t c = 0; c < n_cur; c += kMaxChunk) {
-            int n = (int)std::min<uint64_t>(kMaxChunk, n_cur - c);
+        for (uint64_t c = 0; c < n_cur; c += launch_cap) {
+            int n = (int)std::min<uint64_t>(launch_cap, n_cur - c);
             if (!trace_chunk((int)si, n, nullptr, 0, cur + c, last_prev, nxt, &n_out)) return -1;
             if (cb && !cb((st_uint_t)traced_total, (st_uint_t)(c + n), (st_uint_t)last_prev, (st_uint_t)si + 1,
                           (st_uint_t)nst, cbdata)) {
@@ -1458,7 +1300,7 @@ static int element_stats_impl(Context *c, int stage, int elem, int nx, int ny, i
                                             host_order_key(-1e199)};
         HC(hipMemcpyAsync(d_mm, init, sizeof init, hipMemcpyHostToDevice, s));
         for (size_t k = 0; k < sr.chunks.size(); k++)
-            launch_stats_bounds(sr.chunks[k]->as<DevRecord>(), (uint32_t)sr.counts[k], elem + 1, f, d_mm, s);
+            launch_stats_bounds(sr.ptr(k), (uint32_t)sr.counts[k], elem + 1, f, d_mm, s);
         unsigned long long out[4];
         HC(hipMemcpyAsync(out, d_mm, sizeof out, hipMemcpyDeviceToHost, s));
         HC(hipStreamSynchronize(s));
@@ -1498,7 +1340,7 @@ static int element_stats_impl(Context *c, int stage, int elem, int nx, int ny, i
     launch_fill_u32((uint32_t *)d_acc, 12, 0u, s);
     uint64_t base = 0;
     for (size_t k = 0; k < sr.chunks.size(); k++) {
-        launch_stats_bin(sr.chunks[k]->as<DevRecord>(), (uint32_t)sr.counts[k], elem + 1, f, base, d_bins, d_acc, s);
+        launch_stats_bin(sr.ptr(k), (uint32_t)sr.counts[k], elem + 1, f, base, d_bins, d_acc, s);
         base += sr.counts[k];
     }
     unsigned long long a[6];
@@ -1514,7 +1356,7 @@ static int element_stats_impl(Context *c, int stage, int elem, int nx, int ny, i
         size_t k = 0;
         while (k < sr.chunks.size() && idx >= sr.counts[k]) idx -= sr.counts[k++];
         DevRecord r;
-        HC(hipMemcpyAsync(&r, sr.chunks[k]->as<DevRecord>() + idx, sizeof r, hipMemcpyDeviceToHost, s));
+        HC(hipMemcpyAsync(&r, sr.ptr(k) + idx, sizeof r, hipMemcpyDeviceToHost, s));
         HC(hipStreamSynchronize(s));
         // the third row of st_transform_to_local (coretrace/mathproc.cpp)
         const double d[3] = {r.pos[0] - f.org[0], r.pos[1] - f.org[1], r.pos[2] - f.org[2]};

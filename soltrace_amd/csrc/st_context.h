@@ -1,0 +1,212 @@
+// Host side of the HIP backend shared by the C API (st_context.cpp) and the
+// batched multi-scene runs (st_batch.cpp): device buffers, the per-context
+// scene, run state and results.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "st_kernels.h"
+#include "st_scene.h"
+
+#define STCORE_API extern "C" __attribute__((visibility("default")))
+
+typedef unsigned long st_uint_t;
+typedef void *st_context_t;
+typedef int (*st_callback_t)(st_uint_t, st_uint_t, st_uint_t, st_uint_t, st_uint_t, void *);
+
+namespace st {
+
+inline void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        char b[256];
+        snprintf(b, sizeof b, "HIP error in %s: %s", what, hipGetErrorString(e));
+        throw std::runtime_error(b);
+    }
+}
+#define HC(x) hip_check((x), #x)
+
+// Device buffer that grows on demand and keeps its capacity.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void *ensure(size_t bytes) {
+        if (bytes <= cap && p) return p;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes, 256);
+        HC(hipMalloc(&p, want));
+        cap = want;
+        return p;
+    }
+    template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+template <typename T>
+inline T *upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
+    b.ensure(std::max<size_t>(1, v.size()) * sizeof(T));
+    if (!v.empty()) HC(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return b.as<T>();
+}
+
+// The device address of the allocation-free scratch array on GPU `dev`
+// (st_context.cpp).
+void *scratch_on(int dev);
+
+// A stage's records of one run: chunks of device records, each a range of a
+// buffer.  Buffers a context allocated return to its pool when the results
+// are cleared; a batched run's buffers are shared by its contexts and are
+// not pooled.
+struct StageRecords {
+    std::vector<std::shared_ptr<DevBuf>> chunks;
+    std::vector<size_t> offsets;  // first record of each chunk in its buffer
+    std::vector<size_t> counts;
+    std::vector<char> pooled;
+    size_t total = 0;
+    void add(const std::shared_ptr<DevBuf> &b, size_t off, size_t n, bool pool) {
+        chunks.push_back(b);
+        offsets.push_back(off);
+        counts.push_back(n);
+        pooled.push_back(pool ? 1 : 0);
+        total += n;
+    }
+    DevRecord *ptr(size_t k) const { return chunks[k]->as<DevRecord>() + offsets[k]; }
+};
+
+struct Timing {
+    double mask_ms = 0, trace_ms[kMaxStages] = {0}, other_ms = 0, total_ms = 0, sunpre_ms = 0;
+    uint64_t sun_rays = 0, rays_in[kMaxStages] = {0}, records[kMaxStages] = {0};
+    uint64_t mask_launches = 0, trace_launches[kMaxStages] = {0};
+    // event counters per kernel: [0] sun mask, [1 + stage] trace
+    unsigned long long ev[1 + kMaxStages][5] = {{0}};
+    unsigned long long ev_sun[2] = {0, 0};  // sampler: tries evaluated, tries needed
+};
+
+// Kernel times and counts of the last batched run (st_amd_batch_timing).
+struct BatchTiming {
+    double mask_ms = 0, sunshape_ms = 0, total_ms = 0, trace_ms[kMaxStages] = {0};
+    uint64_t sun_rays = 0, mask_launches = 0, sunshape_launches = 0, trace_launches[kMaxStages] = {0};
+    uint64_t rays_in[kMaxStages] = {0}, records[kMaxStages] = {0};
+};
+
+struct BatchWork;  // workspace of batched runs (st_batch.cpp)
+
+struct Context {
+    SceneModel model;
+    int raycount = 1000, raymax = 100000;
+    bool dyn_group = false, err_sun = true, err_opt = true;
+    int precision = 64;
+    bool cull = true;
+    bool count_events = false;
+    DevBuf d_counters;
+    std::vector<std::string> messages;
+    hipStream_t stream = nullptr;
+    hipStream_t mask_stream = nullptr;  // next batch's sun mask, beside the current batch's trace
+    hipEvent_t ev_hits[2] = {nullptr, nullptr}, ev_ready = nullptr, ev_tot = nullptr;
+    uint32_t *h_tot = nullptr;  // pinned: per-launch totals and error flags
+    int device = 0;
+
+    CompiledScene cs;
+    bool dirty = true;            // a setter ran since the last compile
+    bool compiled_cull = true;
+    SceneView scene_view{};
+    DevBuf d_elems, d_faces, d_aux, d_stages, d_bvh, d_bvh_items, d_grid_cells, d_grid_items, d_foot;
+    // run scratch
+    DevBuf d_mask_scan_tmp, d_mask_total;
+    DevBuf d_mask, d_popc, d_offs, d_scan_tmp, d_scan_tmp2, d_total, d_hits, d_rec_slots, d_rec_count, d_rec_offs, d_ovf_rec,
+        d_ovf_meta, d_ovf_count, d_out_slots, d_out_flag, d_out_offs, d_flux;
+    DevBuf d_rays_a, d_rays_b, d_sunpre, d_sunpre_batch[2];
+
+    // results
+    std::vector<StageRecords> recs;
+    // record buffers of earlier runs, reused so a run allocates no device
+    // memory once the buffers have grown to the scene's size
+    std::vector<std::shared_ptr<DevBuf>> rec_pool;
+    std::vector<hipEvent_t> ev_pool;  // kernel timing events, reused
+    uint64_t sun_ray_count = 0;
+    bool host_valid = false;
+    std::vector<DevRecord> host_recs;  // merged order
+    uint64_t n_hits_done = 0;
+
+    FluxSpec flux{};
+    std::vector<uint32_t> flux_host;
+    Timing timing;
+
+    // stage-data save/resume (coretrace/stapi.cpp:680-728): rows to load in
+    // the next run, and the rows the last run saved
+    const double *load_s1 = nullptr, *load_s2 = nullptr;
+    uint64_t load_n1 = 0, load_n2 = 0;
+    bool save_data = false;
+    std::vector<double> saved_s1, saved_s2;  // 8 and 7 doubles a row
+
+    void *scratch_ptr = nullptr;  // this context's GPU's copy of `scratch`
+    std::shared_ptr<BatchWork> batch[2];  // the two lanes of a batched run, kept by its first context
+
+    Context() {
+        HC(hipGetDevice(&device));
+        scratch_ptr = scratch_on(device);
+        HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HC(hipStreamCreateWithFlags(&mask_stream, hipStreamNonBlocking));
+        for (auto &e : ev_hits) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HC(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+        HC(hipEventCreateWithFlags(&ev_tot, hipEventDisableTiming));
+        HC(hipHostMalloc((void **)&h_tot, 16 * sizeof(uint32_t), hipHostMallocDefault));
+    }
+    ~Context() {
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+        for (auto e : ev_hits)
+            if (e) (void)hipEventDestroy(e);
+        if (ev_ready) (void)hipEventDestroy(ev_ready);
+        if (ev_tot) (void)hipEventDestroy(ev_tot);
+        if (h_tot) (void)hipHostFree(h_tot);
+        if (mask_stream) (void)hipStreamDestroy(mask_stream);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void errlog(const char *fmt, ...) {
+        char b[513];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(b, 512, fmt, ap);
+        va_end(ap);
+        messages.push_back(b);
+    }
+    void clear_results() {
+        for (auto &st : recs)
+            for (size_t k = 0; k < st.chunks.size(); k++)
+                if (st.pooled[k]) rec_pool.push_back(st.chunks[k]);
+        recs.clear();
+        host_valid = false;
+        host_recs.clear();
+        n_hits_done = 0;
+        sun_ray_count = 0;
+    }
+    bool prepare();
+    int run(uint32_t seed, st_callback_t cb, void *cbdata);
+    template <typename R> int run_t(uint32_t seed, st_callback_t cb, void *cbdata);
+    void materialize();
+    size_t total_records() const {
+        size_t n = 0;
+        for (const auto &s : recs) n += s.total;
+        return n;
+    }
+};
+
+inline Context *ctx_of(st_context_t p) { return reinterpret_cast<Context *>(p); }
+
+constexpr uint32_t kMaxChunk = 1u << 22;   // rays per trace launch
+constexpr int kSlots = 4;                  // record slots per ray per launch
+
+}  // namespace st

@@ -61,15 +61,18 @@ struct TraceArgs {
     // later-stage input
     const DevRay *in_rays;
     uint32_t last_raynum;        // LastRayNumberInPreviousStage
-    // outputs
-    DevRecord *rec_slots;        // slots * n_rays, slot-major: record j of ray t at [j * n_rays + t]
-    uint32_t *rec_count;         // n_rays
+    // outputs, indexed by the ray's place in the launch's ray space,
+    // ray_base + t (a batched launch lays several scenes' rays end to end)
+    uint32_t ray_base;
+    uint32_t slot_stride;        // rays in the launch's ray space
+    DevRecord *rec_slots;        // slots * slot_stride, slot-major: record j of ray t at [j * slot_stride + ray_base + t]
+    uint32_t *rec_count;         // per ray of the ray space
     DevRecord *ovf_rec;          // overflow records (seq >= slots)
-    uint32_t *ovf_meta;          // (ray, seq) pairs
+    uint32_t *ovf_meta;          // (ray in the ray space, seq) pairs
     uint32_t *ovf_count;
     uint32_t ovf_cap;
-    DevRay *out_slots;           // n_rays (nullptr: no later stage takes the rays)
-    uint32_t *out_flag;          // n_rays (nullptr with out_slots)
+    DevRay *out_slots;           // per ray of the ray space (nullptr: no later stage takes the rays)
+    uint32_t *out_flag;          // per ray of the ray space (nullptr with out_slots)
     uint32_t *err_flags;         // bit 0: bad interaction, 1: phase mismatch, 2: overflow full
                                  // 3: sunshape sampling, 4: finite-element interpolation
     FluxSpec flux;
@@ -93,7 +96,49 @@ struct MaskArgs {
     int32_t compact;             // screen grid candidates into per-wave pair queues
 };
 
+// Sun rays each lane of the grid sun-mask kernel draws (a wave covers
+// 64 * kMaskRays consecutive sun rays).
+constexpr int kMaskRays = 4;
+
+// ----------------------------------------------------- batched scenes ----
+// A batched launch runs one kernel over the work of several scenes.  The
+// work is cut into tiles; items[k] holds scene k's launch arguments and
+// prefix[k] its first tile (prefix[n] is the total).  The launch is
+// persistent: each wave takes the next tile from a device counter until none
+// is left, so scenes of uneven size share the whole chip.
+struct BatchQueue {
+    const uint32_t *prefix;      // n + 1 tile offsets
+    int32_t n;                   // items
+    uint32_t tile;               // trace: rays per tile (a multiple of 64)
+    uint32_t *counter;           // tiles taken; zero before the launch
+};
+
+// Trace-kernel launch shape the batch autotuner chooses per scene class.
+struct TraceShape {
+    int32_t tile;                // rays per tile
+    int32_t waves;               // waves per SIMD the kernel is built for
+};
+
+// Items of the batched sun-mask compaction: the hit sun indices of scene k's
+// mask words, in order, up to `take` of them.
+struct MaskCompactItem {
+    uint64_t s0;                 // sun index of the scene's first mask bit
+    uint64_t *dst;               // where the scene's next hit goes
+    uint32_t take;
+};
+
 template <typename R> void launch_sun_mask(const MaskArgs &a, hipStream_t s);
+// Batched launches (items in device memory, queue counter cleared by the
+// caller).  grid: blocks of the persistent launch.
+template <typename R> void launch_mask_batch(const MaskArgs *items, const BatchQueue &q, int generic, int grid,
+                                             hipStream_t s);
+template <typename R> void launch_sunshape_batch(const SunPreArgs *items, const BatchQueue &q, uint32_t total,
+                                                 hipStream_t s);
+template <typename R> void launch_trace_batch(const TraceArgs *items, const BatchQueue &q, int generic,
+                                              const TraceShape &shape, int grid, hipStream_t s);
+// The waves-per-SIMD builds launch_trace_batch has for a precision and
+// kernel variant (generic), best first by the single-scene measurements.
+int trace_batch_waves(int bits, int generic, int *out, int max);
 template <typename R> void launch_sunshape_d(const SunPreArgs &a, hipStream_t s);
 template <typename R> void launch_trace_stage(const TraceArgs &a, hipStream_t s);
 
@@ -109,6 +154,26 @@ void launch_tile_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *til
 
 // Fill n words with v (a kernel, so it may sit inside a captured graph).
 void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
+// out[i] = exclusive prefix at position pos[i] of a tile scan over n values
+// (the scan's total when pos[i] >= n).
+void launch_prefix_at(const uint32_t *offsets, const uint32_t *tile_off, size_t n, const uint32_t *total,
+                      const uint32_t *pos, int m, uint32_t *out, hipStream_t s);
+// Batched mask compaction: word w of scene k (q.prefix in words) writes its
+// hits' sun indices to items[k].dst, ranked within the scene.
+void launch_mask_compact_batch(const uint64_t *mask, const uint32_t *offsets, const uint32_t *tile_off,
+                               size_t nwords, const MaskCompactItem *items, const BatchQueue &q, hipStream_t s);
+// Copy ray ranges: ray i of item k (q.prefix in rays) from src[k] to dst[k].
+struct RayCopyItem {
+    const DevRay *src;
+    DevRay *dst;
+};
+void launch_copy_rays_batch(const RayCopyItem *items, const BatchQueue &q, uint32_t total, hipStream_t s);
+// out[k] = ray number of the last of items[k]'s n rays (0 when n is 0).
+struct LastRayItem {
+    const DevRay *rays;
+    uint64_t n;
+};
+void launch_last_raynum(const LastRayItem *items, int m, uint32_t *out, hipStream_t s);
 // Mask words -> per-word popcounts.
 void launch_mask_popc(const uint64_t *mask, uint32_t *counts, size_t nwords, hipStream_t s);
 // Hit list: the first `limit` set bits of the mask, as sun-ray indices.

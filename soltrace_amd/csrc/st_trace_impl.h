@@ -228,9 +228,10 @@ STD bool any_hit(const SceneView &sv, const DevStage &st, const double ps[3], co
 }
 
 // ---------------------------------------------------------- sun mask ------
+// Sun ray t of a mask launch tested by one lane (no sun-plane grid); the
+// wave's 64 rays share one mask word.
 template <typename R, int G>
-__global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void mask_lane(const MaskArgs &a, const uint32_t t) {
     bool hit = false;
     if (t < a.n) {
         const DevStage &st = a.sv.stages[0];
@@ -256,6 +257,11 @@ __global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
     }
 }
 
+template <typename R, int G>
+__global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
+    mask_lane<R, G>(a, blockIdx.x * kBlock + threadIdx.x);
+}
+
 // Sun mask over the sun-plane grid, compacted per wave.  About one sun ray in
 // ten hits something, and those rays are spread over every wave, so a
 // lane-per-ray candidate loop runs the full element test in nearly every
@@ -265,15 +271,17 @@ __global__ __launch_bounds__(kBlock) void sun_mask_kernel(MaskArgs a) {
 // time; the pairs that pass go to a per-wave queue in LDS, and the element
 // tests run on the queue, one pair per lane.  A ray hits when any of its
 // pairs does, which is what any_hit() answers for it.
-constexpr int kMaskRays = 4;
 constexpr int kMaskQueue = 128;      // ring of pairs per wave (holds up to 127)
 // a pair packs its ray slot (r * 64 + lane) in 8 bits above a 24-bit element index
 static_assert(kMaskRays * 64 <= 256, "ray slot must fit in 8 bits");
 
 STD uint64_t lanes_below() { return (1ull << (threadIdx.x & 63)) - 1ull; }
 
+// Wave `wave` of a sun-mask launch: sun rays wave * 64 * kMaskRays onwards.
+// Each wave works in its own share of the block's LDS, so waves need not
+// move together.
 template <typename R, int G>
-__global__ __launch_bounds__(kBlock) void sun_mask_grid_kernel(MaskArgs a) {
+__device__ __forceinline__ void mask_grid_wave(const MaskArgs &a, const uint64_t wave) {
     constexpr int W = kBlock / 64;
     __shared__ uint32_t ring[W][kMaskQueue];
     __shared__ double2 pts[W][kMaskRays * 64];
@@ -281,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void sun_mask_grid_kernel(MaskArgs a) {
     __shared__ uint32_t cstart[W][kMaskRays * 64];  // candidate-list starts, lane-major (lane * kMaskRays + r)
     __shared__ int32_t kbase[W][kMaskRays * 64];    // first grid item of each ray slot (r * 64 + lane)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint64_t ray0 = ((uint64_t)blockIdx.x * W + wv) * (64 * kMaskRays);
+    const uint64_t ray0 = wave * (64 * kMaskRays);
     const SceneView &sv = a.sv;
     const DevStage &st = sv.stages[0];
     if (lane < kMaskRays * 2) hitw[wv][lane] = 0u;
@@ -393,6 +401,11 @@ __global__ __launch_bounds__(kBlock) void sun_mask_grid_kernel(MaskArgs a) {
     }
 }
 
+template <typename R, int G>
+__global__ __launch_bounds__(kBlock) void sun_mask_grid_kernel(MaskArgs a) {
+    mask_grid_wave<R, G>(a, (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
+}
+
 // ------------------------------------------- cooperative sunshape draw -----
 // The 'd' sunshape accepts about one rejection iteration in a hundred for a
 // Buie aureole, and a lane-per-ray loop runs a whole wave until its slowest
@@ -494,9 +507,7 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
 // draws at k0 and k0+1 of the ray's stage-0 stream.  One instantiation per
 // precision file, so each takes that file's contraction setting.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void sunshape_icdf_kernel(SunPreArgs a) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= a.n) return;
+__device__ __forceinline__ void sunshape_icdf_one(const SunPreArgs &a, const int i) {
     Rng rng;
     rng.begin(a.seed, 0, a.sun_index[i]);
     rng.seek(a.k0);
@@ -513,6 +524,13 @@ __global__ __launch_bounds__(kBlock) void sunshape_icdf_kernel(SunPreArgs a) {
             atomicAdd(&a.counters[1], (unsigned long long)__popcll(m));
         }
     }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void sunshape_icdf_kernel(SunPreArgs a) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    sunshape_icdf_one<R>(a, i);
 }
 
 template <typename R>
@@ -559,15 +577,10 @@ STD int flux_bin(const FluxSpec &fl, const double pos[3], const double cos[3]) {
     return (gx >= 0 && gx < fl.nx && gy >= 0 && gy < fl.ny) ? gx * fl.ny + gy : -1;
 }
 
-template <typename R, int G, int WV>
-// Waves per SIMD (WV): the generic variant needs the whole register file;
-// the basic FP32 variant runs 4 for the sun stage (128 registers: its spills
-// cost less than the latency the fourth wave hides; 5 waves spill too much)
-// and 3 for later stages, whose rays spend less time waiting on the BVH;
-// the basic FP64 variant 2 (3 is slower).
-__global__ __launch_bounds__(kBlock, G ? 1 : WV) void trace_stage_kernel(TraceArgs a) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= a.n_rays) return;
+// Ray t of a stage-trace launch: the closest-hit loop, records, optics and
+// multi-hit of coretrace/raytrace.cpp:653-1040 for one ray.
+template <typename R, int G>
+__device__ __forceinline__ void trace_one(const TraceArgs &a, const int t) {
     const SceneView &sv = a.sv;
     const DevStage &st = sv.stages[a.si];  // read where used: a copy would hold ~33 scalar registers
     const bool sun_stage = (a.si == 0);
@@ -613,13 +626,14 @@ __global__ __launch_bounds__(kBlock, G ? 1 : WV) void trace_stage_kernel(TraceAr
         r.element = element;
         r.raynum = raynum;
         if (nrec < (uint32_t)a.slots) {
-            last_loc = (int32_t)nrec * a.n_rays + t;  // slot-major: a wave's first records are contiguous
+            // slot-major: a wave's first records are contiguous
+            last_loc = (int32_t)(nrec * a.slot_stride + a.ray_base + (uint32_t)t);
             a.rec_slots[last_loc] = r;
         } else {
             uint32_t o = atomicAdd(a.ovf_count, 1u);
             if (o < a.ovf_cap) {
                 a.ovf_rec[o] = r;
-                a.ovf_meta[2 * o] = (uint32_t)t;
+                a.ovf_meta[2 * o] = a.ray_base + (uint32_t)t;
                 a.ovf_meta[2 * o + 1] = nrec;
                 last_loc = -1 - (int32_t)o;
             } else {
@@ -735,8 +749,9 @@ __global__ __launch_bounds__(kBlock, G ? 1 : WV) void trace_stage_kernel(TraceAr
             break;
         }
     }
-    a.rec_count[t] = nrec;
-    if (a.out_flag != nullptr) a.out_flag[t] = passed ? 1u : 0u;
+    const uint32_t g = a.ray_base + (uint32_t)t;  // place in the launch's ray space
+    a.rec_count[g] = nrec;
+    if (a.out_flag != nullptr) a.out_flag[g] = passed ? 1u : 0u;
     if (a.counters) {
         atomicAdd(&a.counters[0], 1ull);
         atomicAdd(&a.counters[1], (unsigned long long)ev_lds[0][threadIdx.x]);
@@ -754,9 +769,123 @@ __global__ __launch_bounds__(kBlock, G ? 1 : WV) void trace_stage_kernel(TraceAr
             r.pad = 0;
             r.sun_index = sun_stage ? a.sun_index[t] : a.in_rays[t].sun_index;
         }
-        a.out_slots[t] = r;
+        a.out_slots[g] = r;
     }
     if (fbin >= 0 && (!a.flux.absorbed_only || fabs_)) atomicAdd(&a.flux_bins[fbin], 1u);
+}
+
+// Waves per SIMD (WV): the generic variant needs the whole register file;
+// the basic FP32 variant runs 4 for the sun stage (128 registers: its spills
+// cost less than the latency the fourth wave hides; 5 waves spill too much)
+// and 3 for later stages, whose rays spend less time waiting on the BVH;
+// the basic FP64 variant 2 (3 is slower).
+template <typename R, int G, int WV>
+__global__ __launch_bounds__(kBlock, G ? 1 : WV) void trace_stage_kernel(TraceArgs a) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= a.n_rays) return;
+    trace_one<R, G>(a, t);
+}
+
+// ------------------------------------------------------ batched scenes ---
+// The item holding tile x: the last one whose first tile is <= x.
+STD int batch_item(const BatchQueue &q, uint32_t x) {
+    int lo = 0, hi = q.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (q.prefix[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+// The wave's next tile (wave-uniform).
+STD uint32_t wave_take(uint32_t *counter) {
+    uint32_t v = 0;
+    if ((threadIdx.x & 63) == 0) v = atomicAdd(counter, 1u);
+    return (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)v, 0));
+}
+
+// Sun masks of several scenes: a tile is one wave's 64 * kMaskRays sun
+// rays of one scene.
+// Waves per SIMD held to those of the single-scene mask kernels (4 basic, 2
+// generic): the queue loop would otherwise cost registers and occupancy.
+template <typename R, int G>
+__global__ __launch_bounds__(kBlock, G ? 2 : 4) void mask_batch_kernel(const MaskArgs *__restrict__ items,
+                                                                       BatchQueue q) {
+    const uint32_t total = q.prefix[q.n];
+    for (;;) {
+        const uint32_t tile = wave_take(q.counter);
+        if (tile >= total) break;
+        const int k = __builtin_amdgcn_readfirstlane(batch_item(q, tile));
+        const MaskArgs &a = items[k];
+        const uint32_t local = tile - q.prefix[k];
+        if (a.compact) {
+            mask_grid_wave<R, G>(a, local);
+        } else {
+            for (int r = 0; r < kMaskRays; r++)
+                mask_lane<R, G>(a, (local * kMaskRays + (uint32_t)r) * 64u + (threadIdx.x & 63));
+        }
+    }
+}
+
+// Tabulated-sunshape angles of several scenes' hitting rays: one lane per
+// ray (q.prefix in rays).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void sunshape_batch_kernel(const SunPreArgs *items, BatchQueue q,
+                                                                uint32_t total) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    const int k = batch_item(q, i);
+    sunshape_icdf_one<R>(items[k], (int)(i - q.prefix[k]));
+}
+
+// A stage of several scenes: a tile is q.tile rays of one scene, traced by
+// one wave (lane l takes rays l, l + 64, ...).
+template <typename R, int G, int WV>
+__global__ __launch_bounds__(kBlock, G ? 1 : WV) void trace_batch_kernel(const TraceArgs *__restrict__ items,
+                                                                         BatchQueue q) {
+    const uint32_t total = q.prefix[q.n];
+    const uint32_t lane = threadIdx.x & 63;
+    for (;;) {
+        const uint32_t tile = wave_take(q.counter);
+        if (tile >= total) break;
+        const int k = __builtin_amdgcn_readfirstlane(batch_item(q, tile));
+        const TraceArgs &a = items[k];
+        const uint32_t first = (tile - q.prefix[k]) * q.tile;
+        for (uint32_t r = lane; r < q.tile; r += 64) {
+            const uint32_t t = first + r;
+            if (t < (uint32_t)a.n_rays) trace_one<R, G>(a, (int)t);
+        }
+    }
+}
+
+template <typename R>
+void launch_mask_batch(const MaskArgs *items, const BatchQueue &q, int generic, int grid, hipStream_t s) {
+    if (generic == 2) hipLaunchKernelGGL((mask_batch_kernel<R, 2>), dim3(grid), dim3(kBlock), 0, s, items, q);
+    else if (generic) hipLaunchKernelGGL((mask_batch_kernel<R, 1>), dim3(grid), dim3(kBlock), 0, s, items, q);
+    else hipLaunchKernelGGL((mask_batch_kernel<R, 0>), dim3(grid), dim3(kBlock), 0, s, items, q);
+}
+template <typename R>
+void launch_sunshape_batch(const SunPreArgs *items, const BatchQueue &q, uint32_t total, hipStream_t s) {
+    if (total == 0) return;
+    hipLaunchKernelGGL(sunshape_batch_kernel<R>, dim3((total + kBlock - 1) / kBlock), dim3(kBlock), 0, s, items, q,
+                       total);
+}
+template <typename R>
+void launch_trace_batch(const TraceArgs *items, const BatchQueue &q, int generic, const TraceShape &shape, int grid,
+                        hipStream_t s) {
+    if (generic == 2)
+        hipLaunchKernelGGL((trace_batch_kernel<R, 2, 1>), dim3(grid), dim3(kBlock), 0, s, items, q);
+    else if (generic)
+        hipLaunchKernelGGL((trace_batch_kernel<R, 1, 1>), dim3(grid), dim3(kBlock), 0, s, items, q);
+    else if constexpr (sizeof(R) == 8) {
+        if (shape.waves >= 2) hipLaunchKernelGGL((trace_batch_kernel<R, 0, 2>), dim3(grid), dim3(kBlock), 0, s, items, q);
+        else hipLaunchKernelGGL((trace_batch_kernel<R, 0, 1>), dim3(grid), dim3(kBlock), 0, s, items, q);
+    } else {
+        if (shape.waves >= 4) hipLaunchKernelGGL((trace_batch_kernel<R, 0, 4>), dim3(grid), dim3(kBlock), 0, s, items, q);
+        else if (shape.waves == 3)
+            hipLaunchKernelGGL((trace_batch_kernel<R, 0, 3>), dim3(grid), dim3(kBlock), 0, s, items, q);
+        else hipLaunchKernelGGL((trace_batch_kernel<R, 0, 2>), dim3(grid), dim3(kBlock), 0, s, items, q);
+    }
 }
 
 template <typename R>

@@ -212,3 +212,17 @@ def saved_stage_data(lib: StLib, ctx, api=None):
             f(ctx, which, _dp(a))
         out.append(a)
     return tuple(out)
+
+
+def run_batch(lib: StLib, ctxs, seeds) -> tuple[int, list[int]]:
+    """st_amd_sim_run_batch: trace the contexts' scenes together, context k
+    with seeds[k].  Returns (return code, per-context record counts or -1)."""
+    f = lib.dll.st_amd_sim_run_batch
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(st_context_t), C.c_int, C.POINTER(C.c_uint), C.POINTER(C.c_int)]
+    n = len(ctxs)
+    arr = (st_context_t * n)(*ctxs)
+    sd = (C.c_uint * n)(*[int(s) & 0xFFFFFFFF for s in seeds])
+    res = (C.c_int * n)()
+    rc = f(arr, n, sd, res)
+    return rc, list(res)

@@ -19,6 +19,7 @@ benchmarks and parity tests run, built only from their stated parameters.
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 
@@ -178,3 +179,111 @@ def faceted_field(n_helio: int = 10_000, raycount: int = 50_000_000, facet: floa
                     optic="receiver")
     s.stages = [Stage(elements=els), Stage(elements=[recel])]
     return s
+
+
+# -------------------------------------------------------- batched scenes ---
+
+
+def sun_position(lat: float, day: float, hour: float) -> np.ndarray:
+    """Unit vector toward the sun in this module's frame (x east, y north, z
+    up) from latitude, day of year and solar hour, by the formula of the
+    reference's st_sun_position (coretrace/stapi.cpp:500-527), whose frame is
+    x west, y zenith, z north."""
+    d2r = math.pi / 180.0
+    decl = 180.0 / math.pi * math.asin(0.39795 * math.cos(0.98563 * d2r * (day - 173)))
+    ha = 15.0 * (hour - 12.0)
+    el = 180.0 / math.pi * math.asin(math.sin(decl * d2r) * math.sin(lat * d2r) +
+                                      math.cos(decl * d2r) * math.cos(ha * d2r) * math.cos(lat * d2r))
+    az = 180.0 / math.pi * math.acos((math.sin(d2r * decl) * math.cos(d2r * lat) -
+                                      math.cos(d2r * decl) * math.sin(d2r * lat) * math.cos(d2r * ha)) /
+                                     math.cos(d2r * el) + 0.0000000001)
+    if math.sin(ha * d2r) > 0.0:
+        az = 360.0 - az
+    x = -math.sin(az * d2r) * math.cos(el * d2r)
+    y = math.sin(el * d2r)
+    z = math.cos(az * d2r) * math.cos(el * d2r)
+    return np.array([-x, z, y])
+
+
+def zernike_mon(path: str, focal: float, x0: float, y0: float, rng: np.random.Generator, order: int = 6,
+                sigma: float = 1e-4) -> str:
+    """Write a .mon monomial surface file (coretrace/input.cpp:395-414: the
+    order, then B_ij for i = 0..order, j = 0..i, z = sum B_ij x^j y^(i-j))
+    for the patch of the paraboloid z = r^2 / (4 focal) around (x0, y0), in
+    the frame of a facet at (x0, y0, z(x0, y0)) parallel to the axis, with
+    every coefficient perturbed by N(0, sigma)."""
+    vals = []
+    for i in range(order + 1):
+        for j in range(i + 1):
+            b = rng.normal(0.0, sigma)
+            if i == 1:
+                b += (x0 if j == 1 else y0) / (2.0 * focal)
+            if i == 2 and j in (0, 2):
+                b += 1.0 / (4.0 * focal)
+            vals.append(b)
+    with open(path, "w") as f:
+        f.write(f"{order}\n" + "".join(f"{v:.12g}\n" for v in vals))
+    return path
+
+
+def dish(sun: np.ndarray, raycount: int, mon_dir: str, rng: np.random.Generator, focal: float = 4.0,
+         sun_shape: str = "buie") -> Scene:
+    """A sun-tracking dish of focal length `focal`: seven order-6 Zernike
+    facets ('m', 2.2 m circles: one on the axis, six on a ring of 2.2 m, each
+    its patch of the paraboloid with coefficients perturbed by N(0, 1e-4)),
+    a toroidal rim of four 90-degree torus sections ('d', annular radius
+    3.6 m, tube radius 0.25 m, 'a' aperture with A = B = 0, turned by
+    0/90/180/270 degrees), and a flat 1.5 x 1.5 m receiver at the focus.  The
+    stage frame's z axis points at the sun."""
+    s = Scene(sun_xyz=tuple(sun * 1000.0), raycount=raycount, maxcount=2 ** 31 - 1)
+    if sun_shape == "buie":
+        a, v = buie_profile(0.1)
+        s.sun_shape, s.sun_sigma, s.sun_table = "d", 4.65, (a, v)
+    else:
+        s.sun_shape, s.sun_sigma = "p", 4.65
+    s.optics = [Optic("mirror", OpticFace(dist="g", refl=0.93, slope=1.0, spec=0.2),
+                      OpticFace(dist="g", refl=0.0, slope=1.0, spec=0.2)),
+                Optic("receiver", OpticFace(dist="g", refl=0.05), OpticFace(dist="g", refl=0.05))]
+    els = []
+    spots = [(0.0, 0.0)] + [(2.2 * math.cos(q * math.pi / 3), 2.2 * math.sin(q * math.pi / 3)) for q in range(6)]
+    for q, (x, y) in enumerate(spots):
+        z = (x * x + y * y) / (4.0 * focal)
+        mon = zernike_mon(os.path.join(mon_dir, f"dish{id(s)}_{q}.mon"), focal, x, y, rng)
+        els.append(Element(xyz=(x, y, z), aim=(x, y, z + 1.0), aperture="c", ap_params=(2.2,), surface="m",
+                           surf_file=mon, optic="mirror"))
+    for q in range(4):
+        els.append(Element(xyz=(0.0, 0.0, 0.6), aim=(0.0, 0.0, 1.6), zrot=90.0 * q, aperture="a",
+                           ap_params=(0.0, 0.0, 90.0), surface="d", surf_params=(3.6, 0.25), optic="mirror"))
+    recv = Element(xyz=(0.0, 0.0, focal), aim=(0.0, 0.0, 0.0), aperture="r", ap_params=(1.5, 1.5), surface="f",
+                   optic="receiver")
+    s.stages = [Stage(aim=tuple(sun), elements=els), Stage(aim=tuple(sun), elements=[recv])]
+    return s
+
+
+def batch_scenes(n: int = 256, seed: int = 2026, hits=(100_000, 2_000_000), mon_dir: str = ".",
+                 helio=(200, 5000), lat: float = 35.0) -> list:
+    """The batched-scene workload: n scenes drawn from `seed`, each a tower
+    field of U[200, 5000] spherical heliostats (as :func:`tower`, Buie CSR 0.1
+    sun) or a dish (:func:`dish`), with equal odds; the sun from
+    st_sun_position at latitude 35 N, a uniform day of the year and a solar
+    hour with the sun at least 15 degrees up; U[hits] ray hits.  Returns the
+    scenes; .mon files are written to mon_dir."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        while True:
+            day, hour = float(rng.uniform(1, 366)), float(rng.uniform(6, 18))
+            sun = sun_position(lat, day, hour)
+            if sun[2] > math.sin(math.radians(15.0)):
+                break
+        nh = int(rng.integers(hits[0], hits[1] + 1))
+        if rng.random() < 0.5:
+            nhel = int(rng.integers(helio[0], helio[1] + 1))
+            el = math.degrees(math.asin(sun[2]))
+            az = math.degrees(math.atan2(sun[0], sun[1])) % 360.0
+            sc = tower(n_helio=nhel, raycount=nh, r_max=1000.0 * math.sqrt(nhel / 1000.0), sun_elev=el,
+                       sun_azim=az)
+        else:
+            sc = dish(sun, nh, mon_dir, rng)
+        out.append(sc)
+    return out
