@@ -40,6 +40,23 @@ int st_amd_flux_request(st_context_t pcxt, int stage, int elem, int nx, int ny, 
  * NULL) receives xmin, ymin, bin width, bin height.  Returns nx*ny or -1. */
 int st_amd_flux_result(st_context_t pcxt, unsigned int *counts, double *bin_dims);
 
+/* Per-source flux maps (on = 1) during the next runs: beside the map of
+ * st_amd_flux_request, which must name an element of stage 2 or later, keep
+ * one map per stage-1 element.  A ray counts in the map of its source, the
+ * stage-1 element of its last stage-1 record (the reflection that sent it
+ * on), so the maps add up to the requested map bin for bin.  The images a
+ * flux surrogate is trained on: one receiver image per heliostat.  Rays
+ * resumed from loaded stage data carry no source and count in no map.
+ * Returns 1, or -1 for a bad context.  No counterpart in the reference. */
+int st_amd_flux_by_source(st_context_t pcxt, int on);
+
+/* Copy the last run's per-source maps into counts (capacity n words),
+ * counts[(src * nx + ix) * ny + iy] for 0-based stage-1 element src.
+ * counts may be NULL to ask for the size.  Returns the number of words
+ * (stage-1 elements * nx * ny), or -1 when the last run kept none or n is
+ * too small. */
+long long st_amd_flux_source_result(st_context_t pcxt, unsigned int *counts, long long n);
+
 /* The GUI's flux statistics of element `elem` (0-based) of stage `stage`
  * over the last run's records (app/src/project.cpp:1139-1436,
  * ElementStatistics::Compute and BinRaysXY), binned on the GPU where the
@@ -175,6 +192,41 @@ int st_amd_sim_run_data(st_context_t pcxt, unsigned int seed, const double *s1, 
  * Writes the rows to out when it is not NULL; returns the row count, or -1
  * for another `which`. */
 int st_amd_saved_stage_data(st_context_t pcxt, int which, double *out);
+
+/* ---- Flux surrogate network (no counterpart in the reference) ----------
+ * A multilayer perceptron, n_in -> depth x [width, ReLU] -> n_out, with
+ * BF16 weights and activations, FP32 accumulation on the MFMA units, FP32
+ * master weights and Adam, trained and evaluated by the library's own
+ * kernels.  The flux surrogate maps a heliostat's pose, the sun and its aim
+ * offset to its receiver flux image (the per-source maps above are its
+ * training targets).  Data pointers (x, y, t) are DEVICE pointers on the
+ * GPU current at creation, row-major, one row per sample; stream is a
+ * hipStream_t (NULL: the null stream), and every call is ordered on it.
+ * Calls return 1, or -1 with the reason in st_amd_mlp_error(). */
+
+/* n_in <= 64; width and n_out multiples of 64; 1 <= depth <= 14; batches up
+ * to max_batch.  Weights He-uniform from seed, biases zero.  NULL on error. */
+void *st_amd_mlp_create(int n_in, int width, int depth, int n_out, int max_batch, unsigned long long seed);
+void st_amd_mlp_free(void *net);
+const char *st_amd_mlp_error(void);
+long long st_amd_mlp_num_params(void *net);
+/* Parameters in exchange order, host memory: per layer W [n_out][n_in]
+ * row-major, then b [n_out].  which 0: FP32 master values; 1: the BF16
+ * values the kernels multiply with.  set_params also clears Adam's state. */
+int st_amd_mlp_get_params(void *net, float *out, int which);
+int st_amd_mlp_set_params(void *net, const float *in);
+/* y [batch][n_out] (f32) from x [batch][n_in] (f32, rounded to BF16). */
+int st_amd_mlp_forward(void *net, const float *x, int batch, float *y, void *stream);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on the mean squared error
+ * between the network's output for x and targets t [batch][n_out] (BF16
+ * bits).  loss (host, may be NULL: no synchronisation) receives the loss
+ * before the step. */
+int st_amd_mlp_train_step(void *net, const float *x, const unsigned short *t, int batch, float lr, double *loss,
+                          void *stream);
+/* Loss and its gradient with respect to every parameter (host, exchange
+ * order) at the current weights, without a step.  Synchronises. */
+int st_amd_mlp_gradients(void *net, const float *x, const unsigned short *t, int batch, float *grad, double *loss,
+                         void *stream);
 
 #ifdef __cplusplus
 }

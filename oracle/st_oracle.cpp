@@ -13,6 +13,8 @@
 //                                   the GPU kernels draw from;
 //   st_oracle_ray_sun_index(ctx, out)  0-based sun-ray index of each ray;
 //   st_oracle_flux(...)            the GUI flux map of one element;
+//   st_oracle_flux_by_source(...)  that map split by each ray's stage-1
+//                                   source element;
 //   st_oracle_set_torus_start(ctx, on)  start torus solves where the HIP
 //                                   library starts them for far origins.
 // In mode 0 its records are checked bit for bit against oracle/_ref.
@@ -804,8 +806,12 @@ double st_oracle_counter_uniform(uint32_t seed, uint32_t stage, uint64_t id, uin
 // app/src/project.cpp:1139-1436 (ElementStatistics::Compute / BinRaysXY).
 // Bins the LAST record of each ray on (stage, element), absorbed-only when
 // asked; counts[ix*ny+iy].  Ranges are taken as given unless autoscale.
-int st_oracle_flux(st_context_t p, int stage_idx, int elem_idx, int nx, int ny, int absorbed_only, int autoscale,
-                   double *minx, double *maxx, double *miny, double *maxy, double *counts, double *stats) {
+// src_counts (may be NULL): the same binning split by source, the stage-1
+// element of the ray's last stage-1 record, src_counts[(src-1)*nx*ny + bin]
+// for every stage-1 element (the library's st_amd_flux_by_source maps).
+static int oracle_flux(st_context_t p, int stage_idx, int elem_idx, int nx, int ny, int absorbed_only, int autoscale,
+                       double *minx, double *maxx, double *miny, double *maxy, double *counts, double *stats,
+                       double *src_counts) {
     SYS(p, -1);
     if (stage_idx < 0 || stage_idx >= (int)sys->stages.size()) return -1;
     Stage &st = *sys->stages[stage_idx];
@@ -846,6 +852,16 @@ int st_oracle_flux(st_context_t p, int stage_idx, int elem_idx, int nx, int ny, 
     }
     double bx = gx / nx, by = gy / ny;
     for (int i = 0; i < nx * ny; i++) counts[i] = 0.0;
+    std::vector<int> src_of;  // ray number -> source element (1-based), from the stage-1 records in order
+    if (src_counts) {
+        const size_t nsrc = sys->stages[0]->elements.size();
+        for (size_t i = 0; i < nsrc * (size_t)nx * ny; i++) src_counts[i] = 0.0;
+        for (const Record &r : rd)
+            if (r.stage == 1) {
+                if (r.raynum >= src_of.size()) src_of.resize((size_t)r.raynum + 1, 0);
+                src_of[r.raynum] = std::abs(r.element);
+            }
+    }
     std::vector<size_t> idx;
     for (size_t i = 0; i < rd.size(); i++)
         if (std::abs(rd[i].element) == elem_idx + 1 && rd[i].stage == stage_idx + 1) idx.push_back(i);
@@ -874,6 +890,8 @@ int st_oracle_flux(st_context_t p, int stage_idx, int elem_idx, int nx, int ny, 
         if (gxi >= 0 && gxi < nx && gyi >= 0 && gyi < ny) {
             counts[gxi * ny + gyi] += 1;
             binned++;
+            if (src_counts && r.raynum < src_of.size() && src_of[r.raynum] > 0)
+                src_counts[(size_t)(src_of[r.raynum] - 1) * nx * ny + gxi * ny + gyi] += 1;
         }
     }
     if (stats) {
@@ -882,6 +900,19 @@ int st_oracle_flux(st_context_t p, int stage_idx, int elem_idx, int nx, int ny, 
         stats[2] = (double)binned;
     }
     return 1;
+}
+int st_oracle_flux(st_context_t p, int stage_idx, int elem_idx, int nx, int ny, int absorbed_only, int autoscale,
+                   double *minx, double *maxx, double *miny, double *maxy, double *counts, double *stats) {
+    return oracle_flux(p, stage_idx, elem_idx, nx, ny, absorbed_only, autoscale, minx, maxx, miny, maxy, counts,
+                       stats, nullptr);
+}
+// Per-source maps of a flux element on stage 2 or later, fixed ranges.
+int st_oracle_flux_by_source(st_context_t p, int stage_idx, int elem_idx, int nx, int ny, int absorbed_only,
+                             double minx, double maxx, double miny, double maxy, double *counts,
+                             double *src_counts) {
+    if (stage_idx < 1) return -1;
+    return oracle_flux(p, stage_idx, elem_idx, nx, ny, absorbed_only, 0, &minx, &maxx, &miny, &maxy, counts,
+                       nullptr, src_counts);
 }
 
 // The GUI's element statistics in full (app/src/project.cpp:1139-1436):

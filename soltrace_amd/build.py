@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("SOLTRACE_AMD_ARCH", "gfx950")
 
 SOURCES = ["st_trace_f64.hip", "st_trace_f32.hip", "st_compact.hip", "st_stats.hip", "st_scene.cpp", "st_context.cpp",
-           "st_batch.cpp"]
+           "st_batch.cpp", "st_mlp.hip"]
 # The FP64 trace keeps separate multiply/add roundings (no FMA contraction),
 # the arithmetic the reference's CPU build performs.
 EXTRA = {"st_trace_f64.hip": ["-ffp-contract=off"], "st_trace_f32.hip": ["-ffp-contract=fast"],
@@ -58,26 +58,33 @@ HOST_SOURCES = ["stinput.cpp", "stinput_capi.cpp"]
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, only=None) -> str:
+    """Compile every source (in parallel) and link.  `only` (a list of
+    source names) recompiles just those and relinks with the other objects
+    of an earlier build."""
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(BUILD, exist_ok=True)
     cc = hipcc()
-    objs = []
+    cmds, objs = [], []
     for src in SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmd = [cc, *lang, *HIP_FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        _run(cmd)
+        cmds.append((src, [cc, *lang, *HIP_FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]))
         objs.append(obj)
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + INCLUDE, "-c", os.path.join(STRACE, src),
-               "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        _run(cmd)
+        cmds.append((src, ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + INCLUDE, "-c",
+                           os.path.join(STRACE, src), "-o", obj]))
         objs.append(obj)
+    todo = [c for s, c in cmds if only is None or s in only or not os.path.exists(c[-1])]
+    if verbose:
+        for c in todo:
+            print(" ".join(c), flush=True)
+    jobs = max(1, min(len(todo), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(jobs) as ex:
+        for _ in ex.map(_run, todo):
+            pass
     so = os.path.join(HERE, "coretrace_api.so")
     tmp = so + ".tmp"
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])

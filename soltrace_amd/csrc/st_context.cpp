@@ -194,6 +194,13 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         d_flux.ensure((size_t)fs.nx * fs.ny * sizeof(uint32_t));
         launch_fill_u32(d_flux.as<uint32_t>(), (size_t)fs.nx * fs.ny, 0u, s);
     }
+    // per-source maps: one per stage-1 element, for a flux element on a later stage
+    const bool by_src = fs.active && flux_by_source && fs.stage >= 1;
+    const size_t src_words = by_src ? model.stages[0].elements.size() * (size_t)fs.nx * fs.ny : 0;
+    if (by_src) {
+        d_flux_src.ensure(src_words * sizeof(uint32_t));
+        launch_fill_u32(d_flux_src.as<uint32_t>(), src_words, 0u, s);
+    }
 
     // kernel times: events recorded around the launches, read once the run
     // has finished (no synchronisation per launch)
@@ -327,6 +334,8 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         a.err_flags = err;
         a.flux = fs;
         a.flux_bins = fs.active ? d_flux.as<uint32_t>() : nullptr;
+        a.flux_src = by_src ? d_flux_src.as<uint32_t>() : nullptr;
+        a.flux_nbins = (uint32_t)(fs.nx * fs.ny);
         a.counters = ctrs ? ctrs + 5 * (1 + si) : nullptr;
         a.generic = cs.stages[si].generic;
         a.sun_pre = pre;
@@ -561,7 +570,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             DevRay &r = in[j];
             for (int k = 0; k < 3; k++) { r.pos[k] = v[k]; r.cos[k] = v[k + 3]; }
             r.raynum = (uint32_t)(int32_t)v[6];
-            r.pad = 0;
+            r.src = 0;
             r.sun_index = (uint64_t)r.raynum - 1;  // the draws of a loaded ray
         }
         std::shared_ptr<DevBuf> chunk = take_buffer(r0.size() * sizeof(DevRecord));
@@ -680,6 +689,10 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         flux_host.assign((size_t)fs.nx * fs.ny, 0u);
         HC(hipMemcpyAsync(flux_host.data(), d_flux.p, flux_host.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
                           s));
+        flux_src_host.assign(src_words, 0u);
+        if (by_src)
+            HC(hipMemcpyAsync(flux_src_host.data(), d_flux_src.p, src_words * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
         HC(hipStreamSynchronize(s));
     }
     if (ctrs) {
@@ -1456,6 +1469,21 @@ STCORE_API int st_amd_element_stats_noalloc(st_context_t p, int stage, int elem,
     }
 }
 
+STCORE_API int st_amd_flux_by_source(st_context_t p, int on) {
+    CTX(p, -1);
+    c->flux_by_source = on != 0;
+    return 1;
+}
+STCORE_API long long st_amd_flux_source_result(st_context_t p, unsigned int *counts, long long n) {
+    CTX(p, -1);
+    if (!c->flux.active || c->flux_src_host.empty()) return -1;
+    const long long words = (long long)c->flux_src_host.size();
+    if (counts != nullptr) {
+        if (n < words) return -1;
+        memcpy(counts, c->flux_src_host.data(), (size_t)words * sizeof(uint32_t));
+    }
+    return words;
+}
 STCORE_API int st_amd_flux_result(st_context_t p, unsigned int *counts, double *bin_dims) {
     CTX(p, -1);
     if (!c->flux.active) return -1;

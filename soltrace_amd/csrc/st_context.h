@@ -143,6 +143,10 @@ struct Context {
 
     FluxSpec flux{};
     std::vector<uint32_t> flux_host;
+    // per-source maps: one flux map per stage-1 element (st_amd_flux_by_source)
+    bool flux_by_source = false;
+    DevBuf d_flux_src;
+    std::vector<uint32_t> flux_src_host;
     Timing timing;
 
     // stage-data save/resume (coretrace/stapi.cpp:680-728): rows to load in

@@ -77,6 +77,11 @@ struct TraceArgs {
                                  // 3: sunshape sampling, 4: finite-element interpolation
     FluxSpec flux;
     uint32_t *flux_bins;
+    // per-source maps (nullptr: off): stage 0 tags the rays it hands on with
+    // the element that sent them; the flux stage also bins each ray into the
+    // map of its source, flux_src[(src - 1) * flux_nbins + bin]
+    uint32_t *flux_src;
+    uint32_t flux_nbins;
     // event counters (nullptr: off): rays, element tests, BVH nodes,
     // uniforms drawn, records
     unsigned long long *counters;

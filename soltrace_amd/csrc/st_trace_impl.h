@@ -766,12 +766,26 @@ __device__ __forceinline__ void trace_one(const TraceArgs &a, const int t) {
         } else {
             to_ref(ps, cs, st.org, st.R, r.pos, r.cos);
             r.raynum = raynum;
-            r.pad = 0;
+            // source tag: the stage-1 element that sent the ray on (the
+            // element of its last stage-1 record, a reflection), carried
+            // unchanged through later stages
+            r.src = 0;
+            if (a.flux_src != nullptr) {
+                if (!sun_stage) r.src = a.in_rays[t].src;
+                else if (last_loc >= 0) r.src = (uint32_t)a.rec_slots[last_loc].element;
+                else if (last_loc != kNoRecord) r.src = (uint32_t)a.ovf_rec[-1 - last_loc].element;
+            }
             r.sun_index = sun_stage ? a.sun_index[t] : a.in_rays[t].sun_index;
         }
         a.out_slots[g] = r;
     }
-    if (fbin >= 0 && (!a.flux.absorbed_only || fabs_)) atomicAdd(&a.flux_bins[fbin], 1u);
+    if (fbin >= 0 && (!a.flux.absorbed_only || fabs_)) {
+        atomicAdd(&a.flux_bins[fbin], 1u);
+        if (a.flux_src != nullptr && !sun_stage) {
+            const uint32_t src = a.in_rays[t].src;
+            if (src > 0u) atomicAdd(&a.flux_src[(size_t)(src - 1u) * a.flux_nbins + (uint32_t)fbin], 1u);
+        }
+    }
 }
 
 // Waves per SIMD (WV): the generic variant needs the whole register file;

@@ -134,7 +134,7 @@ struct DevRay {
     double pos[3];
     double cos[3];
     uint32_t raynum;
-    uint32_t pad;
+    uint32_t src;           // 1-based stage-1 element that sent the ray on (0: untagged)
     uint64_t sun_index;
 };
 

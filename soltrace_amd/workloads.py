@@ -98,14 +98,44 @@ def sun_vector(elev_deg: float, azim_deg: float) -> np.ndarray:
     return np.array([math.sin(a) * math.cos(e), math.cos(a) * math.cos(e), math.sin(e)])
 
 
+def receiver_axes(pts: np.ndarray, tower_h: float = 150.0):
+    """The tower receiver's centre and two unit vectors in its plane (u
+    horizontal, v up the receiver) along which aim offsets are measured: the
+    receiver faces the field's centroid."""
+    rec = np.array([0.0, 0.0, tower_h])
+    centroid = np.array([pts[:, 0].mean(), pts[:, 1].mean(), 0.0])
+    n = centroid - rec
+    n /= np.linalg.norm(n)
+    u = np.array([-n[1], n[0], 0.0])
+    u /= np.linalg.norm(u)
+    v = np.cross(n, u)
+    if v[2] < 0:
+        u, v = -u, -v
+    return rec, u, v
+
+
+def aim_heliostat(p: np.ndarray, target: np.ndarray, sv: np.ndarray):
+    """Normal of a heliostat at p that reflects the sun (unit vector sv) to
+    target -- the bisector of the sun and target directions
+    (app/deploy/api/examples/tower-demo.py:49-54) -- and its slant range."""
+    to_t = target - p
+    slant = float(np.linalg.norm(to_t))
+    nrm = to_t / slant + sv
+    return nrm / np.linalg.norm(nrm), slant
+
+
 def tower(n_helio: int = 1000, raycount: int = 10_000_000, helio_size: float = 10.0,
           rec_size: float = 15.0, tower_h: float = 150.0, r_max: float = 1000.0,
           sun_elev: float = 45.0, sun_azim: float = 180.0, csr: float = 0.1, slope: float = 1.5,
-          refl: float = 0.9, rec_abs: float = 0.95, sun: str = "buie", dyn_group: bool = True) -> Scene:
-    """Heliostat field in the north half-plane, flat receiver facing it."""
+          refl: float = 0.9, rec_abs: float = 0.95, sun: str = "buie", dyn_group: bool = True,
+          aim_offsets=None) -> Scene:
+    """Heliostat field in the north half-plane, flat receiver facing it.
+    aim_offsets (n_helio x 2, metres, optional): each heliostat aims at the
+    receiver centre plus du * u + dv * v (receiver_axes) instead of the centre;
+    its focal length follows the aim point's slant range."""
     sv = sun_vector(sun_elev, sun_azim)
     pts = staggered_layout(n_helio, 100.0, r_max, 2.2 * helio_size, north_only=True)
-    rec = np.array([0.0, 0.0, tower_h])
+    rec, ax_u, ax_v = receiver_axes(pts, tower_h)
     s = Scene(sun_xyz=tuple(sv * 1000.0), raycount=raycount, maxcount=2**31 - 1, dyn_group=dyn_group)
     if sun == "buie":
         a, v = buie_profile(csr)
@@ -116,12 +146,10 @@ def tower(n_helio: int = 1000, raycount: int = 10_000_000, helio_size: float = 1
                       OpticFace(dist="g", refl=0.0, slope=slope, spec=0.0)),
                 Optic("receiver", OpticFace(dist="g", refl=1.0 - rec_abs), OpticFace(dist="g", refl=1.0 - rec_abs))]
     els = []
-    for x, y in pts:
+    for i, (x, y) in enumerate(pts):
         p = np.array([x, y, 0.0])
-        to_rec = rec - p
-        slant = float(np.linalg.norm(to_rec))
-        nrm = to_rec / slant + sv
-        nrm /= np.linalg.norm(nrm)
+        target = rec if aim_offsets is None else rec + aim_offsets[i][0] * ax_u + aim_offsets[i][1] * ax_v
+        nrm, slant = aim_heliostat(p, target, sv)
         els.append(Element(xyz=tuple(p), aim=tuple(p + 100.0 * nrm), aperture="r",
                            ap_params=(helio_size, helio_size), surface="s",
                            surf_params=(1.0 / (2.0 * slant),), optic="helio"))

@@ -63,6 +63,9 @@ def hip_lib():
     lib.dll.st_amd_flux_request.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                             C.c_double, C.c_double, C.c_double, C.c_double]
     lib.dll.st_amd_flux_result.argtypes = [C.c_void_p, C.POINTER(C.c_uint), C.POINTER(C.c_double)]
+    lib.dll.st_amd_flux_by_source.argtypes = [C.c_void_p, C.c_int]
+    lib.dll.st_amd_flux_source_result.argtypes = [C.c_void_p, C.POINTER(C.c_uint), C.c_longlong]
+    lib.dll.st_amd_flux_source_result.restype = C.c_longlong
     lib.dll.st_amd_ray_sun_index.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), C.c_int]
     lib.dll.st_amd_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
     return lib

@@ -282,6 +282,58 @@ def test_flux_map_fp64_equals_oracle_binning(hip_lib, oracle_lib, receiver):
 
 
 @pytest.mark.parametrize("bits", [64, 32])
+def test_flux_by_source_equals_oracle(hip_lib, oracle_lib, bits):
+    # one receiver map per heliostat: each ray counts in the map of the
+    # heliostat of its last stage-1 record; the maps add up to the receiver
+    # map bin for bin, and in FP64 equal the oracle's split of its own records
+    import ctypes as C
+    sc = workloads.tower(n_helio=60, raycount=30000, r_max=400.0)
+    nx = ny = 32
+    half = 7.5
+    nsrc = len(sc.stages[0].elements)
+
+    def gpu_setup(lib, ctx):
+        _setup(bits)(lib, ctx)
+        assert lib.dll.st_amd_flux_request(ctx, 1, 0, nx, ny, 1, -half, half, -half, half) == 1
+        assert lib.dll.st_amd_flux_by_source(ctx, 1) == 1
+
+    def gpu_after(lib, ctx):
+        buf = (C.c_uint * (nx * ny))()
+        assert lib.dll.st_amd_flux_result(ctx, buf, None) == nx * ny
+        f = lib.dll.st_amd_flux_source_result
+        f.restype = C.c_longlong
+        f.argtypes = [C.c_void_p, C.POINTER(C.c_uint), C.c_longlong]
+        assert f(ctx, None, 0) == nsrc * nx * ny
+        src = (C.c_uint * (nsrc * nx * ny))()
+        assert f(ctx, src, nsrc * nx * ny - 1) == -1  # too small
+        assert f(ctx, src, nsrc * nx * ny) == nsrc * nx * ny
+        return (np.frombuffer(buf, dtype=np.uint32).reshape(nx, ny).astype(np.float64),
+                np.frombuffer(src, dtype=np.uint32).reshape(nsrc, nx, ny).astype(np.float64))
+
+    def orc_after(lib, ctx):
+        out = (C.c_double * (nx * ny))()
+        src = (C.c_double * (nsrc * nx * ny))()
+        f = lib.dll.st_oracle_flux_by_source
+        f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                      C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        assert f(ctx, 1, 0, nx, ny, 1, -half, half, -half, half, out, src) == 1
+        return (np.frombuffer(out, dtype=np.float64).reshape(nx, ny).copy(),
+                np.frombuffer(src, dtype=np.float64).reshape(nsrc, nx, ny).copy())
+
+    _, _, _, (fo, so) = run_scene(oracle_lib, sc, 9, setup=counter_mode, after=orc_after)
+    _, _, m, (fg, sg) = run_scene(hip_lib, sc, 9, setup=gpu_setup, after=gpu_after)
+    assert fo.sum() > 5000 and (so.sum(axis=(1, 2)) > 0).sum() > 50, (fo.sum(), m)
+    np.testing.assert_array_equal(so.sum(axis=0), fo)   # the oracle's split is a partition
+    np.testing.assert_array_equal(sg.sum(axis=0), fg)   # so is the library's
+    if bits == 64:
+        np.testing.assert_array_equal(sg, so)
+    else:
+        # FP32 hit points move by < 1e-4 m: a few rays may change bin
+        assert np.abs(sg - so).sum() <= 2e-3 * so.sum(), np.abs(sg - so).sum()
+        assert np.abs(sg.sum(axis=(1, 2)) - so.sum(axis=(1, 2))).max() <= 2
+
+
+@pytest.mark.parametrize("bits", [64, 32])
 def test_torus_target(hip_lib, oracle_lib, bits):
     sc = torus_target_scene()
     n_o, r_o, _, _ = run_scene(oracle_lib, sc, 17, setup=counter_mode)
