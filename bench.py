@@ -27,13 +27,20 @@ sunshape draw, and per element test of each stage; the element tests a
 kernel makes are the GPU's own event counts (its cull is exact, so these are
 the tests the search needs); BVH node visits are counted as no flops.
 
+"surrogate" is the learned flux surrogate on the same field: 200,000
+per-heliostat receiver images traced by the library train a BF16 MLP on the
+GPU's MFMA units, which then drives a 500-step aim-point sweep with the
+tracer as fallback and judge; its speed per step is compared with the same
+sweep scoring each candidate by tracing, and its final peak flux with a
+1e8-hit FP64 trace (soltrace_amd/surrogate.py).
+
 `cpu_baseline` times the reference's own coretrace, compiled from source in
 this repository's oracle/ recipe (oracle/_ref/coretrace_ref.so), on the
 host's cores with the GUI's multithreaded scheme (one context per thread,
 rays split evenly, seeds seed, seed+123, seed+369, ...), on a bounded sample
 of the same scene, and on one thread.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-workloads] [--no-surrogate]
 """
 from __future__ import annotations
 
@@ -283,6 +290,85 @@ def cpu_baseline(scene_fn, seed, target_s=12.0):
                       f"on {cal} hits"}
 
 
+def surrogate_workload(lib, seed, log):
+    """The learned flux surrogate on the tower field: 200 random states
+    (sun within 160-200 deg azimuth and 35-55 deg elevation, slope error
+    1-2 mrad, aim offsets within +-6 m) x 1,000 heliostats = 200,000
+    per-heliostat 64 x 64 receiver images at ~2e5 rays each (ten 2e7-hit
+    traces per state, averaged), 3 held-out states at ~1e6 rays per image;
+    the 8 -> 6 x 256 -> 4096 BF16 network trained by the library's MFMA
+    kernels (20,000 Adam steps of 512); a 500-step aim-point sweep at the
+    bench sun with the tracer as fallback (5e7-hit anchors) and judge; the
+    same sweep scoring candidates by 1e7-hit traces, timed over 10 steps;
+    start and end judged by 1e8-hit FP64 traces."""
+    import torch
+    from soltrace_amd import surrogate as S
+
+    f = S.Field(lib, 1000, bits=32)
+    t0 = time.perf_counter()
+    X, T = S.make_dataset(f, 200, 20_000_000, seed, repeats=10, log=log)
+    Xv, Tv = S.make_dataset(f, 3, 20_000_000, seed + 1, repeats=50)
+    data_s = time.perf_counter() - t0
+    sur = S.Surrogate(lib, max_batch=1024, seed=1)
+    fit = sur.fit(X, T, 20_000, batch=512, val=(Xv, Tv), log=log)
+    del X, T
+    # the MFMA kernels' rate: training steps and forwards, HIP events on torch's stream (the one they launch on)
+    n_in, width, depth, n_out = sur.net.shape
+    macs = n_in * width + (depth - 1) * width * width + width * n_out
+    xb = Xv[:512].contiguous()
+    tb = Tv[:512].contiguous()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    sur.net.train_step(xb, tb, 0.0)
+    ev0.record()
+    for _ in range(50):
+        sur.net.train_step(xb, tb, 0.0)
+    ev1.record()
+    torch.cuda.synchronize()
+    step_ms = ev0.elapsed_time(ev1) / 50
+    xf = Xv[:1024].contiguous()
+    sur.net.forward(xf)
+    ev0.record()
+    for _ in range(50):
+        sur.net.forward(xf)
+    ev1.record()
+    torch.cuda.synchronize()
+    fwd_ms = ev0.elapsed_time(ev1) / 50
+    train_tflops = 3 * 2 * 512 * macs / (step_ms * 1e-3) / 1e12
+    fwd_tflops = 2 * 1024 * macs / (fwd_ms * 1e-3) / 1e12
+
+    st0 = S.FieldState(offsets=np.zeros((f.n, 2)))
+    f.set_state(st0)
+    r = S.aim_sweep(f, sur, st0.offsets, steps=500, anchor_hits=50_000_000, seed=seed)
+    f.set_state(st0)
+    rt = S.aim_sweep_traced(f, st0.offsets, steps=10, eval_hits=10_000_000, seed=seed)
+    f64 = S.Field(lib, 1000, bits=64, by_source=False)
+    j0 = float(S.judge(f64, st0.offsets, 100_000_000, seed + 7).max().item())
+    j1 = float(S.judge(f64, r.offsets, 100_000_000, seed + 7).max().item())
+    f.close()
+    f64.close()
+    sur.net.close()
+    ms_net, ms_tr = r.wall_s / r.steps * 1e3, rt.wall_s / rt.steps * 1e3
+    return {
+        "workload": "tower field, 1,000 heliostats: 200,000 per-heliostat 64x64 receiver images at ~2e5 rays "
+                    "each train an 8 -> 6x256 -> 4096 BF16 MLP (FP32 accumulate) on the GPU; 500-step "
+                    "aim-point sweep with tracer fallback, judged by 1e8-hit FP64 traces",
+        "images": 200 * 1000, "rays_per_image": 2e5, "data_s": data_s,
+        "train_s": fit["train_s"], "train_steps": fit["steps"], "batch": fit["batch"],
+        "validation": {k: fit[k] for k in ("rel_rms_bright", "rms_bright_wm2", "peak_rel_median", "peak_rel_p95",
+                                           "power_rel_median", "power_rel_p95")},
+        "mlp": {"train_step_ms": step_ms, "train_tflops": train_tflops, "forward_1024_ms": fwd_ms,
+                "forward_tflops": fwd_tflops, "peak_tflops_bf16": 2500.0,
+                "frac_train": train_tflops / 2500.0, "macs_per_row": macs},
+        "sweep": {"steps": r.steps, "accepted": r.accepted, "fallbacks": r.fallbacks, "traces": r.traces,
+                  "wall_s": r.wall_s, "ms_per_step": ms_net, "judged_errors_max": max(r.judged_errors or [0.0]),
+                  "peak_pred_wm2": r.peak_pred, "peak_judged_start_wm2": j0, "peak_judged_end_wm2": j1,
+                  "peak_error": abs(r.peak_pred - j1) / j1},
+        "traced_sweep": {"steps": rt.steps, "traces": rt.traces, "ms_per_step": ms_tr, "eval_hits": 10_000_000},
+        "speedup_per_step": ms_tr / ms_net,
+        "targets": {"speedup": ">= 20", "peak_error": "<= 0.02"},
+    }
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of known size: what FETCH_SIZE and
     WRITE_SIZE read for it gives the factor between those counters and
@@ -307,6 +393,7 @@ def main():
     ap.add_argument("--hits", type=int, default=N_HITS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-workloads", action="store_true", help="only the tower workload")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the flux-surrogate workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (scale for the HBM byte counters)")
     args = ap.parse_args()
@@ -425,6 +512,8 @@ def main():
             "faceted_fp64": _workload_rate(lib, workloads.faceted_field(n_helio=10_000, raycount=50_000_000), 64,
                                            2, args.seed, sync),
         }
+    if not args.no_workloads and not args.no_surrogate:
+        line["surrogate"] = surrogate_workload(lib, args.seed, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lambda n: workloads.tower(n_helio=1000, raycount=n), args.seed)
     print(json.dumps(line))
