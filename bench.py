@@ -27,6 +27,11 @@ sunshape draw, and per element test of each stage; the element tests a
 kernel makes are the GPU's own event counts (its cull is exact, so these are
 the tests the search needs); BVH node visits are counted as no flops.
 
+"batched" traces 256 uneven scenes (tower fields of 200-5,000 heliostats and
+sun-tracking dishes with Zernike facets and torus sections, suns over a year
+at 35 N, 1e5-2e6 ray hits each) in one st_amd_sim_run_batch call per step,
+beside the same contexts traced one st_sim_run at a time.
+
 "surrogate" is the learned flux surrogate on the same field: 200,000
 per-heliostat receiver images traced by the library train a BF16 MLP on the
 GPU's MFMA units, which then drives a 500-step aim-point sweep with the
@@ -40,7 +45,7 @@ host's cores with the GUI's multithreaded scheme (one context per thread,
 rays split evenly, seeds seed, seed+123, seed+369, ...), on a bounded sample
 of the same scene, and on one thread.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-workloads] [--no-surrogate]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-workloads] [--no-batch] [--no-surrogate]
 """
 from __future__ import annotations
 
@@ -369,6 +374,71 @@ def surrogate_workload(lib, seed, log):
     }
 
 
+def batch_workload(lib, seed, sync, log, n_scenes=256, bits=32, reps=2):
+    """Batched uneven scenes: 256 scenes from a fixed seed, each a tower
+    field of U[200, 5000] heliostats or a sun-tracking dish of order-6
+    Zernike facets and torus sections, suns over a year at 35 N, U[1e5, 2e6]
+    ray hits each (workloads.batch_scenes), traced by st_amd_sim_run_batch;
+    one untimed batch (scene compile, launch-shape autotuning), then `reps`
+    timed batches; the same contexts traced one by one with st_sim_run for
+    comparison."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.perf_counter()
+        scenes = workloads.batch_scenes(n_scenes, seed=2026, hits=(100_000, 2_000_000), mon_dir=d)
+        ctxs = []
+        try:
+            for sc in scenes:
+                c = lib.st_create_context()
+                sc.apply(lib, c)
+                lib.dll.st_amd_set_precision(c, bits)
+                ctxs.append(c)
+            setup_s = time.perf_counter() - t0
+            log(f"batch: {len(ctxs)} scenes set up in {setup_s:.1f} s")
+            seeds = [(seed + 123 * k) & 0xFFFFFFFF for k in range(len(ctxs))]
+            t0 = time.perf_counter()
+            rc, counts = stapi.run_batch(lib, ctxs, seeds)
+            sync()
+            first_s = time.perf_counter() - t0
+            if rc != 1:
+                raise RuntimeError("st_amd_sim_run_batch failed: " +
+                                   "; ".join(m for c in ctxs for m in stapi.messages(lib, c)[-1:]))
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                stapi.run_batch(lib, ctxs, seeds)
+            sync()
+            batch_s = (time.perf_counter() - t0) / reps
+            tm = (C.c_double * 70)()
+            lib.dll.st_amd_batch_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
+            lib.dll.st_amd_batch_timing(ctxs[0], tm, 70)
+            tm = list(tm)
+            t0 = time.perf_counter()
+            for c, sd in zip(ctxs, seeds):
+                _run(lib, c, sd)
+            sync()
+            sep_s = time.perf_counter() - t0
+            tune = (C.c_int * 48)()
+            ntuned = lib.dll.st_amd_batch_tuning(tune, 48)
+        finally:
+            for c in ctxs:
+                lib.st_free_context(c)
+    hits = sum(sc.raycount for sc in scenes)
+    n_field = sum(1 for sc in scenes if sc.stages[0].elements[0].surface == "s")
+    return {
+        "workload": f"{n_scenes} scenes (seed 2026): {n_field} tower fields of 200-5,000 spherical heliostats, "
+                    f"{n_scenes - n_field} dishes of order-6 Zernike facets and torus sections; suns over a year "
+                    f"at 35 N; 1e5-2e6 ray hits each; one st_amd_sim_run_batch call per step",
+        "precision": f"fp{bits}", "ray_hits": hits, "value": hits / batch_s, "unit": "ray hits/s",
+        "ms_per_step": batch_s * 1e3, "first_batch_ms": first_s * 1e3, "setup_s": setup_s,
+        "separate_runs_ms": sep_s * 1e3, "batch_over_separate": sep_s / batch_s,
+        "records": int(sum(counts)), "sun_rays": int(tm[3]),
+        "kernel_ms": {"sun_mask": tm[0], "sunshape": tm[1], "trace_stage0": tm[6], "trace_stage1": tm[7]},
+        "launches": {"sun_mask": int(tm[4]), "trace_stage0": int(tm[22]), "trace_stage1": int(tm[23])},
+        "autotuned": [{"class": tune[3 * i], "tile": tune[3 * i + 1], "waves": tune[3 * i + 2]}
+                      for i in range(min(ntuned, 16))],
+    }
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of known size: what FETCH_SIZE and
     WRITE_SIZE read for it gives the factor between those counters and
@@ -394,6 +464,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-workloads", action="store_true", help="only the tower workload")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the flux-surrogate workload")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scene workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (scale for the HBM byte counters)")
     args = ap.parse_args()
@@ -512,6 +583,8 @@ def main():
             "faceted_fp64": _workload_rate(lib, workloads.faceted_field(n_helio=10_000, raycount=50_000_000), 64,
                                            2, args.seed, sync),
         }
+    if not args.no_workloads and not args.no_batch:
+        line["batched"] = batch_workload(lib, args.seed, sync, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_workloads and not args.no_surrogate:
         line["surrogate"] = surrogate_workload(lib, args.seed, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu:
