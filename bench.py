@@ -158,15 +158,26 @@ def _workload_rate(lib, scene, bits, steps, seed, sync):
 
 
 def _end_to_end(lib, ctx, seed, sync):
-    """One run plus the getters' copy of every record to the host."""
+    """One run plus the result getters' copy of every record into arrays
+    the caller allocates (st_locations, st_cosines, st_elementmap,
+    st_stagemap, st_raynumbers, coretrace/stapi.cpp:564-659)."""
     sync()
     t0 = time.perf_counter()
     n = _run(lib, ctx, seed)
     t1 = time.perf_counter()
-    stapi.fetch_results(lib, ctx)
+    d = [np.empty(n) for _ in range(6)]
+    m = [np.empty(n, dtype=np.int32) for _ in range(3)]
+    dp = [a.ctypes.data_as(C.POINTER(C.c_double)) for a in d]
+    ip = [a.ctypes.data_as(C.POINTER(C.c_int)) for a in m]
+    ok = (lib.st_locations(ctx, *dp[:3]) == 1 and lib.st_cosines(ctx, *dp[3:]) == 1 and
+          lib.st_elementmap(ctx, ip[0]) == 1 and lib.st_stagemap(ctx, ip[1]) == 1 and
+          lib.st_raynumbers(ctx, ip[2]) == 1)
     t2 = time.perf_counter()
+    if not ok:
+        raise RuntimeError("result getters failed")
+    gb = n * (6 * 8 + 3 * 4) / 1e9
     return {"ms_per_step": (t2 - t0) * 1e3, "trace_ms": (t1 - t0) * 1e3, "getters_ms": (t2 - t1) * 1e3,
-            "records": n, "value": N_HITS / (t2 - t0)}
+            "records": n, "getter_gb": gb, "getter_gb_s": gb / (t2 - t1), "value": N_HITS / (t2 - t0)}
 
 
 def counted_work(scene_fn, hits, seed):

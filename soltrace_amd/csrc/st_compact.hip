@@ -273,4 +273,43 @@ void launch_last_raynum(const LastRayItem *items, int m, uint32_t *out, hipStrea
     hipLaunchKernelGGL(last_raynum_kernel, dim3((m + 255) / 256), dim3(256), 0, s, items, m, out);
 }
 
+
+
+// ------------------------------------------------ merged record columns ---
+// The getters' view of a run's records (coretrace/stapi.cpp:564-659 over
+// TRayData::Merge, types.cpp:311-355): output record i comes from the piece
+// holding i, found by a binary search over the pieces' first output
+// positions; the record is split into the columns the getters return.
+__global__ __launch_bounds__(256) void gather_records_kernel(const MergePiece *pieces, int npieces, uint64_t n,
+                                                             RecordColumns out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        int lo = 0, hi = npieces - 1;  // the last piece starting at or below i
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pieces[mid].dst <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const MergePiece p = pieces[lo];
+        const DevRecord r = p.src[i - p.dst];
+        out.x[i] = r.pos[0];
+        out.y[i] = r.pos[1];
+        out.z[i] = r.pos[2];
+        out.cx[i] = r.cos[0];
+        out.cy[i] = r.cos[1];
+        out.cz[i] = r.cos[2];
+        out.element[i] = r.element;
+        out.stage[i] = p.stage;
+        out.raynum[i] = (int32_t)r.raynum;
+    }
+}
+
+void launch_gather_records(const MergePiece *pieces, int npieces, uint64_t n, const RecordColumns &out,
+                           int cus, hipStream_t s) {
+    if (n == 0 || npieces == 0) return;
+    const uint64_t want = (n + 255) / 256;
+    const int blocks = (int)std::min<uint64_t>(want, (uint64_t)cus * 16);
+    hipLaunchKernelGGL(gather_records_kernel, dim3(blocks), dim3(256), 0, s, pieces, npieces, n, out);
+}
+
 }  // namespace st

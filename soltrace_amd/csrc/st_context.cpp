@@ -10,6 +10,8 @@
 // (coretrace/types.cpp:311-355).
 #include "st_context.h"
 
+#include <thread>
+
 namespace st {
 
 // Scratch of st_amd_element_stats_noalloc, the allocation-free element
@@ -46,36 +48,110 @@ void *scratch_on(int dev) {
 
 
 void Context::materialize() {
-    if (host_valid) return;
-    // TRayData::Merge applied stage by stage: full 8192-record blocks first,
-    // partial blocks re-appended after (coretrace/types.cpp:311-355).
+    if (cols_valid) return;
+    // TRayData::Merge applied stage by stage: the records merged so far up
+    // to their last full 8192-record block, then this stage's full blocks,
+    // then the two partial blocks (coretrace/types.cpp:311-355).  Tracked as
+    // pieces of stage chunks, then gathered on the device into columns.
+    struct Run {
+        int stage;
+        size_t first, n;  // records [first, first + n) of the stage
+    };
     const size_t B = 8192;
-    std::vector<std::vector<DevRecord>> per(recs.size());
+    std::vector<Run> all;
+    size_t n_all = 0;
+    auto take = [](const std::vector<Run> &v, size_t from, size_t to, std::vector<Run> &out) {
+        size_t pos = 0;
+        for (const Run &r : v) {
+            const size_t a = std::max(pos, from), b = std::min(pos + r.n, to);
+            if (a < b) out.push_back(Run{r.stage, r.first + (a - pos), b - a});
+            pos += r.n;
+        }
+    };
     for (size_t s = 0; s < recs.size(); s++) {
-        per[s].resize(recs[s].total);
-        size_t off = 0;
-        for (size_t c = 0; c < recs[s].chunks.size(); c++) {
-            if (recs[s].counts[c])
-                HC(hipMemcpyAsync(per[s].data() + off, recs[s].ptr(c), recs[s].counts[c] * sizeof(DevRecord),
-                                  hipMemcpyDeviceToHost, stream));
-            off += recs[s].counts[c];
+        const size_t ns = recs[s].total;
+        const std::vector<Run> src = {Run{(int)s, 0, ns}};
+        const size_t fa = (n_all / B) * B, fs = (ns / B) * B;
+        std::vector<Run> out;
+        take(all, 0, fa, out);
+        take(src, 0, fs, out);
+        take(all, fa, n_all, out);
+        take(src, fs, ns, out);
+        all.swap(out);
+        n_all += ns;
+    }
+    // runs -> pieces of the stages' chunks
+    std::vector<MergePiece> pieces;
+    uint64_t dst = 0;
+    for (const Run &r : all) {
+        const StageRecords &sr = recs[r.stage];
+        size_t pos = 0;
+        for (size_t k = 0; k < sr.chunks.size(); k++) {
+            const size_t a = std::max(pos, r.first), b = std::min(pos + sr.counts[k], r.first + r.n);
+            if (a < b) {
+                pieces.push_back(MergePiece{sr.ptr(k) + (a - pos), dst, r.stage + 1});
+                dst += b - a;
+            }
+            pos += sr.counts[k];
         }
     }
-    HC(hipStreamSynchronize(stream));
-    std::vector<DevRecord> all;
-    for (size_t s = 0; s < per.size(); s++) {
-        const std::vector<DevRecord> &src = per[s];
-        size_t fa = (all.size() / B) * B, fs = (src.size() / B) * B;
-        std::vector<DevRecord> out;
-        out.reserve(all.size() + src.size());
-        out.insert(out.end(), all.begin(), all.begin() + fa);
-        out.insert(out.end(), src.begin(), src.begin() + fs);
-        out.insert(out.end(), all.begin() + fa, all.end());
-        out.insert(out.end(), src.begin() + fs, src.end());
-        all.swap(out);
+    if (dst != n_all) throw std::runtime_error("record merge lost records");
+    const size_t n = n_all;
+    char *base = (char *)d_cols.ensure(std::max<size_t>(n, 1) * (6 * sizeof(double) + 3 * sizeof(int32_t)));
+    double *d = (double *)base;
+    cols = RecordColumns{d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, (int32_t *)(d + 6 * n),
+                         (int32_t *)(d + 6 * n) + n, (int32_t *)(d + 6 * n) + 2 * n};
+    if (n) {
+        if (!cus) HC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        const MergePiece *dp = upload(d_pieces, pieces, stream);
+        launch_gather_records(dp, (int)pieces.size(), n, cols, cus, stream);
+        HC(hipStreamSynchronize(stream));
     }
-    host_recs.swap(all);
-    host_valid = true;
+    cols_valid = true;
+}
+
+// Columns go through two pinned staging halves in at least four chunks:
+// the device copies chunk k into one half while host threads copy chunk
+// k - 1 from the other into the caller's (pageable) memory.  Columns of
+// 256 KiB or less are copied directly.
+void Context::copy_out(void *host, const void *dev, size_t bytes) {
+    constexpr size_t kMaxHalf = (size_t)32 << 20, kMinHalf = (size_t)64 << 10;
+    if (bytes == 0) return;
+    if (bytes <= 4 * kMinHalf) {
+        HC(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream));
+        return;
+    }
+    const size_t kHalf = std::min(kMaxHalf, std::max(kMinHalf, (bytes / 4 + 4095) & ~(size_t)4095));
+    if (!h_stage) HC(hipHostMalloc((void **)&h_stage, 2 * kMaxHalf, hipHostMallocDefault));
+    for (hipEvent_t &e : ev_stage)
+        if (!e) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    const unsigned hw = std::thread::hardware_concurrency();
+    auto host_copy = [&](char *dst, const char *src, size_t len) {
+        const int nthr = len >= ((size_t)4 << 20) ? (int)std::max(1u, std::min(8u, hw ? hw : 1u)) : 1;
+        std::vector<std::thread> th;
+        const size_t part = (len + nthr - 1) / nthr;
+        for (int t = 1; t < nthr; t++) {
+            const size_t a = std::min(len, t * part), b = std::min(len, (t + 1) * part);
+            if (a < b) th.emplace_back([=]() { memcpy(dst + a, src + a, b - a); });
+        }
+        memcpy(dst, src, std::min(len, part));
+        for (auto &x : th) x.join();
+    };
+    const char *d = (const char *)dev;
+    char *h = (char *)host;
+    const size_t nch = (bytes + kHalf - 1) / kHalf;
+    for (size_t k = 0; k <= nch; k++) {
+        if (k < nch) {
+            const size_t len = std::min(kHalf, bytes - k * kHalf);
+            HC(hipMemcpyAsync(h_stage + (k & 1) * kHalf, d + k * kHalf, len, hipMemcpyDeviceToHost, stream));
+            HC(hipEventRecord(ev_stage[k & 1], stream));
+        }
+        if (k >= 1) {
+            const size_t j = k - 1, len = std::min(kHalf, bytes - j * kHalf);
+            HC(hipEventSynchronize(ev_stage[j & 1]));
+            host_copy(h + j * kHalf, h_stage + (j & 1) * kHalf, len);
+        }
+    }
 }
 
 // Compile the scene when a setter changed it (geometry set-up, cull
@@ -1040,53 +1116,43 @@ STCORE_API int st_num_intersections(st_context_t p) {
         c->errlog("result copy failed: %s", e.what());     \
         return -2;                                         \
     }
+// The copies of one getter, then a wait for them: the caller's arrays are
+// complete when the getter returns.
+#define GETTER_COPY(...)                                   \
+    try {                                                  \
+        __VA_ARGS__;                                       \
+        HC(hipStreamSynchronize(c->stream));               \
+        return 1;                                          \
+    } catch (const std::exception &e) {                    \
+        c->errlog("result copy failed: %s", e.what());     \
+        return -2;                                         \
+    }
 STCORE_API int st_locations(st_context_t p, double *x, double *y, double *z) {
     CTX(p, -1);
     MATERIALIZE();
-    for (size_t i = 0; i < c->host_recs.size(); i++) {
-        x[i] = c->host_recs[i].pos[0]; y[i] = c->host_recs[i].pos[1]; z[i] = c->host_recs[i].pos[2];
-    }
-    return 1;
+    const size_t b = c->total_records() * sizeof(double);
+    GETTER_COPY(c->copy_out(x, c->cols.x, b); c->copy_out(y, c->cols.y, b); c->copy_out(z, c->cols.z, b));
 }
 STCORE_API int st_cosines(st_context_t p, double *x, double *y, double *z) {
     CTX(p, -1);
     MATERIALIZE();
-    for (size_t i = 0; i < c->host_recs.size(); i++) {
-        x[i] = c->host_recs[i].cos[0]; y[i] = c->host_recs[i].cos[1]; z[i] = c->host_recs[i].cos[2];
-    }
-    return 1;
+    const size_t b = c->total_records() * sizeof(double);
+    GETTER_COPY(c->copy_out(x, c->cols.cx, b); c->copy_out(y, c->cols.cy, b); c->copy_out(z, c->cols.cz, b));
 }
 STCORE_API int st_elementmap(st_context_t p, int *m) {
     CTX(p, -1);
     MATERIALIZE();
-    for (size_t i = 0; i < c->host_recs.size(); i++) m[i] = c->host_recs[i].element;
-    return 1;
+    GETTER_COPY(c->copy_out(m, c->cols.element, c->total_records() * sizeof(int)));
 }
 STCORE_API int st_stagemap(st_context_t p, int *m) {
     CTX(p, -1);
     MATERIALIZE();
-    // stage of each merged record: recover from the merge layout
-    const size_t B = 8192;
-    std::vector<int> st_of;
-    for (size_t s = 0; s < c->recs.size(); s++) {
-        std::vector<int> src(c->recs[s].total, (int)s + 1);
-        size_t fa = (st_of.size() / B) * B, fs = (src.size() / B) * B;
-        std::vector<int> out;
-        out.reserve(st_of.size() + src.size());
-        out.insert(out.end(), st_of.begin(), st_of.begin() + fa);
-        out.insert(out.end(), src.begin(), src.begin() + fs);
-        out.insert(out.end(), st_of.begin() + fa, st_of.end());
-        out.insert(out.end(), src.begin() + fs, src.end());
-        st_of.swap(out);
-    }
-    for (size_t i = 0; i < st_of.size(); i++) m[i] = st_of[i];
-    return 1;
+    GETTER_COPY(c->copy_out(m, c->cols.stage, c->total_records() * sizeof(int)));
 }
 STCORE_API int st_raynumbers(st_context_t p, int *m) {
     CTX(p, -1);
     MATERIALIZE();
-    for (size_t i = 0; i < c->host_recs.size(); i++) m[i] = (int)c->host_recs[i].raynum;
-    return 1;
+    GETTER_COPY(c->copy_out(m, c->cols.raynum, c->total_records() * sizeof(int)));
 }
 STCORE_API int st_sun_stats(st_context_t p, double *xmin, double *xmax, double *ymin, double *ymax, int *nsun) {
     CTX(p, -1);

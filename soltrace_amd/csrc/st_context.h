@@ -137,8 +137,13 @@ struct Context {
     std::vector<std::shared_ptr<DevBuf>> rec_pool;
     std::vector<hipEvent_t> ev_pool;  // kernel timing events, reused
     uint64_t sun_ray_count = 0;
-    bool host_valid = false;
-    std::vector<DevRecord> host_recs;  // merged order
+    // the records in merged order as the getters' columns, in HBM
+    bool cols_valid = false;
+    DevBuf d_cols, d_pieces;
+    RecordColumns cols{};
+    int cus = 0;
+    char *h_stage = nullptr;  // pinned staging for the getters' copies (2 x 32 MiB)
+    hipEvent_t ev_stage[2] = {nullptr, nullptr};
     uint64_t n_hits_done = 0;
 
     FluxSpec flux{};
@@ -176,6 +181,9 @@ struct Context {
         if (ev_ready) (void)hipEventDestroy(ev_ready);
         if (ev_tot) (void)hipEventDestroy(ev_tot);
         if (h_tot) (void)hipHostFree(h_tot);
+        if (h_stage) (void)hipHostFree(h_stage);
+        for (hipEvent_t e : ev_stage)
+            if (e) (void)hipEventDestroy(e);
         if (mask_stream) (void)hipStreamDestroy(mask_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -192,8 +200,7 @@ struct Context {
             for (size_t k = 0; k < st.chunks.size(); k++)
                 if (st.pooled[k]) rec_pool.push_back(st.chunks[k]);
         recs.clear();
-        host_valid = false;
-        host_recs.clear();
+        cols_valid = false;
         n_hits_done = 0;
         sun_ray_count = 0;
     }
@@ -201,6 +208,8 @@ struct Context {
     int run(uint32_t seed, st_callback_t cb, void *cbdata);
     template <typename R> int run_t(uint32_t seed, st_callback_t cb, void *cbdata);
     void materialize();
+    // copy `bytes` of the merged columns at dev into the caller's host memory
+    void copy_out(void *host, const void *dev, size_t bytes);
     size_t total_records() const {
         size_t n = 0;
         for (const auto &s : recs) n += s.total;

@@ -199,5 +199,19 @@ void launch_stats_bin(const DevRecord *rec, uint32_t n, int32_t elem, const Flux
                       uint32_t *bins, unsigned long long *acc, hipStream_t s);
 void launch_compact_rays(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, const uint32_t *tile_off,
                          int n_rays, DevRay *out, hipStream_t s);
+// The getters' columns of a run's records in merged order: records
+// [dst, dst + n) of the output come from src, of stage `stage` (1-based);
+// pieces are sorted by dst and cover [0, total).
+struct MergePiece {
+    const DevRecord *src;
+    uint64_t dst;
+    int32_t stage;
+};
+struct RecordColumns {
+    double *x, *y, *z, *cx, *cy, *cz;
+    int32_t *element, *stage, *raynum;
+};
+void launch_gather_records(const MergePiece *pieces, int npieces, uint64_t n, const RecordColumns &out, int cus,
+                           hipStream_t s);
 
 }  // namespace st
