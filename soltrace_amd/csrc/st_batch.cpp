@@ -118,14 +118,15 @@ std::mutex tune_mu;
 std::map<int, TraceShape> tuned;
 
 int trace_grid(const BatchWork &W, const TraceShape &sh, int generic) {
-    return W.cus * (generic ? 1 : sh.waves);
+    return W.cus * (generic == 2 ? 1 : sh.waves);
 }
 
 }  // namespace
 
 int trace_batch_waves(int bits, int generic, int *out, int max) {
     int v[3], n = 0;
-    if (generic) v[n++] = 1;
+    if (generic == 1) { v[n++] = 2; v[n++] = 1; }
+    else if (generic) v[n++] = 1;
     else if (bits == 64) { v[n++] = 2; v[n++] = 1; }
     else { v[n++] = 4; v[n++] = 3; v[n++] = 2; }
     for (int i = 0; i < n && i < max; i++) out[i] = v[i];

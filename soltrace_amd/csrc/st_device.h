@@ -563,6 +563,33 @@ STD void zernike(const DevElem &e, const double *aux, R x, R y, R *z, R *dzdx, R
     *dzdy = dy;
 }
 
+// The same monomial surface and its slopes in Horner form, for the float
+// search of FP32 runs: z = sum_j x^j P_j(y), P_j(y) = sum_k B[j+k][j] y^k,
+// each P_j and its derivative by Horner in y, then Horner in x for z and
+// dz/dx (d/dx of Z x + p is DX x + Z) and dz/dy.  No power tables: the loop
+// state is six scalars.  FP64 runs keep the reference's term order
+// (zernike above), and every FP32 hit is finished in double by that.
+template <typename R>
+STD void zernike_horner(const DevElem &e, const double *aux, R x, R y, R *z, R *dzdx, R *dzdy) {
+    const int n = e.fit_order;
+    const int w = e.aux_n;
+    const double *Bm = aux + e.aux;
+    R Z = 0, DX = 0, DY = 0;
+    for (int j = n; j >= 0; --j) {
+        R p = 0, dp = 0;
+        for (int k = n - j; k >= 0; --k) {
+            dp = dp * y + p;
+            p = p * y + (R)Bm[(j + k) * w + j];
+        }
+        DX = DX * x + Z;
+        Z = Z * x + p;
+        DY = DY * x + dp;
+    }
+    *z = Z;
+    *dzdx = DX;
+    *dzdy = DY;
+}
+
 // Numerical-Recipes splint (coretrace/mathproc.cpp:244-276)
 template <typename R>
 STD bool splint(const double *xa, const double *ya, const double *y2a, int n, R x, R *y, R *dydx) {
@@ -828,7 +855,8 @@ STD int surface_eval(const DevElem &e, const double *aux, R X[3], R *F, R DF[3])
     }
     case SURF_ZERNIKE: {
         R zz, dx, dy;
-        zernike<R>(e, aux, x, y, &zz, &dx, &dy, true);
+        if constexpr (sizeof(R) == 4) zernike_horner<R>(e, aux, x, y, &zz, &dx, &dy);
+        else zernike<R>(e, aux, x, y, &zz, &dx, &dy, true);
         X[2] = zz;
         *F = z - zz;
         DF[0] = -dx; DF[1] = -dy; DF[2] = 1;

@@ -788,13 +788,15 @@ __device__ __forceinline__ void trace_one(const TraceArgs &a, const int t) {
     }
 }
 
-// Waves per SIMD (WV): the generic variant needs the whole register file;
-// the basic FP32 variant runs 4 for the sun stage (128 registers: its spills
-// cost less than the latency the fourth wave hides; 5 waves spill too much)
-// and 3 for later stages, whose rays spend less time waiting on the BVH;
-// the basic FP64 variant 2 (3 is slower).
+// Waves per SIMD (WV): the generic variant runs 2 (its registers spill,
+// but a second wave hides more latency than the spills cost; the one with
+// finite elements keeps the whole register file); the basic FP32 variant
+// runs 4 for the sun stage (128 registers: its spills cost less than the
+// latency the fourth wave hides; 5 waves spill too much) and 3 for later
+// stages, whose rays spend less time waiting on the BVH; the basic FP64
+// variant 2 (3 is slower).
 template <typename R, int G, int WV>
-__global__ __launch_bounds__(kBlock, G ? 1 : WV) void trace_stage_kernel(TraceArgs a) {
+__global__ __launch_bounds__(kBlock, WV) void trace_stage_kernel(TraceArgs a) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= a.n_rays) return;
     trace_one<R, G>(a, t);
@@ -855,7 +857,7 @@ __global__ __launch_bounds__(kBlock) void sunshape_batch_kernel(const SunPreArgs
 // A stage of several scenes: a tile is q.tile rays of one scene, traced by
 // one wave (lane l takes rays l, l + 64, ...).
 template <typename R, int G, int WV>
-__global__ __launch_bounds__(kBlock, G ? 1 : WV) void trace_batch_kernel(const TraceArgs *__restrict__ items,
+__global__ __launch_bounds__(kBlock, WV) void trace_batch_kernel(const TraceArgs *__restrict__ items,
                                                                          BatchQueue q) {
     const uint32_t total = q.prefix[q.n];
     const uint32_t lane = threadIdx.x & 63;
@@ -889,6 +891,8 @@ void launch_trace_batch(const TraceArgs *items, const BatchQueue &q, int generic
                         hipStream_t s) {
     if (generic == 2)
         hipLaunchKernelGGL((trace_batch_kernel<R, 2, 1>), dim3(grid), dim3(kBlock), 0, s, items, q);
+    else if (generic && shape.waves >= 2)
+        hipLaunchKernelGGL((trace_batch_kernel<R, 1, 2>), dim3(grid), dim3(kBlock), 0, s, items, q);
     else if (generic)
         hipLaunchKernelGGL((trace_batch_kernel<R, 1, 1>), dim3(grid), dim3(kBlock), 0, s, items, q);
     else if constexpr (sizeof(R) == 8) {
@@ -921,7 +925,7 @@ template <typename R>
 void launch_trace_stage(const TraceArgs &a, hipStream_t s) {
     const int blocks = (a.n_rays + kBlock - 1) / kBlock;
     if (a.generic == 2) hipLaunchKernelGGL((trace_stage_kernel<R, 2, 1>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else if (a.generic) hipLaunchKernelGGL((trace_stage_kernel<R, 1, 1>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else if (a.generic) hipLaunchKernelGGL((trace_stage_kernel<R, 1, 2>), dim3(blocks), dim3(kBlock), 0, s, a);
     else if constexpr (sizeof(R) == 8)
         hipLaunchKernelGGL((trace_stage_kernel<R, 0, 2>), dim3(blocks), dim3(kBlock), 0, s, a);
     else if (a.si == 0) hipLaunchKernelGGL((trace_stage_kernel<R, 0, 4>), dim3(blocks), dim3(kBlock), 0, s, a);
