@@ -223,6 +223,18 @@ int st_amd_mlp_forward(void *net, const float *x, int batch, float *y, void *str
  * before the step. */
 int st_amd_mlp_train_step(void *net, const float *x, const unsigned short *t, int batch, float lr, double *loss,
                           void *stream);
+/* `steps` Adam steps on batches of a resident training set, replayed from
+ * a HIP graph: one step (batch gather, forward, loss, backward, Adam) is
+ * captured once per call and launched `steps` times on `stream`, reading
+ * its batch and learning rate at a device step counter.  x [rows][n_in]
+ * (f32) and t [rows][n_out] (BF16 bits) hold the set; idx [steps][batch]
+ * (int32) the rows of each step's batch (clamped into [0, rows)); lr
+ * [steps] (f32) each step's learning rate; all DEVICE pointers.  The steps
+ * equal `steps` calls of st_amd_mlp_train_step on the same batches bit for
+ * bit.  loss (host, may be NULL) receives the loss of the last step's batch
+ * before its update.  Synchronises. */
+int st_amd_mlp_fit(void *net, const float *x, const unsigned short *t, long long rows, const int *idx, int batch,
+                   int steps, const float *lr, double *loss, void *stream);
 /* Loss and its gradient with respect to every parameter (host, exchange
  * order) at the current weights, without a step.  Synchronises. */
 int st_amd_mlp_gradients(void *net, const float *x, const unsigned short *t, int batch, float *grad, double *loss,

@@ -198,8 +198,9 @@ __global__ __launch_bounds__(kLossBlock) void mse_kernel(const float *Y, const u
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// Bias gradient partials: out[s][n] = sum of dZ[r][n] over row block s.
-constexpr int kColRows = 256;
+// Bias gradient partials: out[s][n] = sum of dZ[r][n] over row block s
+// (16 rows, so a batch of 512 gives 32 blocks of rows per column block).
+constexpr int kColRows = 16;
 __global__ void colsum_kernel(const uint16_t *dZ, long ld, int M, int N, float *out) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
@@ -207,6 +208,21 @@ __global__ void colsum_kernel(const uint16_t *dZ, long ld, int M, int N, float *
     float s = 0.0f;
     for (int r = r0; r < r1; r++) s += bf2f(dZ[(long)r * ld + n]);
     out[(long)blockIdx.y * N + n] = s;
+}
+
+// Split-K partials of a masked product (f32 [splits][M][N], c_split apart)
+// summed in split order, the ReLU mask applied and converted, as the
+// EPI_MASK_BF16 epilogue does for one split.
+__global__ void mask_sum_kernel(const float *part, int splits, long c_split, int M, int N, const uint16_t *mask,
+                                long ldm, uint16_t *C, long ldc) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)M * N) return;
+    const int r = (int)(i / N), c = (int)(i % N);
+    float v = 0.0f;
+    for (int k = 0; k < splits; k++) v += part[(long)k * c_split + (long)r * N + c];
+    const uint16_t h = mask[(long)r * ldm + c];
+    const bool on = h != 0 && !(h & 0x8000u);
+    C[(long)r * ldc + c] = on ? f2bf(v) : (uint16_t)0;
 }
 
 // One tensor of an Adam launch: parameters, their bf16 copy, moments and
@@ -223,6 +239,9 @@ struct AdamArgs {
     Seg seg[kMaxSeg];
     int nseg;
     float lr, b1, b2, eps, bc1, bc2;
+    // graph replays: lr, bc1, bc2 of step *step_ctr from table (3 floats a step)
+    const float *table;
+    const int *step_ctr;
     int update;   // 0: only sum the partials into grad_out
     float *grad_out;  // (may be null) summed gradient, segments laid end to end
     long grad_off[kMaxSeg];
@@ -237,11 +256,18 @@ __global__ void adam_kernel(AdamArgs a) {
     for (int k = 0; k < q.nsplit; k++) g += q.g[(long)k * q.gstride + i];
     if (a.grad_out) a.grad_out[a.grad_off[s] + i] = g;
     if (!a.update) return;
+    float lr = a.lr, bc1 = a.bc1, bc2 = a.bc2;
+    if (a.table) {
+        const float *t = a.table + 3 * (long)*a.step_ctr;
+        lr = t[0];
+        bc1 = t[1];
+        bc2 = t[2];
+    }
     const float m = a.b1 * q.m[i] + (1.0f - a.b1) * g;
     const float v = a.b2 * q.v[i] + (1.0f - a.b2) * g * g;
     q.m[i] = m;
     q.v[i] = v;
-    const float p = q.p[i] - a.lr * (m / a.bc1) / (sqrtf(v / a.bc2) + a.eps);
+    const float p = q.p[i] - lr * (m / bc1) / (sqrtf(v / bc2) + a.eps);
     q.p[i] = p;
     if (q.pb) q.pb[i] = f2bf(p);  // biases have no bf16 copy: the epilogue reads the f32 value
 }
@@ -250,6 +276,27 @@ __global__ void to_bf16_kernel(const float *p, uint16_t *pb, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) pb[i] = f2bf(p[i]);
 }
+
+// The batch of step *step_ctr of a resident training set: rows
+// idx[step * b + r] of x (f32 [rows][n_in]) packed as pack_input_kernel
+// packs them, and of t (bf16 [rows][n_out]) into tb; padding rows of tb zero.
+// An index outside [0, rows) is clamped into it.
+__global__ void gather_batch_kernel(const float *x, const uint16_t *t, const int *idx, const int *step_ctr, int b,
+                                    long rows, int n_in, int n_out, int M, uint16_t *X, uint16_t *tb) {
+    const long per_row = kInPad + n_out;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)M * per_row) return;
+    const int r = (int)(i / per_row), c = (int)(i % per_row);
+    long src = -1;
+    if (r < b) {
+        src = (long)idx[(long)*step_ctr * b + r];
+        src = src < 0 ? 0 : (src >= rows ? rows - 1 : src);
+    }
+    if (c < kInPad) X[(long)r * kInPad + c] = (src >= 0 && c < n_in) ? f2bf(x[src * n_in + c]) : (uint16_t)0;
+    else tb[(long)r * n_out + (c - kInPad)] = src >= 0 ? t[src * n_out + (c - kInPad)] : (uint16_t)0;
+}
+__global__ void set_int_kernel(int *p, int v) { *p = v; }
+__global__ void bump_kernel(int *p) { *p += 1; }
 
 // ------------------------------------------------------------- host -----
 inline void hc(hipError_t e, const char *what) {
@@ -267,6 +314,10 @@ struct Buf {
         if (p) (void)hipFree(p);
     }
     template <typename T> T *as() const { return (T *)p; }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
     void alloc(size_t bytes) {
         MC(hipMalloc(&p, std::max<size_t>(bytes, 16)));
         MC(hipMemset(p, 0, std::max<size_t>(bytes, 16)));
@@ -284,11 +335,21 @@ struct Net {
     int n_in, width, depth, n_out, max_batch, Mcap, rowblocks;
     std::vector<Layer> L;
     Buf X, Y, dZ[2], loss_part;
+    Buf dXpart;          // split-K partials of the widest layer's input gradient
+    int dx_splits = 1;
     std::vector<Buf> H;  // depth hidden activations
     int rowblocks_used = 1;  // bias partials the last backward pass wrote
     int step = 0;
     int lossblocks = 256;
     std::vector<float> part_host;
+    // graph replays (fit): the gathered targets, the step counter, the
+    // per-step table, and a private stream for capture
+    Buf Tb, ctr, table;
+    size_t table_steps = 0;
+    hipStream_t cap = nullptr;
+    ~Net() {
+        if (cap) (void)hipStreamDestroy(cap);
+    }
 };
 
 thread_local std::string g_error;
@@ -314,7 +375,7 @@ int rows_padded(int b) { return (b + kTM - 1) / kTM * kTM; }
 // writes f32 rows < b to y (ldc n_out) when y is given, else to net.Y.
 void forward(Net &n, const float *x, int b, float *y, hipStream_t s) {
     const int M = rows_padded(b);
-    {
+    if (x) {  // x == nullptr: net.X already holds the packed batch
         const int tot = M * kInPad;
         hipLaunchKernelGGL(pack_input_kernel, dim3((tot + 255) / 256), dim3(256), 0, s, x, b, n.n_in, M,
                            n.X.as<uint16_t>());
@@ -393,7 +454,9 @@ void backward(Net &n, const uint16_t *t, int b, hipStream_t s) {
             n.rowblocks_used = used;
         }
         if (l > 0) {
-            // dH[M][K] = dZ W, masked by H > 0: A = dZ [M][N], B = W stored [N][Kp] (transposed)
+            // dH[M][K] = dZ W, masked by H > 0: A = dZ [M][N], B = W stored [N][Kp] (transposed).
+            // A long reduction (the output layer's) is split over blockIdx.z
+            // into f32 partials, summed and masked by a second pass.
             GemmArgs g{};
             g.A = dz;
             g.lda = lddz;
@@ -402,20 +465,37 @@ void backward(Net &n, const uint16_t *t, int b, hipStream_t s) {
             g.M = M;
             g.N = ly.Kp;
             g.K = ly.N;
-            g.kchunk = ly.N;
-            g.mask = hin;
-            g.ldm = ldh;
-            g.C = n.dZ[cur ^ 1].p;
-            g.ldc = ly.Kp;
-            gemm<false, true, EPI_MASK_BF16>(g, 1, s);
+            const int splits = (ly.N >= 1024 && n.dx_splits > 1) ? n.dx_splits : 1;
+            g.kchunk = ly.N / splits;
+            if (splits > 1) {
+                g.C = n.dXpart.p;
+                g.ldc = ly.Kp;
+                g.c_split = (long)M * ly.Kp;
+                gemm<false, true, EPI_F32>(g, splits, s);
+                const long tot = (long)M * ly.Kp;
+                hipLaunchKernelGGL(mask_sum_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                                   n.dXpart.as<float>(), splits, g.c_split, M, ly.Kp, hin, ldh,
+                                   n.dZ[cur ^ 1].as<uint16_t>(), (long)ly.Kp);
+                MC(hipGetLastError());
+            } else {
+                g.kchunk = ly.N;
+                g.mask = hin;
+                g.ldm = ldh;
+                g.C = n.dZ[cur ^ 1].p;
+                g.ldc = ly.Kp;
+                gemm<false, true, EPI_MASK_BF16>(g, 1, s);
+            }
             cur ^= 1;
         }
     }
 }
 
 // Sum the gradient partials; with update, take an Adam step.
-void adam(Net &n, bool update, float lr, float *grad_out, hipStream_t s) {
+void adam(Net &n, bool update, float lr, float *grad_out, hipStream_t s, const float *table = nullptr,
+          const int *step_ctr = nullptr) {
     AdamArgs a{};
+    a.table = table;
+    a.step_ctr = step_ctr;
     int blocks = 0;
     long off = 0;
     auto add = [&](float *p, float *m, float *v, uint16_t *pb, const float *g, long cnt, long gstride, int nsplit) {
@@ -515,6 +595,12 @@ Net *create(int n_in, int width, int depth, int n_out, int max_batch, unsigned l
     n->Y.alloc((size_t)n->Mcap * n_out * sizeof(float));
     for (Buf &d : n->dZ) d.alloc((size_t)n->Mcap * std::max(width, n_out) * sizeof(uint16_t));
     n->loss_part.alloc(n->lossblocks * sizeof(float));
+    // split-K of the output layer's input gradient: eight parts of n_out / 8
+    // (a multiple of the 32-deep reduction step) when n_out allows it
+    if (n_out >= 1024 && (n_out / 8) % kTK == 0) {
+        n->dx_splits = 8;
+        n->dXpart.alloc((size_t)8 * n->Mcap * width * sizeof(float));
+    }
     n->part_host.resize(n->lossblocks);
     MC(hipDeviceSynchronize());
     return n.release();
@@ -593,6 +679,86 @@ long padded_count(const Net &n) {
 
 void check_batch(const Net &n, int b) {
     if (b < 1 || b > n.max_batch) throw std::runtime_error("mlp: batch outside 1..max_batch");
+}
+
+// `steps` Adam steps on batches of a resident training set: one step
+// (gather, forward, loss, backward, Adam, counter) is captured into a HIP
+// graph on a private stream and replayed `steps` times on s.  The step's
+// batch, learning rate and bias corrections are read on the device at the
+// step counter, so every replay is the same graph.  The bias corrections
+// are computed here as the eager step computes them, so the two paths give
+// the same bits.
+double fit(Net &n, const float *x, const uint16_t *t, long rows, const int *idx, int b, int steps, const float *lr,
+           bool want_loss, hipStream_t s) {
+    if (steps < 1 || rows < 1) throw std::runtime_error("mlp: fit needs steps >= 1 and rows >= 1");
+    check_batch(n, b);
+    const int M = rows_padded(b);
+    if (!n.Tb.p) n.Tb.alloc((size_t)n.Mcap * n.n_out * sizeof(uint16_t));
+    if (!n.ctr.p) n.ctr.alloc(sizeof(int));
+    if (n.table_steps < (size_t)steps) {
+        n.table.release();
+        n.table.alloc((size_t)steps * 3 * sizeof(float));
+        n.table_steps = steps;
+    }
+    if (!n.cap) MC(hipStreamCreateWithFlags(&n.cap, hipStreamNonBlocking));
+    // per-step lr and bias corrections (the host's powf, as adam() uses)
+    std::vector<float> lr_h(steps), tab(3 * (size_t)steps);
+    MC(hipMemcpyAsync(lr_h.data(), lr, steps * sizeof(float), hipMemcpyDeviceToHost, s));
+    MC(hipStreamSynchronize(s));
+    for (int k = 0; k < steps; k++) {
+        const int st = n.step + 1 + k;
+        tab[3 * k] = lr_h[k];
+        tab[3 * k + 1] = 1.0f - powf(0.9f, (float)st);
+        tab[3 * k + 2] = 1.0f - powf(0.999f, (float)st);
+    }
+    MC(hipMemcpyAsync(n.table.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, n.ctr.as<int>(), 0);
+    MC(hipGetLastError());
+    // the private stream starts after everything queued on s so far
+    hipEvent_t ev;
+    MC(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    MC(hipEventRecord(ev, s));
+    MC(hipStreamWaitEvent(n.cap, ev, 0));
+    MC(hipStreamSynchronize(n.cap));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto cleanup = [&]() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipEventDestroy(ev);
+    };
+    try {
+        MC(hipStreamBeginCapture(n.cap, hipStreamCaptureModeThreadLocal));
+        try {
+            const long tot = (long)M * (kInPad + n.n_out);
+            hipLaunchKernelGGL(gather_batch_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, n.cap, x, t,
+                               idx, n.ctr.as<int>(), b, rows, n.n_in, n.n_out, M, n.X.as<uint16_t>(),
+                               n.Tb.as<uint16_t>());
+            MC(hipGetLastError());
+            forward(n, nullptr, b, nullptr, n.cap);
+            backward(n, n.Tb.as<uint16_t>(), b, n.cap);
+            adam(n, true, 0.0f, nullptr, n.cap, n.table.as<float>(), n.ctr.as<int>());
+            hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(1), 0, n.cap, n.ctr.as<int>());
+            MC(hipGetLastError());
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(n.cap, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        MC(hipStreamEndCapture(n.cap, &graph));
+        MC(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        for (int k = 0; k < steps; k++) MC(hipGraphLaunch(exec, s));
+        n.step += steps;  // adam() counted one step at capture
+        n.step -= 1;
+        double loss = want_loss ? read_loss(n, b, s) : 0.0;
+        MC(hipStreamSynchronize(s));
+        cleanup();
+        return loss;
+    } catch (...) {
+        cleanup();
+        throw;
+    }
 }
 
 }  // namespace mlp
@@ -683,6 +849,16 @@ STMLP_API int st_amd_mlp_gradients(void *net, const float *x, const unsigned sho
         if (loss) *loss = read_loss(n, batch, s);
         MC(hipStreamSynchronize(s));
         unpad_grad(n, g, grad);
+        return 1;
+    });
+}
+STMLP_API int st_amd_mlp_fit(void *net, const float *x, const unsigned short *t, long long rows, const int *idx,
+                             int batch, int steps, const float *lr, double *loss, void *stream) {
+    return guarded([&] {
+        if (!net || !x || !t || !idx || !lr) return -1;
+        Net &n = *reinterpret_cast<Net *>(net);
+        const double l = fit(n, x, t, (long)rows, idx, batch, steps, lr, loss != nullptr, as_stream(stream));
+        if (loss) *loss = l;
         return 1;
     });
 }

@@ -45,6 +45,8 @@ def _mlp_sigs(dll):
         "st_amd_mlp_train_step": (C.c_int, [vp, vp, vp, C.c_int, C.c_float, C.POINTER(C.c_double), vp]),
         "st_amd_mlp_gradients": (C.c_int, [vp, vp, vp, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_double),
                                            vp]),
+        "st_amd_mlp_fit": (C.c_int, [vp, vp, vp, C.c_longlong, vp, C.c_int, C.c_int, vp, C.POINTER(C.c_double),
+                                     vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(dll, name)
@@ -124,6 +126,21 @@ class Mlp:
         self._ok(self.dll.st_amd_mlp_train_step(self.h, self._dev(x, torch.float32), self._dev(t, torch.bfloat16),
                                                 x.shape[0], lr, C.byref(loss) if want_loss else None,
                                                 _stream()), "train_step")
+        return loss.value if want_loss else None
+
+    def fit(self, X, T, idx, lr, want_loss: bool = False):
+        """len(lr) Adam steps replayed from a HIP graph: step k trains on rows
+        idx[k] (cuda int32 (steps, batch)) of X (cuda float32 (rows, n_in))
+        and T (cuda bfloat16 (rows, n_out)) at learning rate lr[k] (cuda
+        float32).  Returns the last step's loss when asked."""
+        import torch
+        steps, batch = idx.shape
+        assert lr.shape == (steps,) and X.shape[0] == T.shape[0]
+        loss = C.c_double(0.0)
+        self._ok(self.dll.st_amd_mlp_fit(self.h, self._dev(X, torch.float32), self._dev(T, torch.bfloat16),
+                                         X.shape[0], self._dev(idx, torch.int32), batch, steps,
+                                         self._dev(lr, torch.float32), C.byref(loss) if want_loss else None,
+                                         _stream()), "fit")
         return loss.value if want_loss else None
 
     def gradients(self, x, t):
@@ -358,14 +375,18 @@ class Surrogate:
         g.manual_seed(seed)
         t0 = time.perf_counter()
         hist = []
-        for s in range(steps):
-            idx = torch.randint(0, n_tr, (batch,), device="cuda", generator=g)
-            a = lr_end + 0.5 * (lr - lr_end) * (1 + math.cos(math.pi * s / max(1, steps - 1)))
-            want = log is not None and (s % 500 == 0 or s == steps - 1)
-            loss = self.net.train_step(X.index_select(0, idx), T.index_select(0, idx), a, want_loss=want)
-            if want:
-                hist.append((s, loss))
-                log(f"train: step {s}, loss {loss:.4e}, {time.perf_counter() - t0:.1f} s")
+        # batches and the cosine schedule on the device; the steps run as
+        # replays of one captured step, in chunks between loss readings
+        sched = torch.tensor([lr_end + 0.5 * (lr - lr_end) * (1 + math.cos(math.pi * s / max(1, steps - 1)))
+                              for s in range(steps)], dtype=torch.float32, device="cuda")
+        chunk = 2000
+        for s0 in range(0, steps, chunk):
+            k = min(chunk, steps - s0)
+            idx = torch.randint(0, n_tr, (k, batch), device="cuda", generator=g, dtype=torch.int32)
+            loss = self.net.fit(X, T, idx, sched[s0:s0 + k].contiguous(), want_loss=log is not None)
+            if log is not None:
+                hist.append((s0 + k - 1, loss))
+                log(f"train: step {s0 + k - 1}, loss {loss:.4e}, {time.perf_counter() - t0:.1f} s")
         torch.cuda.synchronize()
         train_s = time.perf_counter() - t0
         ev = self.evaluate(*(val if val is not None else (X[-2048:], T[-2048:])))

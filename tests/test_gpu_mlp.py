@@ -115,3 +115,34 @@ def test_training_reduces_loss_and_params_round_trip(hip_lib):
     assert torch.equal(y1, y2)
     net.close()
     net2.close()
+
+
+def test_graph_fit_equals_eager_steps(hip_lib):
+    """st_amd_mlp_fit replays one captured training step from a HIP graph;
+    its parameters and loss must equal the same steps taken one eager
+    train_step call at a time, bit for bit, after every call (each call
+    captures a new graph and replays it; eager work runs in between)."""
+    import torch
+    shape = (8, 128, 3, 256)
+    rng = np.random.default_rng(4)
+    rows = 1000
+    X = torch.from_numpy(rng.uniform(-1, 1, (rows, 8)).astype(np.float32)).cuda()
+    tb = mo.bf16_bits(rng.normal(0, 0.5, (rows, 256)).astype(np.float32))
+    T = torch.from_numpy(tb.view(np.int16)).cuda().view(torch.bfloat16)
+    g = _mlp(hip_lib, shape, max_batch=256, seed=5)
+    e = _mlp(hip_lib, shape, max_batch=256, seed=5)
+    for steps in (7, 1, 9):
+        idx = torch.from_numpy(rng.integers(0, rows, (steps, 200)).astype(np.int32)).cuda()
+        lr = torch.from_numpy(np.linspace(1e-3, 5e-4, steps).astype(np.float32)).cuda()
+        loss_g = g.fit(X, T, idx, lr, want_loss=True)
+        loss_e = None
+        for k in range(steps):
+            rows_k = idx[k].long()
+            loss_e = e.train_step(X.index_select(0, rows_k).contiguous(), T.index_select(0, rows_k).contiguous(),
+                                  float(lr[k].item()), want_loss=(k == steps - 1))
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(g.get_params(0), e.get_params(0))
+        np.testing.assert_array_equal(g.get_params(1), e.get_params(1))
+        assert loss_g == loss_e
+    g.close()
+    e.close()
