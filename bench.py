@@ -46,6 +46,7 @@ rays split evenly, seeds seed, seed+123, seed+369, ...), on a bounded sample
 of the same scene, and on one thread.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-workloads] [--no-batch] [--no-surrogate]
+       [--only batched|surrogate|mlp]
 """
 from __future__ import annotations
 
@@ -450,6 +451,30 @@ def batch_workload(lib, seed, sync, log, n_scenes=256, bits=32, reps=2):
     }
 
 
+def mlp_micro(batch=512, steps=200, rows=20_000):
+    """The surrogate network's training step alone: `steps` graph-replayed
+    Adam steps of the 8 -> 6 x 256 -> 4096 network on random data, timed
+    with HIP events on torch's stream (where they launch)."""
+    import torch
+    from soltrace_amd.surrogate import Mlp
+    net = Mlp(_lib(), 8, 256, 6, 4096, max_batch=1024, seed=1)
+    X = torch.rand(rows, 8, device="cuda") * 2 - 1
+    T = torch.rand(rows, 4096, device="cuda").to(torch.bfloat16)
+    idx = torch.randint(0, rows, (steps, batch), device="cuda", dtype=torch.int32)
+    lr = torch.full((steps,), 1e-4, device="cuda")
+    net.fit(X, T, idx, lr)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    net.fit(X, T, idx, lr)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    macs = 8 * 256 + 5 * 256 * 256 + 256 * 4096
+    tf = 3 * 2 * batch * macs / (ms * 1e-3) / 1e12
+    net.close()
+    return {"batch": batch, "steps": steps, "train_step_ms": ms, "tflops": tf, "frac_bf16": tf / 2500.0}
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of known size: what FETCH_SIZE and
     WRITE_SIZE read for it gives the factor between those counters and
@@ -476,6 +501,8 @@ def main():
     ap.add_argument("--no-workloads", action="store_true", help="only the tower workload")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the flux-surrogate workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scene workload")
+    ap.add_argument("--only", choices=["batched", "surrogate", "mlp"], default=None,
+                    help="run one sub-workload alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (scale for the HBM byte counters)")
     args = ap.parse_args()
@@ -491,6 +518,16 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     sync = torch.cuda.synchronize
+    if args.only:
+        log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+        if args.only == "batched":
+            out = batch_workload(_lib(), args.seed, sync, log)
+        elif args.only == "surrogate":
+            out = surrogate_workload(_lib(), args.seed, log)
+        else:
+            out = mlp_micro()
+        print(json.dumps({args.only: out}))
+        return
 
     lib = _lib()
     scene = workloads.tower(n_helio=1000, raycount=args.hits)

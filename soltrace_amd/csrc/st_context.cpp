@@ -194,6 +194,8 @@ bool Context::prepare() {
         v.bvh_items = upload(d_bvh_items, cs.bvh_items, s);
         v.grid_cells = upload(d_grid_cells, cs.grid_cells, s);
         v.grid_items = upload(d_grid_items, cs.grid_items, s);
+        v.grid_always = upload(d_grid_always, cs.grid_always, s);
+        v.n_always = (int32_t)cs.grid_always.size();
         v.foot = upload(d_foot, cs.foot, s);
         v.grid = cs.grid;
         v.use_grid = cs.use_grid ? 1 : 0;

@@ -18,6 +18,8 @@ struct SceneView {
     const int32_t *bvh_items;
     const int32_t *grid_cells;   // ncell + 1 starts
     const int32_t *grid_items;
+    const int32_t *grid_always;  // stage-1 elements every sun ray is tested against
+    int32_t n_always;
     const DevFoot *foot;         // per stage-1 element
     DevSunGrid grid;
     int32_t use_grid;

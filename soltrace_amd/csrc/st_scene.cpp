@@ -1183,6 +1183,7 @@ bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
     std::vector<Rect> rects;
     std::vector<double> sizes;
     out.foot.assign(2 * (size_t)std::max(1, s0.nelem), DevFoot{});
+    out.grid_always.clear();
     for (DevFoot &f : out.foot) f.kind = 2;
     for (int j = 0; j < s0.nelem; j++) {
         const DevElem &d = out.elems[s0.elem0 + j];
@@ -1190,7 +1191,7 @@ bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
         const int nb = out.enbox[s0.elem0 + j];
         if (nb == 0) {
             out.foot[2 * j].kind = 1;
-            rects.push_back({j, -HUGE_VAL, HUGE_VAL, -HUGE_VAL, HUGE_VAL});
+            out.grid_always.push_back(j);
             continue;
         }
         for (int b = 0; b < nb; b++) {

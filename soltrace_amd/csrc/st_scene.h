@@ -84,6 +84,9 @@ struct CompiledScene {
     std::vector<DevBvhNode> bvh;
     std::vector<int32_t> bvh_items;
     std::vector<int32_t> grid_cells, grid_items;
+    // stage-1 elements no footprint bounds (torus, finite element): tested
+    // for every sun ray, kept out of the grid's cell lists
+    std::vector<int32_t> grid_always;
     std::vector<DevFoot> foot;  // 2 per stage-1 element, one per hit box
     std::vector<double> ebox;   // per element: 2 element-frame hit boxes lo[3], hi[3]
     std::vector<int8_t> enbox;  // per element: boxes that bound its hits (0: none)

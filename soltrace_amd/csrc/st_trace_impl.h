@@ -154,11 +154,13 @@ STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3]
     b.fefail = 0;
     const int32_t *items = nullptr;  // nullptr: the item is the element index
     int k = 0, kend = 0;
-    bool bvh = false;
+    bool bvh = false, always = false;
     if (cell >= 0) {
+        // the cell's candidates, then the elements no footprint bounds
         items = sv.grid_items;
         k = sv.grid_cells[cell];
         kend = sv.grid_cells[cell + 1];
+        always = sv.n_always > 0;
     } else if (st.nbvh > 0) {
         bvh = true;
         items = sv.bvh_items;
@@ -182,6 +184,13 @@ STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3]
             ++k;
             if (screen && !foot_contains(sv.foot, j, sx, sy)) continue;
             consider<R, G>(sv, st, j, ps, cs, b);
+            continue;
+        }
+        if (always) {
+            always = false;
+            items = sv.grid_always;
+            k = 0;
+            kend = sv.n_always;
             continue;
         }
         if (!bvh) break;
@@ -388,6 +397,35 @@ __device__ __forceinline__ void mask_grid_wave(const MaskArgs &a, const uint64_t
         if (count >= 64) drain(64);
     }
     if (count > 0) drain(count);
+    // The elements no footprint bounds (torus, finite element), after the
+    // others: only for rays no element has accepted yet, as the mask asks
+    // whether any element accepts a ray.  Finite-element stages test every
+    // ray, for the least failing sun-ray index.
+    if (sv.n_always > 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        bool open[kMaskRays];
+#pragma unroll
+        for (int r = 0; r < kMaskRays; r++) {
+            const int slot = r * 64 + lane;
+            const bool hit = (hitw[wv][r * 2 + (lane >> 5)] >> (lane & 31)) & 1u;
+            open[r] = ray0 + (uint64_t)slot < a.n && (G == 2 || !hit);
+        }
+        for (int q = 0; q < sv.n_always; q++) {
+            const int j = sv.grid_always[q];
+#pragma unroll
+            for (int r = 0; r < kMaskRays; r++) {
+                const uint64_t bits = __ballot(open[r]);
+                if (open[r]) {
+                    const uint32_t pos = (head + count + (uint32_t)__popcll(bits & lanes_below())) & (kMaskQueue - 1);
+                    ring[wv][pos] = ((uint32_t)(r * 64 + lane) << 24) | (uint32_t)j;
+                }
+                count += (uint32_t)__popcll(bits);
+                if (count >= 64) drain(64);
+            }
+        }
+        if (count > 0) drain(count);
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (lane < kMaskRays) {
