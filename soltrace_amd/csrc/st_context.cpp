@@ -192,6 +192,7 @@ bool Context::prepare() {
         v.stages = upload(d_stages, cs.stages, s);
         v.bvh = upload(d_bvh, cs.bvh, s);
         v.bvh_items = upload(d_bvh_items, cs.bvh_items, s);
+        v.always_items = upload(d_always_items, cs.always_items, s);
         v.grid_cells = upload(d_grid_cells, cs.grid_cells, s);
         v.grid_items = upload(d_grid_items, cs.grid_items, s);
         v.grid_always = upload(d_grid_always, cs.grid_always, s);

@@ -16,6 +16,7 @@ struct SceneView {
     const DevStage *stages;
     const DevBvhNode *bvh;
     const int32_t *bvh_items;
+    const int32_t *always_items;  // a stage's elements no box bounds (DevStage::always0)
     const int32_t *grid_cells;   // ncell + 1 starts
     const int32_t *grid_items;
     const int32_t *grid_always;  // stage-1 elements every sun ray is tested against

@@ -839,6 +839,7 @@ bool compile_scene(SceneModel &m, bool use_cull, CompiledScene &out, std::string
         ds.multihit = st.multihit;
         ds.tracethrough = st.tracethrough;
         std::vector<BuildItem> items;
+        std::vector<int32_t> unbounded;
         for (size_t j = 0; j < st.elements.size(); j++) {
             HElement &e = st.elements[j];
             DevElem d{};
@@ -934,7 +935,8 @@ bool compile_scene(SceneModel &m, bool use_cull, CompiledScene &out, std::string
             if (d.surf != SURF_FLAT && d.surf > SURF_CYLINDER) ds.generic = std::max(ds.generic, 1);
             if (d.surf == SURF_FE) ds.generic = 2;
             out.elems.push_back(d);
-            for (int q = 0; e.enabled && q < std::max(nb, 1); q++) {
+            if (e.enabled && nb == 0) unbounded.push_back((int32_t)j);
+            for (int q = 0; e.enabled && q < nb; q++) {
                 // stage-frame box from the 8 corners of an element-frame box
                 BuildItem it;
                 double slo[3] = {1e300, 1e300, 1e300}, shi[3] = {-1e300, -1e300, -1e300};
@@ -951,16 +953,21 @@ bool compile_scene(SceneModel &m, bool use_cull, CompiledScene &out, std::string
                 }
                 for (int k = 0; k < 3; k++) {
                     double pad = 1e-2 + 1e-5 * std::max(fabs(slo[k]), fabs(shi[k]));
-                    it.lo[k] = nb > 0 ? round_down(slo[k] - pad) : -1e15f;
-                    it.hi[k] = nb > 0 ? round_up(shi[k] + pad) : 1e15f;
-                    it.c[k] = nb > 0 ? 0.5f * (it.lo[k] + it.hi[k]) : (float)(0.5 * (slo[k] + shi[k]));
+                    it.lo[k] = round_down(slo[k] - pad);
+                    it.hi[k] = round_up(shi[k] + pad);
+                    it.c[k] = 0.5f * (it.lo[k] + it.hi[k]);
                 }
                 it.idx = (int32_t)j;
                 items.push_back(it);
             }
         }
-        if (use_cull && items.size() > 8) {
+        // the BVH holds the bounded elements; the others (torus, finite
+        // element) are tested before every walk
+        if (use_cull && items.size() + unbounded.size() > 8 && !items.empty()) {
             ds.bvh0 = (int)out.bvh.size();
+            ds.always0 = (int)out.always_items.size();
+            ds.nalways = (int)unbounded.size();
+            out.always_items.insert(out.always_items.end(), unbounded.begin(), unbounded.end());
             std::vector<DevBvhNode> nodes;
             build_bvh(items, 0, (int)items.size(), nodes, out.bvh_items);
             ds.nbvh = (int)nodes.size();

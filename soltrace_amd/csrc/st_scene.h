@@ -83,6 +83,7 @@ struct CompiledScene {
     std::vector<DevStage> stages;
     std::vector<DevBvhNode> bvh;
     std::vector<int32_t> bvh_items;
+    std::vector<int32_t> always_items;  // per stage with a BVH: elements no box bounds
     std::vector<int32_t> grid_cells, grid_items;
     // stage-1 elements no footprint bounds (torus, finite element): tested
     // for every sun ray, kept out of the grid's cell lists

@@ -162,8 +162,11 @@ STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3]
         kend = sv.grid_cells[cell + 1];
         always = sv.n_always > 0;
     } else if (st.nbvh > 0) {
+        // the stage's unbounded elements (torus, finite element), then the walk
         bvh = true;
-        items = sv.bvh_items;
+        items = sv.always_items;
+        k = st.always0;
+        kend = st.always0 + st.nalways;
     } else {
         kend = st.nelem;
     }
@@ -194,6 +197,7 @@ STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3]
             continue;
         }
         if (!bvh) break;
+        items = sv.bvh_items;
         bool leaf = false;
         while (i < st.nbvh) {
             const DevBvhNode nd = nodes[i];

@@ -68,6 +68,7 @@ struct DevStage {
     int32_t elem0, nelem;   // slice of the element table
     int32_t virt, multihit, tracethrough;
     int32_t bvh0, nbvh;     // slice of the BVH node table (0 nodes: brute force)
+    int32_t always0, nalways;  // with a BVH: slice of the unbounded-element list, tested before the walk
     int32_t generic;        // 1: an element needs the torus or a Newton surface other than flat;
                             // 2: and one is a finite-element surface
 };
