@@ -56,6 +56,11 @@ int st_amd_flux_by_source(st_context_t pcxt, int on);
  * (stage-1 elements * nx * ny), or -1 when the last run kept none or n is
  * too small. */
 long long st_amd_flux_source_result(st_context_t pcxt, unsigned int *counts, long long n);
+/* The same maps copied device to device: counts is a DEVICE pointer on the
+ * context's GPU, and the copy is queued on `stream` (a hipStream_t, NULL:
+ * the null stream) without waiting.  The maps stay in HBM until the next
+ * run.  Returns the number of words, or -1 as above. */
+long long st_amd_flux_source_result_device(st_context_t pcxt, unsigned int *counts, long long n, void *stream);
 
 /* The GUI's flux statistics of element `elem` (0-based) of stage `stage`
  * over the last run's records (app/src/project.cpp:1139-1436,

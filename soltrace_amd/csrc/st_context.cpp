@@ -768,11 +768,8 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         flux_host.assign((size_t)fs.nx * fs.ny, 0u);
         HC(hipMemcpyAsync(flux_host.data(), d_flux.p, flux_host.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
                           s));
-        flux_src_host.assign(src_words, 0u);
-        if (by_src)
-            HC(hipMemcpyAsync(flux_src_host.data(), d_flux_src.p, src_words * sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, s));
         HC(hipStreamSynchronize(s));
+        flux_src_words = src_words;  // per-source maps stay in HBM until asked for
     }
     if (ctrs) {
         HC(hipMemcpyAsync(&timing.ev[0][0], ctrs, sizeof(timing.ev), hipMemcpyDeviceToHost, s));
@@ -1545,11 +1542,35 @@ STCORE_API int st_amd_flux_by_source(st_context_t p, int on) {
 }
 STCORE_API long long st_amd_flux_source_result(st_context_t p, unsigned int *counts, long long n) {
     CTX(p, -1);
-    if (!c->flux.active || c->flux_src_host.empty()) return -1;
-    const long long words = (long long)c->flux_src_host.size();
+    if (!c->flux.active || c->flux_src_words == 0) return -1;
+    const long long words = (long long)c->flux_src_words;
     if (counts != nullptr) {
         if (n < words) return -1;
-        memcpy(counts, c->flux_src_host.data(), (size_t)words * sizeof(uint32_t));
+        try {
+            HC(hipMemcpyAsync(counts, c->d_flux_src.p, (size_t)words * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              c->stream));
+            HC(hipStreamSynchronize(c->stream));
+        } catch (const std::exception &e) {
+            c->errlog("per-source map copy failed: %s", e.what());
+            return -1;
+        }
+    }
+    return words;
+}
+STCORE_API long long st_amd_flux_source_result_device(st_context_t p, unsigned int *counts, long long n,
+                                                      void *stream) {
+    CTX(p, -1);
+    if (!c->flux.active || c->flux_src_words == 0) return -1;
+    const long long words = (long long)c->flux_src_words;
+    if (counts != nullptr) {
+        if (n < words) return -1;
+        try {
+            HC(hipMemcpyAsync(counts, c->d_flux_src.p, (size_t)words * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                              reinterpret_cast<hipStream_t>(stream)));
+        } catch (const std::exception &e) {
+            c->errlog("per-source map copy failed: %s", e.what());
+            return -1;
+        }
     }
     return words;
 }

@@ -151,7 +151,7 @@ struct Context {
     // per-source maps: one flux map per stage-1 element (st_amd_flux_by_source)
     bool flux_by_source = false;
     DevBuf d_flux_src;
-    std::vector<uint32_t> flux_src_host;
+    size_t flux_src_words = 0;  // words of d_flux_src the last run filled (0: none)
     Timing timing;
 
     // stage-data save/resume (coretrace/stapi.cpp:680-728): rows to load in
@@ -201,6 +201,7 @@ struct Context {
                 if (st.pooled[k]) rec_pool.push_back(st.chunks[k]);
         recs.clear();
         cols_valid = false;
+        flux_src_words = 0;
         n_hits_done = 0;
         sun_ray_count = 0;
     }

@@ -215,8 +215,8 @@ class Field:
                                           C.c_double, C.c_double, C.c_double, C.c_double]
         d.st_amd_flux_result.argtypes = [C.c_void_p, C.POINTER(C.c_uint), C.POINTER(C.c_double)]
         d.st_amd_flux_by_source.argtypes = [C.c_void_p, C.c_int]
-        d.st_amd_flux_source_result.argtypes = [C.c_void_p, C.POINTER(C.c_uint), C.c_longlong]
-        d.st_amd_flux_source_result.restype = C.c_longlong
+        d.st_amd_flux_source_result_device.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p]
+        d.st_amd_flux_source_result_device.restype = C.c_longlong
         self.ctx = lib.st_create_context()
         if not self.ctx:
             raise RuntimeError("st_create_context failed")
@@ -301,13 +301,15 @@ class Field:
         total = total.cuda() * w
         imgs = None
         if want:
+            # device to device, on torch's stream: the maps never cross PCIe
             words = self.n * IMG * IMG
             if self._src is None:
-                self._src = np.empty(words, dtype=np.uint32)
-            got = self.dll.st_amd_flux_source_result(self.ctx, self._src.ctypes.data_as(C.POINTER(C.c_uint)), words)
+                self._src = torch.empty(words, dtype=torch.int32, device="cuda")
+            got = self.dll.st_amd_flux_source_result_device(self.ctx, C.c_void_p(self._src.data_ptr()), words,
+                                                            _stream())
             if got != words:
-                raise RuntimeError("st_amd_flux_source_result failed")
-            imgs = torch.from_numpy(self._src.view(np.int32)).cuda().view(self.n, IMG, IMG).to(torch.float32) * w
+                raise RuntimeError("st_amd_flux_source_result_device failed")
+            imgs = self._src.view(self.n, IMG, IMG).to(torch.float32) * w
         return imgs, total, p_ray
 
 

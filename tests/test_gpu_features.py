@@ -307,6 +307,16 @@ def test_flux_by_source_equals_oracle(hip_lib, oracle_lib, bits):
         src = (C.c_uint * (nsrc * nx * ny))()
         assert f(ctx, src, nsrc * nx * ny - 1) == -1  # too small
         assert f(ctx, src, nsrc * nx * ny) == nsrc * nx * ny
+        # the device-to-device copy gives the same words
+        import torch
+        fd = lib.dll.st_amd_flux_source_result_device
+        fd.restype = C.c_longlong
+        fd.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p]
+        dev = torch.zeros(nsrc * nx * ny, dtype=torch.int32, device="cuda")
+        assert fd(ctx, C.c_void_p(dev.data_ptr()), nsrc * nx * ny - 1, None) == -1  # too small
+        assert fd(ctx, C.c_void_p(dev.data_ptr()), nsrc * nx * ny, None) == nsrc * nx * ny
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(dev.cpu().numpy().view(np.uint32), np.frombuffer(src, dtype=np.uint32))
         return (np.frombuffer(buf, dtype=np.uint32).reshape(nx, ny).astype(np.float64),
                 np.frombuffer(src, dtype=np.uint32).reshape(nsrc, nx, ny).astype(np.float64))
 
