@@ -74,11 +74,16 @@ struct BatchWork {
     std::vector<std::shared_ptr<DevBuf>> arenas;  // record buffers; reused once no context holds them
     uint32_t *h = nullptr;                        // pinned staging for small reads
     size_t h_words = 0;
+    // pinned staging for the launches' small uploads (item lists, prefixes):
+    // filled front to back, reset whenever the lane's stream is idle
+    char *pin = nullptr;
+    size_t pin_cap = 0, pin_off = 0;
     int cus = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
     BatchTiming timing;
     ~BatchWork() {
         if (h) (void)hipHostFree(h);
+        if (pin) (void)hipHostFree(pin);
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
     }
@@ -107,9 +112,33 @@ struct BatchWork {
 
 namespace {
 
+// Upload a small host array through the lane's pinned staging (one DMA
+// copy, no pageable staging), in stream order.
 template <typename T>
-T *put(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
-    return upload(b, v, s);
+T *put(BatchWork &W, DevBuf &b, const std::vector<T> &v, hipStream_t s) {
+    constexpr size_t kPin = (size_t)4 << 20;
+    b.ensure(std::max<size_t>(1, v.size()) * sizeof(T));
+    if (v.empty()) return b.as<T>();
+    const size_t bytes = v.size() * sizeof(T), need = (bytes + 255) & ~(size_t)255;
+    if (need > kPin) return upload(b, v, s);
+    if (!W.pin) {
+        HC(hipHostMalloc((void **)&W.pin, kPin, hipHostMallocDefault));
+        W.pin_cap = kPin;
+        W.pin_off = 0;
+    }
+    if (W.pin_off + need > W.pin_cap) {  // earlier copies must finish before their staging is reused
+        HC(hipStreamSynchronize(s));
+        W.pin_off = 0;
+    }
+    memcpy(W.pin + W.pin_off, v.data(), bytes);
+    HC(hipMemcpyAsync(b.p, W.pin + W.pin_off, bytes, hipMemcpyHostToDevice, s));
+    W.pin_off += need;
+    return b.as<T>();
+}
+// The lane's stream is idle: its staging can be reused from the start.
+void idle(BatchWork &W, hipStream_t s) {
+    HC(hipStreamSynchronize(s));
+    W.pin_off = 0;
 }
 
 // Trace-launch shapes chosen per class: key = bits * 100 + generic * 10 +
@@ -178,18 +207,18 @@ static bool run_batch(std::vector<Scene> &sc, BatchWork &W, bool count, hipStrea
     // prefix positions -> exclusive prefix values, read to the host
     auto prefix_read = [&](const uint32_t *offs, const uint32_t *tmp, size_t len, const uint32_t *total,
                            const std::vector<uint32_t> &pos) -> std::vector<uint32_t> {
-        const uint32_t *dpos = put(W.pos, pos, s);
+        const uint32_t *dpos = put(W, W.pos, pos, s);
         uint32_t *dout = (uint32_t *)W.pre_out.ensure(pos.size() * sizeof(uint32_t));
         launch_prefix_at(offs, tmp, len, total, dpos, (int)pos.size(), dout, s);
         uint32_t *h = W.host(pos.size());
         HC(hipMemcpyAsync(h, dout, pos.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HC(hipStreamSynchronize(s));
+        idle(W, s);
         return std::vector<uint32_t>(h, h + pos.size());
     };
     auto check_errors = [&]() {
         uint32_t *h = W.host((size_t)n);
         HC(hipMemcpyAsync(h, err, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HC(hipStreamSynchronize(s));
+        idle(W, s);
         for (int k = 0; k < n; k++) {
             if (!h[k] || sc[k].result < 0) continue;
             Context *c = sc[k].c;
@@ -258,9 +287,9 @@ static bool run_batch(std::vector<Scene> &sc, BatchWork &W, bool count, hipStrea
                 }
                 if (acc) {
                     pfx.push_back(acc);
-                    BatchQueue q{put(W.prefix, pfx, s), (int32_t)items.size(), 1u, queue};
+                    BatchQueue q{put(W, W.prefix, pfx, s), (int32_t)items.size(), 1u, queue};
                     HC(hipEventRecord(W.ev[0], s));
-                    launch_sunshape_batch<R>(put(W.pre_items, items, s), q, acc, s);
+                    launch_sunshape_batch<R>(put(W, W.pre_items, items, s), q, acc, s);
                     HC(hipEventRecord(W.ev[1], s));
                     HC(hipEventSynchronize(W.ev[1]));
                     W.timing.sunshape_ms += elapsed();
@@ -323,12 +352,12 @@ static bool run_batch(std::vector<Scene> &sc, BatchWork &W, bool count, hipStrea
                         have = true;
                     }
                 }
-                const TraceArgs *ditems = put(W.trace_items, items, s);
+                const TraceArgs *ditems = put(W, W.trace_items, items, s);
                 auto launch = [&](const TraceShape &sh) {
                     std::vector<uint32_t> pfx(items.size() + 1, 0u);
                     for (size_t j = 0; j < items.size(); j++)
                         pfx[j + 1] = pfx[j] + ((uint32_t)items[j].n_rays + sh.tile - 1) / sh.tile;
-                    BatchQueue q{put(W.prefix, pfx, s), (int32_t)items.size(), (uint32_t)sh.tile, queue};
+                    BatchQueue q{put(W, W.prefix, pfx, s), (int32_t)items.size(), (uint32_t)sh.tile, queue};
                     launch_fill_u32(queue, 1, 0u, s);
                     launch_fill_u32(ovf_count, 1, 0u, s);
                     HC(hipEventRecord(W.ev[0], s));
@@ -404,10 +433,10 @@ static bool run_batch(std::vector<Scene> &sc, BatchWork &W, bool count, hipStrea
                     S.n_nxt += cnt;
                 }
                 pfx.push_back(acc);
-                BatchQueue q{put(W.prefix, pfx, s), (int32_t)items.size(), 1u, queue};
-                launch_copy_rays_batch(put(W.copy_items, items, s), q, acc, s);
+                BatchQueue q{put(W, W.prefix, pfx, s), (int32_t)items.size(), 1u, queue};
+                launch_copy_rays_batch(put(W, W.copy_items, items, s), q, acc, s);
             }
-            HC(hipStreamSynchronize(s));
+            idle(W, s);
         }
     };
 
@@ -470,10 +499,10 @@ static bool run_batch(std::vector<Scene> &sc, BatchWork &W, bool count, hipStrea
                     tp.push_back(tp.back() + (ma.n + 64 * kMaskRays - 1) / (64 * kMaskRays));
                 }
             if (items.empty()) continue;
-            BatchQueue q{put(W.prefix, tp, s), (int32_t)items.size(), 0u, queue};
+            BatchQueue q{put(W, W.prefix, tp, s), (int32_t)items.size(), 0u, queue};
             launch_fill_u32(queue, 1, 0u, s);
             HC(hipEventRecord(W.ev[0], s));
-            launch_mask_batch<R>(put(W.mask_items, items, s), q, gv, W.cus * 8, s);
+            launch_mask_batch<R>(put(W, W.mask_items, items, s), q, gv, W.cus * 8, s);
             HC(hipEventRecord(W.ev[1], s));
             HC(hipEventSynchronize(W.ev[1]));
             W.timing.mask_ms += elapsed();
@@ -492,9 +521,9 @@ static bool run_batch(std::vector<Scene> &sc, BatchWork &W, bool count, hipStrea
             take[j] = std::min<uint64_t>(h, S.N - S.hits);
             citems[j] = MaskCompactItem{S.s0, S.c->d_hits.as<uint64_t>() + S.hits, (uint32_t)take[j]};
         }
-        BatchQueue cq{put(W.prefix, wpfx, s), (int32_t)m, 0u, queue};
-        launch_mask_compact_batch(mask, offs, stmp, nwords, put(W.compact_items, citems, s), cq, s);
-        HC(hipStreamSynchronize(s));
+        BatchQueue cq{put(W, W.prefix, wpfx, s), (int32_t)m, 0u, queue};
+        launch_mask_compact_batch(mask, offs, stmp, nwords, put(W, W.compact_items, citems, s), cq, s);
+        idle(W, s);
         // sun-ray counts: up to the last hit taken when a scene is complete
         std::vector<Seg> segs;
         for (int j = 0; j < m; j++) {
@@ -503,7 +532,7 @@ static bool run_batch(std::vector<Scene> &sc, BatchWork &W, bool count, hipStrea
                 uint64_t last = 0;
                 HC(hipMemcpyAsync(&last, S.c->d_hits.as<uint64_t>() + S.hits + take[j] - 1, sizeof last,
                                   hipMemcpyDeviceToHost, s));
-                HC(hipStreamSynchronize(s));
+                idle(W, s);
                 S.sun_count = last + 1;
             } else {
                 S.sun_count = S.s0 + B[j];
@@ -521,7 +550,7 @@ static bool run_batch(std::vector<Scene> &sc, BatchWork &W, bool count, hipStrea
     {
         std::vector<unsigned long long> f(n);
         HC(hipMemcpyAsync(f.data(), fe_fail, (size_t)n * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-        HC(hipStreamSynchronize(s));
+        idle(W, s);
         for (int k = 0; k < n; k++)
             if (sc[k].result >= 0 && sc[k].c->cs.stages[0].generic == 2 && f[k] < sc[k].sun_count) {
                 sc[k].c->errlog("trace error: Interpolation failed to find nearby nodes to ray hit in FEA surface.");
@@ -540,10 +569,10 @@ static bool run_batch(std::vector<Scene> &sc, BatchWork &W, bool count, hipStrea
         std::vector<LastRayItem> items(n);
         for (int k = 0; k < n; k++) items[k] = LastRayItem{sc[k].nxt, sc[k].n_nxt};
         uint32_t *d = (uint32_t *)W.pre_out.ensure((size_t)n * sizeof(uint32_t));
-        launch_last_raynum(put(W.last_items, items, s), n, d, s);
+        launch_last_raynum(put(W, W.last_items, items, s), n, d, s);
         uint32_t *h = W.host((size_t)n);
         HC(hipMemcpyAsync(h, d, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HC(hipStreamSynchronize(s));
+        idle(W, s);
         return std::vector<uint32_t>(h, h + n);
     };
     {
@@ -586,7 +615,7 @@ static bool run_batch(std::vector<Scene> &sc, BatchWork &W, bool count, hipStrea
     if (ctr) {
         std::vector<unsigned long long> h((size_t)n * ctr_words);
         HC(hipMemcpyAsync(h.data(), ctr, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-        HC(hipStreamSynchronize(s));
+        idle(W, s);
         for (int k = 0; k < n; k++) {
             Timing &t = sc[k].c->timing;
             memcpy(&t.ev[0][0], h.data() + (size_t)k * ctr_words, sizeof(t.ev));

@@ -924,8 +924,20 @@ STD int surface_eval(const DevElem &e, const double *aux, R X[3], R *F, R DF[3])
 }
 
 // z of the surface at (x, y) for the start-plane choice
-// (coretrace/surfacezatxypair.cpp:56-280), double.
+// (coretrace/surfacezatxypair.cpp:56-280), double.  HORNER evaluates the
+// monomial surface in Horner form (the FP32 path: the start plane only
+// picks where Newton starts, and a 1-ulp difference of z cannot change that
+// choice for a ray nudged 1e-5 m off its last hit).
+template <bool HORNER = false>
 STD bool surface_z_at_xy(const DevElem &e, const double *aux, double x, double y, double *F) {
+    if constexpr (HORNER) {
+        if (e.surf == SURF_ZERNIKE) {
+            double z, dx, dy;
+            zernike_horner<double>(e, aux, x, y, &z, &dx, &dy);
+            *F = z;
+            return true;
+        }
+    }
     switch (e.surf) {
     case SURF_GENERAL: {
         double rho2 = (e.stype == 1) ? x * x + y * y : x * x;
@@ -1143,7 +1155,7 @@ STD bool element_hit(const DevElem &e, const double *aux, const double P[3], con
             if (!spline_start(e, aux, P, D, &ZStart)) return false;
         } else if (s != SURF_FLAT) {
             double FXY = 0.0;
-            surface_z_at_xy(e, aux, P[0], P[1], &FXY);
+            surface_z_at_xy<sizeof(R) == 4>(e, aux, P[0], P[1], &FXY);
             if (P[2] <= 0.0 && D[2] > 0.0) ZStart = 0.0;
             else if (P[2] <= FXY && D[2] > 0.0) ZStart = 0.0;
             else if (P[2] <= FXY && D[2] < 0.0 && P[2] > e.zap && za_in) ZStart = 0.0;
