@@ -275,6 +275,21 @@ void launch_last_raynum(const LastRayItem *items, int m, uint32_t *out, hipStrea
 
 
 
+// ----------------------------------------------------- ray order ---------
+__global__ void order_scatter_kernel(const uint32_t *key_rank, int n, const uint32_t *offsets,
+                                     const uint32_t *tile_off, uint32_t *order) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t key = key_rank[2 * i], rank = key_rank[2 * i + 1];
+    order[offsets[key] + tile_off[key / kScanTile] + rank] = (uint32_t)i;
+}
+void launch_order_scatter(const uint32_t *key_rank, int n, const uint32_t *offsets, const uint32_t *tile_off,
+                          uint32_t *order, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(order_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, key_rank, n, offsets, tile_off,
+                       order);
+}
+
 // ------------------------------------------------ merged record columns ---
 // The getters' view of a run's records (coretrace/stapi.cpp:564-659 over
 // TRayData::Merge, types.cpp:311-355): output record i comes from the piece

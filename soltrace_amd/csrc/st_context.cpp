@@ -387,7 +387,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     };
     auto trace_launch = [&](int si, int n, const uint64_t *sun_idx, uint32_t raynum_base, const DevRay *in,
                             uint32_t last_raynum, DevRay *next_rays, uint64_t *n_next,
-                            const SunPre *pre) -> Pending {
+                            const SunPre *pre, const uint32_t *order = nullptr) -> Pending {
         TraceArgs a{};
         a.sv = sv;
         a.si = si;
@@ -418,6 +418,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         a.counters = ctrs ? ctrs + 5 * (1 + si) : nullptr;
         a.generic = cs.stages[si].generic;
         a.sun_pre = pre;
+        a.order = order;
         if (si == 0 && presample && pre == nullptr) {
             SunPre *out = (SunPre *)d_sunpre.ensure((size_t)launch_cap * sizeof(SunPre));
             launch_presample(s, sun_idx, n, out);
@@ -489,6 +490,9 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         HC(hipStreamWaitEvent(sm, ev_ready, 0));
         uint64_t hits = 0, s0 = 0, batch = 0;
         int parity = 0;
+        // many stage-1 elements under a BVH: order each launch's rays by
+        // sun-plane bucket (the walks of a wave's reflected rays then agree)
+        const bool sort_rays = sv.use_grid && !sv.sun.point_source && cs.stages[0].nbvh > 0;
         double rate = 0.0;  // hits per sun ray observed so far
         auto plan = [&](uint64_t want) -> uint64_t {
             uint64_t b = (rate > 0) ? (uint64_t)(want / rate * 1.05) + 4096
@@ -522,6 +526,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             uint32_t take;
             uint64_t raynum_base;
             SunPre *pre;
+            const uint32_t *order;  // lane -> ray of the batch (nullptr: identity)
             hipEvent_t ev;
             bool more;           // another batch was launched behind this one
             uint64_t sun_count;  // sun rays drawn up to this batch's last hit (or its end)
@@ -545,6 +550,20 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             if (presample && r.take) {
                 r.pre = (SunPre *)d_sunpre_batch[parity].ensure((size_t)r.take * sizeof(SunPre));
                 launch_presample(sm, r.hit_list, (int)r.take, r.pre);
+            }
+            // the batch's rays in sun-plane bucket order, for a single launch
+            r.order = nullptr;
+            if (sort_rays && r.take && r.take <= launch_cap) {
+                uint32_t *kr = (uint32_t *)d_keyrank.ensure((size_t)r.take * 2 * sizeof(uint32_t));
+                uint32_t *hist = (uint32_t *)d_hist.ensure(kSunBuckets * sizeof(uint32_t));
+                uint32_t *hoffs = (uint32_t *)d_hoffs.ensure(kSunBuckets * sizeof(uint32_t));
+                uint32_t *htmp = (uint32_t *)d_htmp.ensure(scan_tmp_words(kSunBuckets) * sizeof(uint32_t));
+                uint32_t *htot = (uint32_t *)d_htot.ensure(sizeof(uint32_t));
+                uint32_t *ord = (uint32_t *)d_order[parity].ensure((size_t)r.take * sizeof(uint32_t));
+                launch_sun_buckets<R>(sv, seed, r.hit_list, (int)r.take, kr, hist, sm);
+                launch_tile_scan(hist, hoffs, kSunBuckets, htmp, htot, sm);
+                launch_order_scatter(kr, (int)r.take, hoffs, htmp, ord, sm);
+                r.order = ord;
             }
             if (r.take == want) {
                 uint64_t last = 0;
@@ -589,7 +608,8 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
                 // a single stage hands no rays on
                 Pending p = trace_launch(0, n, cur_b.hit_list + c, (uint32_t)(cur_b.raynum_base + c), nullptr, 0,
                                          nst > 1 ? next : nullptr, &n_next,
-                                         cur_b.pre != nullptr ? cur_b.pre + c : nullptr);
+                                         cur_b.pre != nullptr ? cur_b.pre + c : nullptr,
+                                         c == 0 && (uint32_t)n == cur_b.take ? cur_b.order : nullptr);
                 if (c == 0 && cur_b.more) {
                     next_b = prepare();  // overlaps the trace just queued
                     have_next = true;

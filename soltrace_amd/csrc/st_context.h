@@ -129,6 +129,7 @@ struct Context {
     DevBuf d_mask, d_popc, d_offs, d_scan_tmp, d_scan_tmp2, d_total, d_hits, d_rec_slots, d_rec_count, d_rec_offs, d_ovf_rec,
         d_ovf_meta, d_ovf_count, d_out_slots, d_out_flag, d_out_offs, d_flux;
     DevBuf d_rays_a, d_rays_b, d_sunpre, d_sunpre_batch[2];
+    DevBuf d_keyrank, d_hist, d_hoffs, d_htmp, d_htot, d_order[2];  // stage-0 ray order by sun-plane bucket
 
     // results
     std::vector<StageRecords> recs;

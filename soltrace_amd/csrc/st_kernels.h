@@ -89,6 +89,11 @@ struct TraceArgs {
     // uniforms drawn, records
     unsigned long long *counters;
     int32_t generic;             // kernel variant: DevStage::generic of stage si
+    // lane -> ray: lane i of the launch traces ray order[i] (nullptr: ray i).
+    // Every output is indexed by the ray, so the order changes no result;
+    // stage-0 launches order their rays by sun-plane bucket, so a wave's rays
+    // leave neighbouring mirrors in similar directions and walk the BVH alike.
+    const uint32_t *order;
 };
 
 // Arguments of the sun-ray hit-mask launch.
@@ -202,6 +207,19 @@ void launch_stats_bin(const DevRecord *rec, uint32_t n, int32_t elem, const Flux
                       uint32_t *bins, unsigned long long *acc, hipStream_t s);
 void launch_compact_rays(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, const uint32_t *tile_off,
                          int n_rays, DevRay *out, hipStream_t s);
+// Ray order of a stage-0 launch by sun-plane bucket (a counting sort):
+// the sun-ray bucket of each of the n rays (launch_sun_buckets, per
+// precision file) counted into hist (kSunBuckets words, zeroed here) with its
+// rank, then order[start of the bucket + rank] = ray.  Ranks come from
+// atomics, so the order within a bucket varies between runs; the results
+// do not.
+constexpr uint32_t kSunBucketSide = 256;
+constexpr uint32_t kSunBuckets = kSunBucketSide * kSunBucketSide;
+template <typename R>
+void launch_sun_buckets(const SceneView &sv, uint32_t seed, const uint64_t *sun_index, int n, uint32_t *key_rank,
+                        uint32_t *hist, hipStream_t s);
+void launch_order_scatter(const uint32_t *key_rank, int n, const uint32_t *offsets, const uint32_t *tile_off,
+                          uint32_t *order, hipStream_t s);
 // The getters' columns of a run's records in merged order: records
 // [dst, dst + n) of the output come from src, of stage `stage` (1-based);
 // pieces are sorted by dst and cover [0, total).

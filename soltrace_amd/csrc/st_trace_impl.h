@@ -839,9 +839,42 @@ __device__ __forceinline__ void trace_one(const TraceArgs &a, const int t) {
 // variant 2 (3 is slower).
 template <typename R, int G, int WV>
 __global__ __launch_bounds__(kBlock, WV) void trace_stage_kernel(TraceArgs a) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= a.n_rays) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n_rays) return;
+    const int t = a.order != nullptr ? (int)a.order[i] : i;
     trace_one<R, G>(a, t);
+}
+
+// Sun-plane bucket of stage-0 ray i (the point of its sun-ray index's two
+// sun-plane draws on a kSunBucketSide-square grid over the sun box, in Morton
+// order so neighbouring buckets stay close), counted into hist: key_rank
+// holds (bucket, rank within the bucket) for each ray.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void sun_bucket_kernel(DevSun sun, uint32_t seed, const uint64_t *sun_index,
+                                                            int n, uint32_t *key_rank, uint32_t *hist) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    Rng rng;
+    rng.begin(seed, 0, sun_index[i]);
+    double p[2];
+    sun_plane_point(rng, sun, p);
+    const double fx = (p[0] - sun.minx) / (sun.maxx - sun.minx), fy = (p[1] - sun.miny) / (sun.maxy - sun.miny);
+    uint32_t bx = (uint32_t)fmin(fmax(fx * kSunBucketSide, 0.0), (double)(kSunBucketSide - 1));
+    uint32_t by = (uint32_t)fmin(fmax(fy * kSunBucketSide, 0.0), (double)(kSunBucketSide - 1));
+    uint32_t key = 0;
+#pragma unroll
+    for (int b = 0; b < 8; b++) key |= (((bx >> b) & 1u) << (2 * b)) | (((by >> b) & 1u) << (2 * b + 1));
+    key_rank[2 * i] = key;
+    key_rank[2 * i + 1] = atomicAdd(&hist[key], 1u);
+}
+
+template <typename R>
+void launch_sun_buckets(const SceneView &sv, uint32_t seed, const uint64_t *sun_index, int n, uint32_t *key_rank,
+                        uint32_t *hist, hipStream_t s) {
+    launch_fill_u32(hist, kSunBuckets, 0u, s);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(sun_bucket_kernel<R>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sv.sun, seed,
+                       sun_index, n, key_rank, hist);
 }
 
 // ------------------------------------------------------ batched scenes ---
