@@ -71,6 +71,7 @@ struct BatchWork {
     DevBuf rec_slots, rec_count, rec_offs, rec_tmp, ovf_rec, ovf_meta, ovf_count;
     DevBuf out_slots, out_flag, out_offs, out_tmp, rays_round, sunpre;
     DevBuf err, fe_fail, counters;
+    DevBuf bucket_items, keyrank, hist, hoffs, htmp, htot, order;  // stage-0 ray order by sun-plane bucket
     std::vector<std::shared_ptr<DevBuf>> arenas;  // record buffers; reused once no context holds them
     uint32_t *h = nullptr;                        // pinned staging for small reads
     size_t h_words = 0;
@@ -296,6 +297,47 @@ static bool run_batch(std::vector<Scene> &sc, BatchWork &W, bool count, hipStrea
                     W.timing.sunshape_launches++;
                 }
             }
+            // stage-0 rays assigned to lanes by sun-plane bucket within each
+            // scene (keys: segment * 64^2 + bucket), when a scene's stage 0
+            // has a BVH for its reflected rays to walk
+            const uint32_t *order = nullptr;
+            if (si == 0) {
+                bool any = false;
+                for (const Seg &g : segs) {
+                    const Context *c = sc[g.k].c;
+                    any = any || (c->scene_view.use_grid && !c->scene_view.sun.point_source &&
+                                  c->cs.stages[0].nbvh > 0);
+                }
+                if (any) {
+                    constexpr uint32_t per = 1u << (2 * kBatchBucketLog2);
+                    std::vector<BucketItem> bi;
+                    std::vector<uint32_t> bp;
+                    for (size_t j = 0; j < segs.size(); j++) {
+                        const Seg &g = segs[j];
+                        const Scene &S = sc[g.k];
+                        BucketItem it{};
+                        it.sun = S.c->scene_view.sun;
+                        it.seed = S.seed;
+                        it.key0 = (uint32_t)j * per;
+                        it.sun_index = S.c->d_hits.as<uint64_t>() + g.first;
+                        bi.push_back(it);
+                        bp.push_back(g.base);
+                    }
+                    bp.push_back(tot);
+                    const uint32_t nkeys = (uint32_t)segs.size() * per;
+                    uint32_t *kr = (uint32_t *)W.keyrank.ensure((size_t)tot * 2 * sizeof(uint32_t));
+                    uint32_t *hist = (uint32_t *)W.hist.ensure((size_t)nkeys * sizeof(uint32_t));
+                    uint32_t *hoffs = (uint32_t *)W.hoffs.ensure((size_t)nkeys * sizeof(uint32_t));
+                    uint32_t *htmp = (uint32_t *)W.htmp.ensure(scan_tmp_words(nkeys) * sizeof(uint32_t));
+                    uint32_t *htot = (uint32_t *)W.htot.ensure(sizeof(uint32_t));
+                    uint32_t *ord = (uint32_t *)W.order.ensure((size_t)tot * sizeof(uint32_t));
+                    BatchQueue bq{put(W, W.prefix, bp, s), (int32_t)bi.size(), 1u, queue};
+                    launch_sun_buckets_batch<R>(put(W, W.bucket_items, bi, s), bq, tot, kr, hist, nkeys, s);
+                    launch_tile_scan(hist, hoffs, nkeys, htmp, htot, s);
+                    launch_order_scatter(kr, (int)tot, hoffs, htmp, ord, s);
+                    order = ord;
+                }
+            }
             // the trace launches, one per kernel variant present
             std::vector<TraceArgs> all(segs.size());
             for (size_t j = 0; j < segs.size(); j++) {
@@ -333,6 +375,7 @@ static bool run_batch(std::vector<Scene> &sc, BatchWork &W, bool count, hipStrea
                 a.flux_bins = nullptr;
                 a.counters = ctr ? ctr + (size_t)g.k * ctr_words + 5 * (1 + si) : nullptr;
                 a.generic = c->cs.stages[si].generic;
+                a.order = order;
                 all[j] = a;
             }
             launch_fill_u32(ovf_count, 1, 0u, s);

@@ -89,10 +89,11 @@ struct TraceArgs {
     // uniforms drawn, records
     unsigned long long *counters;
     int32_t generic;             // kernel variant: DevStage::generic of stage si
-    // lane -> ray: lane i of the launch traces ray order[i] (nullptr: ray i).
-    // Every output is indexed by the ray, so the order changes no result;
-    // stage-0 launches order their rays by sun-plane bucket, so a wave's rays
-    // leave neighbouring mirrors in similar directions and walk the BVH alike.
+    // lane -> ray, in the launch's ray space: the lane of place ray_base + i
+    // traces the ray at place order[ray_base + i] (nullptr: ray i).  Every
+    // output is indexed by the ray, so the order changes no result; stage-0
+    // launches order their rays by sun-plane bucket, so a wave's rays leave
+    // neighbouring mirrors in similar directions and walk the BVH alike.
     const uint32_t *order;
 };
 
@@ -218,6 +219,19 @@ constexpr uint32_t kSunBuckets = kSunBucketSide * kSunBucketSide;
 template <typename R>
 void launch_sun_buckets(const SceneView &sv, uint32_t seed, const uint64_t *sun_index, int n, uint32_t *key_rank,
                         uint32_t *hist, hipStream_t s);
+// The same over the segments of a batched launch: ray i of item k (q.prefix
+// in rays) gets key items[k].key0 + its bucket on a 2^side_log2-square grid;
+// hist (nkeys words) is zeroed here.
+struct BucketItem {
+    DevSun sun;
+    uint32_t seed;
+    uint32_t key0;
+    const uint64_t *sun_index;
+};
+constexpr uint32_t kBatchBucketLog2 = 6;  // 64 x 64 buckets per scene
+template <typename R>
+void launch_sun_buckets_batch(const BucketItem *items, const BatchQueue &q, uint32_t total, uint32_t *key_rank,
+                              uint32_t *hist, uint32_t nkeys, hipStream_t s);
 void launch_order_scatter(const uint32_t *key_rank, int n, const uint32_t *offsets, const uint32_t *tile_off,
                           uint32_t *order, hipStream_t s);
 // The getters' columns of a run's records in merged order: records

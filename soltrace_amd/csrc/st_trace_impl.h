@@ -841,7 +841,7 @@ template <typename R, int G, int WV>
 __global__ __launch_bounds__(kBlock, WV) void trace_stage_kernel(TraceArgs a) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n_rays) return;
-    const int t = a.order != nullptr ? (int)a.order[i] : i;
+    const int t = a.order != nullptr ? (int)(a.order[a.ray_base + (uint32_t)i] - a.ray_base) : i;
     trace_one<R, G>(a, t);
 }
 
@@ -849,21 +849,28 @@ __global__ __launch_bounds__(kBlock, WV) void trace_stage_kernel(TraceArgs a) {
 // sun-plane draws on a kSunBucketSide-square grid over the sun box, in Morton
 // order so neighbouring buckets stay close), counted into hist: key_rank
 // holds (bucket, rank within the bucket) for each ray.
+STD uint32_t sun_bucket(const DevSun &sun, uint32_t seed, uint64_t sun_index, uint32_t side_log2) {
+    Rng rng;
+    rng.begin(seed, 0, sun_index);
+    double p[2];
+    sun_plane_point(rng, sun, p);
+    const double side = (double)(1u << side_log2);
+    const double fx = (p[0] - sun.minx) / (sun.maxx - sun.minx), fy = (p[1] - sun.miny) / (sun.maxy - sun.miny);
+    const uint32_t bx = (uint32_t)fmin(fmax(fx * side, 0.0), side - 1.0);
+    const uint32_t by = (uint32_t)fmin(fmax(fy * side, 0.0), side - 1.0);
+    uint32_t key = 0;
+    for (uint32_t b = 0; b < side_log2; b++)
+        key |= (((bx >> b) & 1u) << (2 * b)) | (((by >> b) & 1u) << (2 * b + 1));
+    return key;
+}
+
 template <typename R>
 __global__ __launch_bounds__(kBlock) void sun_bucket_kernel(DevSun sun, uint32_t seed, const uint64_t *sun_index,
                                                             int n, uint32_t *key_rank, uint32_t *hist) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    Rng rng;
-    rng.begin(seed, 0, sun_index[i]);
-    double p[2];
-    sun_plane_point(rng, sun, p);
-    const double fx = (p[0] - sun.minx) / (sun.maxx - sun.minx), fy = (p[1] - sun.miny) / (sun.maxy - sun.miny);
-    uint32_t bx = (uint32_t)fmin(fmax(fx * kSunBucketSide, 0.0), (double)(kSunBucketSide - 1));
-    uint32_t by = (uint32_t)fmin(fmax(fy * kSunBucketSide, 0.0), (double)(kSunBucketSide - 1));
-    uint32_t key = 0;
-#pragma unroll
-    for (int b = 0; b < 8; b++) key |= (((bx >> b) & 1u) << (2 * b)) | (((by >> b) & 1u) << (2 * b + 1));
+    constexpr uint32_t kLog2 = __builtin_ctz(kSunBucketSide);
+    const uint32_t key = sun_bucket(sun, seed, sun_index[i], kLog2);
     key_rank[2 * i] = key;
     key_rank[2 * i + 1] = atomicAdd(&hist[key], 1u);
 }
@@ -887,6 +894,28 @@ STD int batch_item(const BatchQueue &q, uint32_t x) {
         else hi = mid - 1;
     }
     return lo;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void sun_bucket_batch_kernel(const BucketItem *items, BatchQueue q,
+                                                                  uint32_t total, uint32_t *key_rank,
+                                                                  uint32_t *hist) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    const int k = batch_item(q, i);
+    const BucketItem &it = items[k];
+    const uint32_t key = it.key0 + sun_bucket(it.sun, it.seed, it.sun_index[i - q.prefix[k]], kBatchBucketLog2);
+    key_rank[2 * i] = key;
+    key_rank[2 * i + 1] = atomicAdd(&hist[key], 1u);
+}
+
+template <typename R>
+void launch_sun_buckets_batch(const BucketItem *items, const BatchQueue &q, uint32_t total, uint32_t *key_rank,
+                              uint32_t *hist, uint32_t nkeys, hipStream_t s) {
+    launch_fill_u32(hist, nkeys, 0u, s);
+    if (total == 0) return;
+    hipLaunchKernelGGL(sun_bucket_batch_kernel<R>, dim3((total + kBlock - 1) / kBlock), dim3(kBlock), 0, s, items,
+                       q, total, key_rank, hist);
 }
 // The wave's next tile (wave-uniform).
 STD uint32_t wave_take(uint32_t *counter) {
@@ -944,7 +973,8 @@ __global__ __launch_bounds__(kBlock, WV) void trace_batch_kernel(const TraceArgs
         const uint32_t first = (tile - q.prefix[k]) * q.tile;
         for (uint32_t r = lane; r < q.tile; r += 64) {
             const uint32_t t = first + r;
-            if (t < (uint32_t)a.n_rays) trace_one<R, G>(a, (int)t);
+            if (t < (uint32_t)a.n_rays)
+                trace_one<R, G>(a, a.order != nullptr ? (int)(a.order[a.ray_base + t] - a.ray_base) : (int)t);
         }
     }
 }
