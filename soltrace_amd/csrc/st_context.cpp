@@ -426,7 +426,12 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         }
         timing.trace_launches[si]++;
         launch_fill_u32(ovf_count, 1, 0u, s);
-        timed(timing.trace_ms[si], [&]() { launch_trace_stage<R>(a, s); });
+        if (!h_args) HC(hipHostMalloc((void **)&h_args, kArgsRing * sizeof(TraceArgs), hipHostMallocDefault));
+        TraceArgs *dev_a = (TraceArgs *)d_args.ensure(kArgsRing * sizeof(TraceArgs)) + args_next;
+        h_args[args_next] = a;
+        HC(hipMemcpyAsync(dev_a, &h_args[args_next], sizeof(TraceArgs), hipMemcpyHostToDevice, s));
+        args_next = (args_next + 1) % kArgsRing;
+        timed(timing.trace_ms[si], [&]() { launch_trace_stage<R>(a, dev_a, s); });
         timing.rays_in[si] += (uint64_t)n;
         launch_tile_scan(a.rec_count, d_rec_offs.as<uint32_t>(), (size_t)n, d_scan_tmp.as<uint32_t>(), total, s);
         if (next_rays)

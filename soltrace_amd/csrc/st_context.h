@@ -130,6 +130,14 @@ struct Context {
         d_ovf_meta, d_ovf_count, d_out_slots, d_out_flag, d_out_offs, d_flux;
     DevBuf d_rays_a, d_rays_b, d_sunpre, d_sunpre_batch[2];
     DevBuf d_keyrank, d_hist, d_hoffs, d_htmp, d_htot, d_order[2];  // stage-0 ray order by sun-plane bucket
+    // trace-launch arguments in device memory: a ring of slots, each filled
+    // from its pinned host twin by a copy queued before its launch (a run
+    // finishes each launch before the next, so a slot is free again long
+    // before the ring comes round)
+    static constexpr int kArgsRing = 8;
+    TraceArgs *h_args = nullptr;
+    DevBuf d_args;
+    int args_next = 0;
 
     // results
     std::vector<StageRecords> recs;
@@ -183,6 +191,7 @@ struct Context {
         if (ev_tot) (void)hipEventDestroy(ev_tot);
         if (h_tot) (void)hipHostFree(h_tot);
         if (h_stage) (void)hipHostFree(h_stage);
+        if (h_args) (void)hipHostFree(h_args);
         for (hipEvent_t e : ev_stage)
             if (e) (void)hipEventDestroy(e);
         if (mask_stream) (void)hipStreamDestroy(mask_stream);

@@ -154,7 +154,10 @@ template <typename R> void launch_trace_batch(const TraceArgs *items, const Batc
 // kernel variant (generic), best first by the single-scene measurements.
 int trace_batch_waves(int bits, int generic, int *out, int max);
 template <typename R> void launch_sunshape_d(const SunPreArgs &a, hipStream_t s);
-template <typename R> void launch_trace_stage(const TraceArgs &a, hipStream_t s);
+// a: the launch's arguments on the host (they pick the kernel and grid);
+// dev_a: the same bytes in device memory, which the kernel reads (held by
+// pointer, its many uniform fields need not stay in scalar registers).
+template <typename R> void launch_trace_stage(const TraceArgs &a, const TraceArgs *dev_a, hipStream_t s);
 
 // Tile scan of n uint32 values: out[i] is the exclusive prefix sum of in[i]
 // within its tile of kScanTileElems values, tile_off[k] the exclusive prefix

@@ -4,7 +4,7 @@
 namespace st {
 template void launch_sun_mask<float>(const MaskArgs &, hipStream_t);
 template void launch_sunshape_d<float>(const SunPreArgs &, hipStream_t);
-template void launch_trace_stage<float>(const TraceArgs &, hipStream_t);
+template void launch_trace_stage<float>(const TraceArgs &, const TraceArgs *, hipStream_t);
 template void launch_sun_buckets_batch<float>(const BucketItem *, const BatchQueue &, uint32_t, uint32_t *, uint32_t *,
                                                 uint32_t, hipStream_t);
 template void launch_sun_buckets<float>(const SceneView &, uint32_t, const uint64_t *, int, uint32_t *, uint32_t *,

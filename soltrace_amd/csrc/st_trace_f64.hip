@@ -5,7 +5,7 @@
 namespace st {
 template void launch_sun_mask<double>(const MaskArgs &, hipStream_t);
 template void launch_sunshape_d<double>(const SunPreArgs &, hipStream_t);
-template void launch_trace_stage<double>(const TraceArgs &, hipStream_t);
+template void launch_trace_stage<double>(const TraceArgs &, const TraceArgs *, hipStream_t);
 template void launch_sun_buckets_batch<double>(const BucketItem *, const BatchQueue &, uint32_t, uint32_t *, uint32_t *,
                                                 uint32_t, hipStream_t);
 template void launch_sun_buckets<double>(const SceneView &, uint32_t, const uint64_t *, int, uint32_t *, uint32_t *,

@@ -838,7 +838,8 @@ __device__ __forceinline__ void trace_one(const TraceArgs &a, const int t) {
 // stages, whose rays spend less time waiting on the BVH; the basic FP64
 // variant 2 (3 is slower).
 template <typename R, int G, int WV>
-__global__ __launch_bounds__(kBlock, WV) void trace_stage_kernel(TraceArgs a) {
+__global__ __launch_bounds__(kBlock, WV) void trace_stage_kernel(const TraceArgs *__restrict__ ap) {
+    const TraceArgs &a = *ap;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n_rays) return;
     const int t = a.order != nullptr ? (int)(a.order[a.ray_base + (uint32_t)i] - a.ray_base) : i;
@@ -1027,14 +1028,14 @@ void launch_sun_mask(const MaskArgs &a, hipStream_t s) {
     else hipLaunchKernelGGL((sun_mask_kernel<R, 0>), dim3(blocks), dim3(kBlock), 0, s, a);
 }
 template <typename R>
-void launch_trace_stage(const TraceArgs &a, hipStream_t s) {
+void launch_trace_stage(const TraceArgs &a, const TraceArgs *dev_a, hipStream_t s) {
     const int blocks = (a.n_rays + kBlock - 1) / kBlock;
-    if (a.generic == 2) hipLaunchKernelGGL((trace_stage_kernel<R, 2, 1>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else if (a.generic) hipLaunchKernelGGL((trace_stage_kernel<R, 1, 2>), dim3(blocks), dim3(kBlock), 0, s, a);
+    if (a.generic == 2) hipLaunchKernelGGL((trace_stage_kernel<R, 2, 1>), dim3(blocks), dim3(kBlock), 0, s, dev_a);
+    else if (a.generic) hipLaunchKernelGGL((trace_stage_kernel<R, 1, 2>), dim3(blocks), dim3(kBlock), 0, s, dev_a);
     else if constexpr (sizeof(R) == 8)
-        hipLaunchKernelGGL((trace_stage_kernel<R, 0, 2>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else if (a.si == 0) hipLaunchKernelGGL((trace_stage_kernel<R, 0, 4>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL((trace_stage_kernel<R, 0, 3>), dim3(blocks), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((trace_stage_kernel<R, 0, 2>), dim3(blocks), dim3(kBlock), 0, s, dev_a);
+    else if (a.si == 0) hipLaunchKernelGGL((trace_stage_kernel<R, 0, 4>), dim3(blocks), dim3(kBlock), 0, s, dev_a);
+    else hipLaunchKernelGGL((trace_stage_kernel<R, 0, 3>), dim3(blocks), dim3(kBlock), 0, s, dev_a);
 }
 
 }  // namespace st
