@@ -24,6 +24,21 @@
 namespace st {
 
 static constexpr int kBlock = 256;
+
+// Event counters (a diagnostic mode): the sum of v over the wave's active
+// lanes, added by one lane, so a launch makes one atomic per wave and
+// counter rather than one per lane on the same address.
+STD void wave_add(unsigned long long *p, unsigned long long v) {
+    const uint64_t act = __ballot(true);
+    unsigned long long sum = 0;
+    for (uint64_t m = act; m != 0; m &= m - 1) {
+        const int l = __ffsll((unsigned long long)m) - 1;
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+        sum += ((unsigned long long)hi << 32) | lo;
+    }
+    if ((int)(threadIdx.x & 63) == __ffsll((unsigned long long)act) - 1) atomicAdd(p, sum);
+}
 static constexpr int32_t kNoRecord = INT32_MIN;  // a record lost to a full overflow buffer
 
 // ------------------------------------------------------ sun-ray source ----
@@ -258,9 +273,9 @@ __device__ __forceinline__ void mask_lane(const MaskArgs &a, const uint32_t t) {
                             &fefail);
         if (G == 2 && fefail) atomicMin(a.fe_fail, (unsigned long long)(a.s0 + t));
         if (a.counters) {
-            atomicAdd(&a.counters[0], 1ull);
-            atomicAdd(&a.counters[1], (unsigned long long)tests);
-            atomicAdd(&a.counters[3], (unsigned long long)rng.k);
+            wave_add(&a.counters[0], 1ull);
+            wave_add(&a.counters[1], (unsigned long long)tests);
+            wave_add(&a.counters[3], (unsigned long long)rng.k);
         }
     }
     const uint64_t word = __ballot(hit);
@@ -437,8 +452,8 @@ __device__ __forceinline__ void mask_grid_wave(const MaskArgs &a, const uint64_t
         if (w * 64 < a.n) a.mask[w] = (uint64_t)hitw[wv][lane * 2] | ((uint64_t)hitw[wv][lane * 2 + 1] << 32);
     }
     if (a.counters) {
-        atomicAdd(&a.counters[0], (unsigned long long)rays);
-        atomicAdd(&a.counters[3], (unsigned long long)draws);
+        wave_add(&a.counters[0], (unsigned long long)rays);
+        wave_add(&a.counters[3], (unsigned long long)draws);
         if (lane == 0) atomicAdd(&a.counters[1], (unsigned long long)tests);
     }
 }
@@ -540,8 +555,8 @@ __global__ __launch_bounds__(kBlock) void sunshape_d_kernel(SunPreArgs a) {
         U &= ~__ballot(resolved);
     }
     if (a.counters) {
-        atomicAdd(&a.counters[0], (unsigned long long)ev_eval);
-        atomicAdd(&a.counters[1], (unsigned long long)ev_need);
+        wave_add(&a.counters[0], (unsigned long long)ev_eval);
+        wave_add(&a.counters[1], (unsigned long long)ev_need);
     }
 }
 
@@ -795,11 +810,11 @@ __device__ __forceinline__ void trace_one(const TraceArgs &a, const int t) {
     a.rec_count[g] = nrec;
     if (a.out_flag != nullptr) a.out_flag[g] = passed ? 1u : 0u;
     if (a.counters) {
-        atomicAdd(&a.counters[0], 1ull);
-        atomicAdd(&a.counters[1], (unsigned long long)ev_lds[0][threadIdx.x]);
-        atomicAdd(&a.counters[2], (unsigned long long)ev_lds[1][threadIdx.x]);
-        atomicAdd(&a.counters[3], (unsigned long long)(rng.k - ev_lds[2][threadIdx.x]));
-        atomicAdd(&a.counters[4], (unsigned long long)nrec);
+        wave_add(&a.counters[0], 1ull);
+        wave_add(&a.counters[1], (unsigned long long)ev_lds[0][threadIdx.x]);
+        wave_add(&a.counters[2], (unsigned long long)ev_lds[1][threadIdx.x]);
+        wave_add(&a.counters[3], (unsigned long long)(rng.k - ev_lds[2][threadIdx.x]));
+        wave_add(&a.counters[4], (unsigned long long)nrec);
     }
     if (passed && a.out_slots != nullptr) {
         DevRay r;
