@@ -1192,6 +1192,12 @@ bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
     out.foot.assign(2 * (size_t)std::max(1, s0.nelem), DevFoot{});
     out.grid_always.clear();
     for (DevFoot &f : out.foot) f.kind = 2;
+    // every sun ray travels along the sun-frame z axis
+    double sun_dir[3];
+    {
+        const double ez[3] = {0.0, 0.0, 1.0};
+        mtv(Rs, ez, sun_dir);
+    }
     for (int j = 0; j < s0.nelem; j++) {
         const DevElem &d = out.elems[s0.elem0 + j];
         if (!d.enabled) continue;
@@ -1203,6 +1209,23 @@ bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
         }
         for (int b = 0; b < nb; b++) {
             const double *bx = &out.ebox[12 * (size_t)(s0.elem0 + j) + 6 * b];
+            if (b == 1 && d.surf == SURF_SPHERE && d.vcx != 0.0) {
+                // The far box of a sphere holds only exit points: a hit
+                // there is the larger root t1 (the smaller one is taken only
+                // at z <= zap, on the near branch), where the ray leaves the
+                // sphere, so D . grad F = 2 (Kx x Dx + Ky y Dy + (z - 1/c) Dz)
+                // >= 0.  A sun direction for which that is strongly negative
+                // over the whole box never ends a sun ray there.
+                double dl[3];
+                mv(d.R, sun_dir, dl);
+                const double zc = 1.0 / d.vcx, ky = d.stype == 7 ? 0.0 : 1.0;
+                double g = -HUGE_VAL;
+                for (int q = 0; q < 8; q++) {
+                    const double x = (q & 1) ? bx[3] : bx[0], y = (q & 2) ? bx[4] : bx[1], z = (q & 4) ? bx[5] : bx[2];
+                    g = std::max(g, x * dl[0] + ky * y * dl[1] + (z - zc) * dl[2]);
+                }
+                if (g < -0.01 * fabs(zc)) continue;
+            }
             std::vector<std::pair<double, double>> pts;
             double x0 = HUGE_VAL, x1 = -HUGE_VAL, y0 = HUGE_VAL, y1 = -HUGE_VAL;
             for (int q = 0; q < 8; q++) {
