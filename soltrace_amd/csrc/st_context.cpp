@@ -198,6 +198,10 @@ bool Context::prepare() {
         v.grid_always = upload(d_grid_always, cs.grid_always, s);
         v.n_always = (int32_t)cs.grid_always.size();
         v.foot = upload(d_foot, cs.foot, s);
+        if (!cs.beam.empty()) {
+            v.beam = upload(d_beam, cs.beam, s);
+            v.beam_items = upload(d_beam_items, cs.beam_items, s);
+        }
         v.grid = cs.grid;
         v.use_grid = cs.use_grid ? 1 : 0;
         v.nstages = (int)cs.stages.size();

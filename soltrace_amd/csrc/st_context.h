@@ -123,7 +123,7 @@ struct Context {
     bool dirty = true;            // a setter ran since the last compile
     bool compiled_cull = true;
     SceneView scene_view{};
-    DevBuf d_elems, d_faces, d_aux, d_stages, d_bvh, d_bvh_items, d_always_items, d_grid_cells, d_grid_items, d_grid_always, d_foot;
+    DevBuf d_elems, d_faces, d_aux, d_stages, d_bvh, d_bvh_items, d_always_items, d_grid_cells, d_grid_items, d_grid_always, d_foot, d_beam, d_beam_items;
     // run scratch
     DevBuf d_mask_scan_tmp, d_mask_total;
     DevBuf d_mask, d_popc, d_offs, d_scan_tmp, d_scan_tmp2, d_total, d_hits, d_rec_slots, d_rec_count, d_rec_offs, d_ovf_rec,

@@ -22,6 +22,8 @@ struct SceneView {
     const int32_t *grid_always;  // stage-1 elements every sun ray is tested against
     int32_t n_always;
     const DevFoot *foot;         // per stage-1 element
+    const DevBeam *beam;         // per stage-1 element, or nullptr
+    const DevBeamItem *beam_items;
     DevSunGrid grid;
     int32_t use_grid;
     int32_t nstages;

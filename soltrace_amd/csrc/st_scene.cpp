@@ -959,6 +959,13 @@ bool compile_scene(SceneModel &m, bool use_cull, CompiledScene &out, std::string
                 }
                 it.idx = (int32_t)j;
                 items.push_back(it);
+                if (i == 0) {
+                    DevBeamItem bi{};
+                    for (int k = 0; k < 3; k++) { bi.lo[k] = it.lo[k]; bi.hi[k] = it.hi[k]; }
+                    bi.j = (int32_t)j;
+                    bi.box = q;
+                    out.s0_boxes.push_back(bi);
+                }
             }
         }
         // the BVH holds the bounded elements; the others (torus, finite
@@ -1031,6 +1038,114 @@ static bool foot_hull(std::vector<std::pair<double, double>> pts, double pad, De
 
 // Sun frame, sun-plane box and the stage-1 sun-plane grid
 // (coretrace/suntoprimarystage.cpp:59-293).
+// Beam lists of the stage-1 elements (DevBeam).  Element i's list holds
+// every stage-1 hit box that a ray starting in the sphere S about i's near
+// box, in a direction within angle alpha of the axis a, can reach: for a
+// box inside the sphere B (centre v from S's centre, radius rho together
+// with S's radius) the directions from S to B lie within asin(rho / |v|) of
+// v, so the box is kept when angle(v, a) <= alpha + asin(rho / |v|) or
+// |v| <= rho.  The host test uses a slightly larger sphere and angle than
+// the device's check, so rounding there cannot admit a ray the list does
+// not cover.  Lists go nearest box first (an early hit prunes the rest).
+static void compile_beams(CompiledScene &out, const double sun_dir[3]) {
+    out.beam.clear();
+    out.beam_items.clear();
+    if (!out.use_grid || out.stages.empty()) return;
+    const DevStage &s0 = out.stages[0];
+    if (s0.nbvh == 0 || s0.nelem == 0) return;
+    const DevSun &sun = out.sun;
+    double th_sun;
+    switch (sun.shape) {
+    case 'p': case 'P': th_sun = sun.sigma / 1000.0; break;
+    case 'g': case 'G': th_sun = 5.0 * sun.sigma / 1000.0; break;
+    case 'd': case 'D': th_sun = sun.max_angle / 1000.0; break;
+    default: return;
+    }
+    // error angle bound of a face distribution: the pillbox's radius, five
+    // standard deviations of the Gaussian (a draw beyond it, about 4e-6 of
+    // them, takes the BVH walk); a diffuse face has no bound
+    auto err_bound = [](int dist, double mrad, double *th) {
+        if (mrad == 0.0) { *th = 0.0; return true; }
+        if (dist == 'p' || dist == 'P') { *th = mrad / 1000.0; return true; }
+        if (dist == 'g' || dist == 'G') { *th = 5.0 * mrad / 1000.0; return true; }
+        return false;
+    };
+    std::vector<int> near(s0.nelem, -1);
+    for (size_t q = 0; q < out.s0_boxes.size(); q++)
+        if (out.s0_boxes[q].box == 0) near[out.s0_boxes[q].j] = (int)q;
+    auto centre = [](const DevBeamItem &b, double c[3]) {
+        double r2 = 0.0;
+        for (int k = 0; k < 3; k++) {
+            c[k] = 0.5 * ((double)b.lo[k] + (double)b.hi[k]);
+            const double h = 0.5 * ((double)b.hi[k] - (double)b.lo[k]);
+            r2 += h * h;
+        }
+        return sqrt(r2);
+    };
+    out.beam.assign(s0.nelem, DevBeam{});
+    int lists = 0;
+    for (int i = 0; i < s0.nelem; i++) {
+        DevBeam &bm = out.beam[i];
+        bm.count = -1;
+        const DevElem &d = out.elems[s0.elem0 + i];
+        if (!d.enabled || near[i] < 0 || d.interaction != 2 || d.face_front < 0) continue;
+        if (!(d.surf == SURF_FLAT || (d.surf == SURF_SPHERE && d.stype != 7))) continue;
+        const DevFace &f = out.faces[d.face_front];
+        double th_slope, th_spec;
+        if (!err_bound(f.dist, f.slope, &th_slope) || !err_bound(f.dist, f.spec, &th_spec)) continue;
+        // normal tilt over the aperture: asin(r c) for the sphere
+        double th_curv = 0.0;
+        if (d.surf == SURF_SPHERE) {
+            const double *bx = &out.ebox[12 * (size_t)(s0.elem0 + i)];
+            const double rx = std::max(fabs(bx[0]), fabs(bx[3])), ry = std::max(fabs(bx[1]), fabs(bx[4]));
+            const double rc = sqrt(rx * rx + ry * ry) * fabs(d.vcx);
+            if (!(rc < 1.0)) continue;
+            th_curv = asin(rc);
+        }
+        const double alpha = th_sun + 2.0 * (th_curv + th_slope) + th_spec + 2e-3;
+        if (!(alpha < 0.5)) continue;
+        // axis: the sun direction reflected at the vertex normal (the
+        // element's z axis in stage frame)
+        const double n[3] = {d.R[6], d.R[7], d.R[8]};
+        const double dn = sun_dir[0] * n[0] + sun_dir[1] * n[1] + sun_dir[2] * n[2];
+        double a[3], an = 0.0;
+        for (int k = 0; k < 3; k++) { a[k] = sun_dir[k] - 2.0 * dn * n[k]; an += a[k] * a[k]; }
+        an = sqrt(an);
+        for (int k = 0; k < 3; k++) a[k] /= an;
+        double c[3];
+        const double r = centre(out.s0_boxes[near[i]], c);
+        const double r_host = r * (1.0 + 1e-6) + 1e-3, alpha_host = alpha + 1e-4;
+        std::vector<std::pair<double, int>> keep;
+        for (size_t q = 0; q < out.s0_boxes.size(); q++) {
+            double cq[3];
+            const double rho = centre(out.s0_boxes[q], cq) + r_host;
+            double v[3], dist = 0.0, va = 0.0;
+            for (int k = 0; k < 3; k++) { v[k] = cq[k] - c[k]; dist += v[k] * v[k]; va += v[k] * a[k]; }
+            dist = sqrt(dist);
+            bool in = dist <= rho;
+            if (!in) {
+                const double ang = acos(std::max(-1.0, std::min(1.0, va / dist)));
+                in = ang <= alpha_host + asin(std::min(1.0, rho / dist));
+            }
+            if (in) keep.emplace_back(dist, (int)q);
+        }
+        // a long list walks no faster than the BVH
+        if (keep.size() > 32) continue;
+        std::sort(keep.begin(), keep.end());
+        for (int k = 0; k < 3; k++) { bm.c[k] = c[k]; bm.a[k] = (float)a[k]; }
+        bm.r2 = (float)(r * r);
+        bm.cosa = (float)cos(alpha);
+        bm.first = (int32_t)out.beam_items.size();
+        bm.count = (int32_t)keep.size();
+        for (auto &kq : keep) out.beam_items.push_back(out.s0_boxes[kq.second]);
+        lists++;
+    }
+    if (lists == 0) {
+        out.beam.clear();
+        out.beam_items.clear();
+    }
+}
+
 bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
     HStage &st = m.stages[0];
     HSun &sun = m.sun;
@@ -1292,6 +1407,7 @@ bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
     each_cell([&](size_t c, int j) { out.grid_items[fill[c]++] = j; });
     out.grid_cells.swap(cnt);
     out.use_grid = !sun.point_source;
+    compile_beams(out, sun_dir);
     return true;
 }
 

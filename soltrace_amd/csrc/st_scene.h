@@ -91,6 +91,10 @@ struct CompiledScene {
     std::vector<DevFoot> foot;  // 2 per stage-1 element, one per hit box
     std::vector<double> ebox;   // per element: 2 element-frame hit boxes lo[3], hi[3]
     std::vector<int8_t> enbox;  // per element: boxes that bound its hits (0: none)
+    // stage-1 hit boxes in stage frame, as the BVH holds them (j = element)
+    std::vector<DevBeamItem> s0_boxes;
+    std::vector<DevBeam> beam;            // per stage-1 element (empty: no lists)
+    std::vector<DevBeamItem> beam_items;
     DevSunGrid grid{};
     bool use_grid = false;
     DevSun sun{};

@@ -157,19 +157,33 @@ STD bool foot_contains(const DevFoot *foot, int j, double sx, double sy) {
     return g.kind == 0 ? in : g.kind == 1;
 }
 
+// Whether a ray leaving stage-1 element `prev` at ps along cs is covered by
+// that element's beam list (DevBeam).
+STD bool in_beam(const SceneView &sv, int prev, const double ps[3], const double cs[3]) {
+    if (prev < 0 || sv.beam == nullptr) return false;
+    const DevBeam &bm = sv.beam[prev];
+    if (bm.count < 0) return false;
+    const float dx = (float)(ps[0] - bm.c[0]), dy = (float)(ps[1] - bm.c[1]), dz = (float)(ps[2] - bm.c[2]);
+    const float dot = (float)cs[0] * bm.a[0] + (float)cs[1] * bm.a[1] + (float)cs[2] * bm.a[2];
+    return dx * dx + dy * dy + dz * dz <= bm.r2 && dot >= bm.cosa;
+}
+
 // Candidates: sun-plane grid cell `cell` for a first hit of a sun ray (cell
 // >= 0), screened by the elements' footprints at (sx, sy) when `screen`; the
-// stage BVH when present; every element otherwise.  One loop feeds them all
-// to a single consider() so the element code is instantiated once.
+// beam list of stage-1 element `prev`, the element the ray leaves, when the
+// ray is within its beam; the stage BVH when present; every element
+// otherwise.  One loop feeds them all to a single consider() so the element
+// code is instantiated once.
 template <typename R, int G>
 STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3], const double cs[3], int cell,
-                     bool screen, double sx, double sy, Best &b) {
+                     bool screen, double sx, double sy, Best &b, int prev = -1) {
     b.path = 1e99;
     b.j = 0x7fffffff;
     b.fefail = 0;
     const int32_t *items = nullptr;  // nullptr: the item is the element index
+    const DevBeamItem *boxes = nullptr;  // beam list entries (a box test each), else nullptr
     int k = 0, kend = 0;
-    bool bvh = false, always = false;
+    bool bvh = false, always = false, beam = false;
     if (cell >= 0) {
         // the cell's candidates, then the elements no footprint bounds
         items = sv.grid_items;
@@ -177,8 +191,10 @@ STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3]
         kend = sv.grid_cells[cell + 1];
         always = sv.n_always > 0;
     } else if (st.nbvh > 0) {
-        // the stage's unbounded elements (torus, finite element), then the walk
-        bvh = true;
+        // the stage's unbounded elements (torus, finite element), then the
+        // beam list or the walk
+        beam = in_beam(sv, prev, ps, cs);
+        bvh = !beam;
         items = sv.always_items;
         k = st.always0;
         kend = st.always0 + st.nalways;
@@ -198,10 +214,35 @@ STD void closest_hit(const SceneView &sv, const DevStage &st, const double ps[3]
     int i = 0;
     for (;;) {
         if (k < kend) {
-            const int j = items != nullptr ? items[k] : k;
-            ++k;
-            if (screen && !foot_contains(sv.foot, j, sx, sy)) continue;
+            int j;
+            if (boxes != nullptr) {
+                const DevBeamItem it = boxes[k];
+                ++k;
+                b.nodes++;
+                float t0 = 0.0f, t1 = 3.0e38f;
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const float ta = (it.lo[q] - o[q]) * inv[q];
+                    const float tb = (it.hi[q] - o[q]) * inv[q];
+                    t0 = fmaxf(t0, fminf(ta, tb));
+                    t1 = fminf(t1, fmaxf(ta, tb));
+                }
+                if (!(t0 <= t1) || (double)t0 > b.path * 1.000001 + 1.0e-3) continue;
+                j = it.j;
+            } else {
+                j = items != nullptr ? items[k] : k;
+                ++k;
+                if (screen && !foot_contains(sv.foot, j, sx, sy)) continue;
+            }
             consider<R, G>(sv, st, j, ps, cs, b);
+            continue;
+        }
+        if (beam) {
+            beam = false;
+            const DevBeam &bm = sv.beam[prev];
+            boxes = sv.beam_items;
+            k = bm.first;
+            kend = bm.first + bm.count;
             continue;
         }
         if (always) {
@@ -704,13 +745,15 @@ __device__ __forceinline__ void trace_one(const TraceArgs &a, const int t) {
     const bool flux_here = a.flux.active && a.flux.stage == a.si;
     int fbin = -1;
     bool fabs_ = false;
+    int prev = -1;  // the element the ray leaves
 
     for (;;) {
         Best b;
         b.tests = 0;
         b.nodes = 0;
-        // rays here are known to hit, so footprints would screen out little
-        closest_hit<R, G>(sv, st, ps, cs, mhc == 0 ? cell : -1, false, 0.0, 0.0, b);
+        // rays here are known to hit, so footprints would screen out little;
+        // a ray leaving a stage-1 element may take its beam list
+        closest_hit<R, G>(sv, st, ps, cs, mhc == 0 ? cell : -1, false, 0.0, 0.0, b, sun_stage ? prev : -1);
         if (G == 2 && b.fefail) atomicOr(a.err_flags, 16u);
         if (a.counters) {
             ev_lds[0][threadIdx.x] += b.tests;
@@ -730,6 +773,7 @@ __device__ __forceinline__ void trace_one(const TraceArgs &a, const int t) {
             break;
         }
         const DevElem &e = sv.elems[st.elem0 + b.j];
+        prev = b.j;
         double pe[3], ce[3], xd[3], df[3];
         int back;
         element_ray(e, ps, cs, pe, ce);

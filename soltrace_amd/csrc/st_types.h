@@ -122,6 +122,31 @@ struct DevBvhNode {
     int32_t leaf;           // 0: inner node; else (first item << 3) | count (1..7)
 };
 
+// Beam list of a stage-1 element: the elements a ray leaving it can reach.
+// A ray that starts within radius sqrt(r2) of c and whose direction lies
+// within the cone (axis a, cos of the half-angle cosa) can only hit items
+// of the list; any other ray walks the BVH.  The axis is the mirror
+// reflection of the sun direction at the element's vertex; the half-angle
+// covers the sunshape, the surface's curvature and its slope and
+// specularity errors.
+struct DevBeam {
+    double c[3];            // stage frame
+    float r2;
+    float a[3];
+    float cosa;
+    int32_t first, count;   // slice of the beam item table (count < 0: no list)
+    int32_t pad;
+};
+
+// A beam list entry: one hit box of element j (stage frame, float, padded
+// outward like the BVH's boxes).
+struct DevBeamItem {
+    float lo[3];
+    int32_t j;
+    float hi[3];
+    int32_t box;            // which of the element's boxes (0 near, 1 far)
+};
+
 // Output record in stage frame (coretrace/types.h:242-249 fields).
 struct DevRecord {
     double pos[3];

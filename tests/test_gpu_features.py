@@ -59,14 +59,29 @@ def test_surface_files(hip_lib, oracle_lib, bits):
             assert_records_match(r_g, r_o, pos_tol=TOL[bits], cos_tol=TOL[bits])
 
 
-def test_cull_equals_full_scan(hip_lib):
-    sc = workloads.tower(n_helio=300, raycount=20000, r_max=700.0)
-    a = run_scene(hip_lib, sc, 4, setup=_setup(64, 1))
-    b = run_scene(hip_lib, sc, 4, setup=_setup(64, 0))
+# Culled runs (sun-plane grid, BVH, and the beam lists of rays leaving a
+# heliostat) against full scans.  The dense fields put rings 11 m apart for
+# 10 m heliostats, so blocking rays are common: a beam list that missed a
+# blocker would change records.  Wide slope errors widen the beam cones
+# (longer lists, and elements whose list would be too long walk the BVH).
+CULL_SCENES = {
+    "field": dict(n_helio=300, raycount=20000, r_max=700.0),
+    "dense": dict(n_helio=200, raycount=40000, r_max=200.0, sun_elev=30.0),
+    "dense_wide": dict(n_helio=200, raycount=40000, r_max=200.0, sun_elev=60.0, slope=12.0, sun="pillbox"),
+}
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+@pytest.mark.parametrize("name", sorted(CULL_SCENES))
+def test_cull_equals_full_scan(hip_lib, name, bits):
+    sc = workloads.tower(**CULL_SCENES[name])
+    a = run_scene(hip_lib, sc, 4, setup=_setup(bits, 1))
+    b = run_scene(hip_lib, sc, 4, setup=_setup(bits, 0))
     assert a[0] == b[0] > 0
     for f in ("element", "stage", "raynum"):
         np.testing.assert_array_equal(getattr(a[1], f), getattr(b[1], f))
     np.testing.assert_array_equal(a[1].pos, b[1].pos)
+    np.testing.assert_array_equal(a[1].cos, b[1].cos)
 
 
 def test_vshot_surface_fp64(hip_lib, oracle_lib):

@@ -33,3 +33,13 @@ def test_tower_fp64_matches_oracle(hip_lib, oracle_lib):
     n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 5, setup=_fp(64))
     assert n_o > 0 and n_g == n_o, (n_g, n_o, m_g)
     assert_records_match(r_g, r_o, pos_tol=1e-6, cos_tol=1e-6)
+
+
+def test_dense_tower_fp64_matches_oracle(hip_lib, oracle_lib):
+    # rings 11 m apart for 10 m heliostats at a 30 degree sun: about one ray
+    # in twenty is blocked by a neighbour after its first reflection
+    sc = workloads.tower(n_helio=200, raycount=20000, r_max=200.0, sun_elev=30.0)
+    n_o, r_o, m_o, _ = run_scene(oracle_lib, sc, 4, setup=counter_mode)
+    n_g, r_g, m_g, _ = run_scene(hip_lib, sc, 4, setup=_fp(64))
+    assert n_o > 0 and n_g == n_o, (n_g, n_o, m_g)
+    assert_records_match(r_g, r_o, pos_tol=1e-6, cos_tol=1e-6)
