@@ -892,10 +892,9 @@ __device__ __forceinline__ void trace_one(const TraceArgs &a, const int t) {
 // Waves per SIMD (WV): the generic variant runs 2 (its registers spill,
 // but a second wave hides more latency than the spills cost; the one with
 // finite elements keeps the whole register file); the basic FP32 variant
-// runs 4 for the sun stage (128 registers: its spills cost less than the
-// latency the fourth wave hides; 5 waves spill too much) and 3 for later
-// stages, whose rays spend less time waiting on the BVH; the basic FP64
-// variant 2 (3 is slower).
+// runs 3 (168 registers, no spills: with beam lists the rays of the sun
+// stage wait little on the BVH, and 4 waves in 128 registers spill more
+// than the fourth wave hides); the basic FP64 variant 2 (3 is slower).
 template <typename R, int G, int WV>
 __global__ __launch_bounds__(kBlock, WV) void trace_stage_kernel(const TraceArgs *__restrict__ ap) {
     const TraceArgs &a = *ap;
@@ -1093,7 +1092,6 @@ void launch_trace_stage(const TraceArgs &a, const TraceArgs *dev_a, hipStream_t 
     else if (a.generic) hipLaunchKernelGGL((trace_stage_kernel<R, 1, 2>), dim3(blocks), dim3(kBlock), 0, s, dev_a);
     else if constexpr (sizeof(R) == 8)
         hipLaunchKernelGGL((trace_stage_kernel<R, 0, 2>), dim3(blocks), dim3(kBlock), 0, s, dev_a);
-    else if (a.si == 0) hipLaunchKernelGGL((trace_stage_kernel<R, 0, 4>), dim3(blocks), dim3(kBlock), 0, s, dev_a);
     else hipLaunchKernelGGL((trace_stage_kernel<R, 0, 3>), dim3(blocks), dim3(kBlock), 0, s, dev_a);
 }
 
