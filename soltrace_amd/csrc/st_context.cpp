@@ -10,6 +10,7 @@
 // (coretrace/types.cpp:311-355).
 #include "st_context.h"
 
+#include <deque>
 #include <thread>
 
 namespace st {
@@ -242,30 +243,38 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     hipStream_t s = stream;
     const SceneView sv = scene_view;
 
-    // total[0..1]: scan totals; total[3]: error flags, read with the totals;
-    // total[4..5]: the least sun-ray index whose finite-element
-    // interpolation failed in a hit-mask launch (all ones: none)
-    uint32_t *total = (uint32_t *)d_total.ensure(6 * sizeof(uint32_t));
-    uint32_t *err = total + 3;
-    unsigned long long *fe_fail = (unsigned long long *)(total + 4);
+    // total[4p .. 4p+3]: launch-buffer set p's scan totals (records, rays)
+    // and its launches' error flags, read together; total[8..9]: the least
+    // sun-ray index whose finite-element interpolation failed in a hit-mask
+    // launch (all ones: none); total[10]: error flags of the sunshape
+    // sampler, read once stage 1 is done
+    uint32_t *total = (uint32_t *)d_total.ensure(12 * sizeof(uint32_t));
+    uint32_t *err = total + 10;
+    unsigned long long *fe_fail = (unsigned long long *)(total + 8);
+    launch_fill_u32(total, 8, 0u, s);
     launch_fill_u32(err, 1, 0u, s);
-    launch_fill_u32(total + 4, 2, 0xFFFFFFFFu, s);
-    uint32_t *ovf_count = (uint32_t *)d_ovf_count.ensure(sizeof(uint32_t));
+    launch_fill_u32(total + 8, 2, 0xFFFFFFFFu, s);
+    uint32_t *ovf_counts = (uint32_t *)d_ovf_count.ensure(2 * sizeof(uint32_t));
     // Per-launch scratch sized for this run: no launch takes more than N
     // rays (stage-1 hits or rays entering a later stage), so a context of a
     // small scene holds little device memory, however many contexts exist.
+    // The second set is sized when a second launch first needs it.
     const uint32_t launch_cap = (uint32_t)std::min<uint64_t>(kMaxChunk, std::max<uint64_t>(N, 1));
     const uint32_t ovf_cap = std::max<uint32_t>(launch_cap / 2, std::min<uint32_t>(kMaxChunk / 2, 65536u));
-    d_ovf_rec.ensure((size_t)ovf_cap * sizeof(DevRecord));
-    d_ovf_meta.ensure((size_t)ovf_cap * 2 * sizeof(uint32_t));
-    d_rec_slots.ensure((size_t)launch_cap * kSlots * sizeof(DevRecord));
-    d_rec_count.ensure((size_t)launch_cap * sizeof(uint32_t));
-    d_rec_offs.ensure((size_t)launch_cap * sizeof(uint32_t));
-    d_out_slots.ensure((size_t)launch_cap * sizeof(DevRay));
-    d_out_flag.ensure((size_t)launch_cap * sizeof(uint32_t));
-    d_out_offs.ensure((size_t)launch_cap * sizeof(uint32_t));
-    d_scan_tmp.ensure(scan_tmp_words((size_t)1 << 26) * sizeof(uint32_t));
-    d_scan_tmp2.ensure(scan_tmp_words((size_t)1 << 26) * sizeof(uint32_t));
+    auto size_set = [&](int p) {
+        d_ovf_rec[p].ensure((size_t)ovf_cap * sizeof(DevRecord));
+        d_ovf_meta[p].ensure((size_t)ovf_cap * 2 * sizeof(uint32_t));
+        d_rec_slots[p].ensure((size_t)launch_cap * kSlots * sizeof(DevRecord));
+        d_rec_count[p].ensure((size_t)launch_cap * sizeof(uint32_t));
+        d_rec_offs[p].ensure((size_t)launch_cap * sizeof(uint32_t));
+        d_out_slots[p].ensure((size_t)launch_cap * sizeof(DevRay));
+        d_out_flag[p].ensure((size_t)launch_cap * sizeof(uint32_t));
+        d_out_offs[p].ensure((size_t)launch_cap * sizeof(uint32_t));
+        d_scan_tmp[p].ensure(scan_tmp_words((size_t)1 << 26) * sizeof(uint32_t));
+        d_scan_tmp2[p].ensure(scan_tmp_words((size_t)1 << 26) * sizeof(uint32_t));
+    };
+    size_set(0);
+    bool set_sized[2] = {true, false};
 
     unsigned long long *ctrs = nullptr;
     if (count_events) {
@@ -380,18 +389,32 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
     // in two halves so the host can queue other work between them: the
     // launch half ends with an asynchronous copy of the totals into pinned
     // memory; the finish half waits for it, checks the error flags, takes a
-    // record buffer and compacts.  Passed rays are appended to next_rays at
-    // *n_next.
+    // record buffer and compacts on the compaction stream.  Passed rays are
+    // appended to next_rays at *n_next.  Launch k works in buffer set k mod
+    // 2 and first waits for the compaction of launch k - 2 (same set), so
+    // with two launches in flight a launch traces while the one before it is
+    // compacted.
     struct Pending {
         TraceArgs a;
-        int si, n;
+        int si, n, set;
         DevRay *next_rays;
         uint64_t *n_next;
         std::chrono::steady_clock::time_point t0;
     };
+    hipStream_t cs_ = comp_stream;
+    uint64_t launch_seq = 0;
+    bool comp_queued[2] = {false, false};
     auto trace_launch = [&](int si, int n, const uint64_t *sun_idx, uint32_t raynum_base, const DevRay *in,
                             uint32_t last_raynum, DevRay *next_rays, uint64_t *n_next,
                             const SunPre *pre, const uint32_t *order = nullptr) -> Pending {
+        const int set = (int)(launch_seq++ & 1);
+        if (!set_sized[set]) {
+            size_set(set);
+            set_sized[set] = true;
+        }
+        if (comp_queued[set]) HC(hipStreamWaitEvent(s, ev_comp[set], 0));
+        uint32_t *tot_set = total + 4 * set;
+        uint32_t *ovf_count = ovf_counts + set;
         TraceArgs a{};
         a.sv = sv;
         a.si = si;
@@ -406,15 +429,15 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         a.raynum_base = raynum_base;
         a.in_rays = in;
         a.last_raynum = last_raynum;
-        a.rec_slots = d_rec_slots.as<DevRecord>();
-        a.rec_count = d_rec_count.as<uint32_t>();
-        a.ovf_rec = d_ovf_rec.as<DevRecord>();
-        a.ovf_meta = d_ovf_meta.as<uint32_t>();
+        a.rec_slots = d_rec_slots[set].as<DevRecord>();
+        a.rec_count = d_rec_count[set].as<uint32_t>();
+        a.ovf_rec = d_ovf_rec[set].as<DevRecord>();
+        a.ovf_meta = d_ovf_meta[set].as<uint32_t>();
         a.ovf_count = ovf_count;
         a.ovf_cap = ovf_cap;
-        a.out_slots = next_rays ? d_out_slots.as<DevRay>() : nullptr;
-        a.out_flag = next_rays ? d_out_flag.as<uint32_t>() : nullptr;
-        a.err_flags = err;
+        a.out_slots = next_rays ? d_out_slots[set].as<DevRay>() : nullptr;
+        a.out_flag = next_rays ? d_out_flag[set].as<uint32_t>() : nullptr;
+        a.err_flags = tot_set + 3;
         a.flux = fs;
         a.flux_bins = fs.active ? d_flux.as<uint32_t>() : nullptr;
         a.flux_src = by_src ? d_flux_src.as<uint32_t>() : nullptr;
@@ -437,30 +460,36 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         args_next = (args_next + 1) % kArgsRing;
         timed(timing.trace_ms[si], [&]() { launch_trace_stage<R>(a, dev_a, s); });
         timing.rays_in[si] += (uint64_t)n;
-        launch_tile_scan(a.rec_count, d_rec_offs.as<uint32_t>(), (size_t)n, d_scan_tmp.as<uint32_t>(), total, s);
+        launch_tile_scan(a.rec_count, d_rec_offs[set].as<uint32_t>(), (size_t)n, d_scan_tmp[set].as<uint32_t>(),
+                         tot_set, s);
         if (next_rays)
-            launch_tile_scan(a.out_flag, d_out_offs.as<uint32_t>(), (size_t)n, d_scan_tmp2.as<uint32_t>(), total + 1,
-                             s);
+            launch_tile_scan(a.out_flag, d_out_offs[set].as<uint32_t>(), (size_t)n, d_scan_tmp2[set].as<uint32_t>(),
+                             tot_set + 1, s);
         else
-            launch_fill_u32(total + 1, 1, 0u, s);
-        HC(hipMemcpyAsync(h_tot, total, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HC(hipEventRecord(ev_tot, s));
-        return Pending{a, si, n, next_rays, n_next, std::chrono::steady_clock::now()};
+            launch_fill_u32(tot_set + 1, 1, 0u, s);
+        HC(hipMemcpyAsync(h_tot + 4 * set, tot_set, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HC(hipEventRecord(ev_tot[set], s));
+        return Pending{a, si, n, set, next_rays, n_next, std::chrono::steady_clock::now()};
     };
     auto trace_finish = [&](Pending &p) -> bool {
-        HC(hipEventSynchronize(ev_tot));
-        uint32_t tot[4] = {h_tot[0], h_tot[1], h_tot[2], h_tot[3]};
+        const int set = p.set;
+        HC(hipEventSynchronize(ev_tot[set]));
+        const uint32_t *ht = h_tot + 4 * set;
+        uint32_t tot[4] = {ht[0], ht[1], ht[2], ht[3]};
         if (!check_err(tot[3])) return false;
         const TraceArgs &a = p.a;
         const int n = p.n, si = p.si;
         std::shared_ptr<DevBuf> chunk = take_buffer((size_t)std::max<uint32_t>(1, tot[0]) * sizeof(DevRecord));
-        launch_compact_records(a.rec_slots, a.rec_count, d_rec_offs.as<uint32_t>(), d_scan_tmp.as<uint32_t>(), n,
-                               kSlots, chunk->as<DevRecord>(), s);
-        launch_compact_overflow(a.ovf_rec, a.ovf_meta, ovf_count, ovf_cap, d_rec_offs.as<uint32_t>(),
-                                d_scan_tmp.as<uint32_t>(), chunk->as<DevRecord>(), s);
+        HC(hipStreamWaitEvent(cs_, ev_tot[set], 0));
+        launch_compact_records(a.rec_slots, a.rec_count, d_rec_offs[set].as<uint32_t>(), d_scan_tmp[set].as<uint32_t>(),
+                               n, kSlots, chunk->as<DevRecord>(), cs_);
+        launch_compact_overflow(a.ovf_rec, a.ovf_meta, a.ovf_count, ovf_cap, d_rec_offs[set].as<uint32_t>(),
+                                d_scan_tmp[set].as<uint32_t>(), chunk->as<DevRecord>(), cs_);
         if (tot[1] && p.next_rays)
-            launch_compact_rays(a.out_slots, a.out_flag, d_out_offs.as<uint32_t>(), d_scan_tmp2.as<uint32_t>(), n,
-                                p.next_rays + *p.n_next, s);
+            launch_compact_rays(a.out_slots, a.out_flag, d_out_offs[set].as<uint32_t>(),
+                                d_scan_tmp2[set].as<uint32_t>(), n, p.next_rays + *p.n_next, cs_);
+        HC(hipEventRecord(ev_comp[set], cs_));
+        comp_queued[set] = true;
         recs[si].add(chunk, 0, tot[0], true);
         timing.records[si] += tot[0];
         if (p.n_next) *p.n_next += tot[1];
@@ -468,11 +497,26 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         traced_total += (uint64_t)n;
         return true;
     };
-    auto trace_chunk = [&](int si, int n, const uint64_t *sun_idx, uint32_t raynum_base, const DevRay *in,
-                           uint32_t last_raynum, DevRay *next_rays, uint64_t *n_next,
-                           const SunPre *pre = nullptr) -> bool {
-        Pending p = trace_launch(si, n, sun_idx, raynum_base, in, last_raynum, next_rays, n_next, pre);
-        return trace_finish(p);
+    // Launches whose totals the host has not read yet: with a progress
+    // callback each launch is finished before the next (the callback sees
+    // every launch's rays done), else two stay in flight.
+    std::deque<Pending> inflight;
+    const size_t depth = cb ? 1 : 2;
+    auto finish_to = [&](size_t keep) -> bool {
+        while (inflight.size() > keep) {
+            Pending p = inflight.front();
+            inflight.pop_front();
+            if (!trace_finish(p)) {
+                inflight.clear();
+                return false;
+            }
+        }
+        return true;
+    };
+    // every queued compaction done before stream s goes on
+    auto join_comp = [&]() {
+        HC(hipEventRecord(ev_join, cs_));
+        HC(hipStreamWaitEvent(s, ev_join, 0));
     };
 
     // ------------------------------------------------------ stage 0 -------
@@ -537,10 +581,12 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             SunPre *pre;
             const uint32_t *order;  // lane -> ray of the batch (nullptr: identity)
             hipEvent_t ev;
+            int par;             // parity of its sunshape-angle and order buffers
             bool more;           // another batch was launched behind this one
             uint64_t sun_count;  // sun rays drawn up to this batch's last hit (or its end)
         };
         bool maxed = false;
+        bool traced_rec[2] = {false, false};
         // Read the pending mask batch's hit count, compact its hit list (and
         // sample its sunshape angles), and launch the next mask batch if hits
         // are still missing.
@@ -551,6 +597,9 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             uint32_t h = 0;
             HC(hipMemcpyAsync(&h, mtotal, sizeof h, hipMemcpyDeviceToHost, sm));
             HC(hipStreamSynchronize(sm));
+            // the batch two back, which used this parity's buffers, is traced
+            if (traced_rec[parity]) HC(hipStreamWaitEvent(sm, ev_traced[parity], 0));
+            r.par = parity;
             r.take = (uint32_t)std::min<uint64_t>(h, want);
             r.hit_list = sun_index_all + hits;
             r.raynum_base = hits;
@@ -615,18 +664,21 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             for (uint32_t c = 0; c < cur_b.take && ok; c += launch_cap) {
                 const int n = (int)std::min<uint32_t>(launch_cap, cur_b.take - c);
                 // a single stage hands no rays on
-                Pending p = trace_launch(0, n, cur_b.hit_list + c, (uint32_t)(cur_b.raynum_base + c), nullptr, 0,
-                                         nst > 1 ? next : nullptr, &n_next,
-                                         cur_b.pre != nullptr ? cur_b.pre + c : nullptr,
-                                         c == 0 && (uint32_t)n == cur_b.take ? cur_b.order : nullptr);
+                inflight.push_back(trace_launch(0, n, cur_b.hit_list + c, (uint32_t)(cur_b.raynum_base + c), nullptr,
+                                                0, nst > 1 ? next : nullptr, &n_next,
+                                                cur_b.pre != nullptr ? cur_b.pre + c : nullptr,
+                                                c == 0 && (uint32_t)n == cur_b.take ? cur_b.order : nullptr));
                 if (c == 0 && cur_b.more) {
                     next_b = prepare();  // overlaps the trace just queued
                     have_next = true;
                 }
-                ok = trace_finish(p);
+                ok = finish_to(depth - 1);
             }
+            HC(hipEventRecord(ev_traced[cur_b.par], s));
+            traced_rec[cur_b.par] = true;
             if (!ok) {
                 HC(hipStreamSynchronize(sm));
+                HC(hipStreamSynchronize(cs_));
                 return -1;
             }
             n_hits_done = (uint64_t)cur_b.raynum_base + cur_b.take;
@@ -642,6 +694,12 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             if (!have_next) break;
             cur_b = next_b;
         }
+        if (!finish_to(0)) {
+            HC(hipStreamSynchronize(sm));
+            HC(hipStreamSynchronize(cs_));
+            return -1;
+        }
+        join_comp();
         HC(hipStreamSynchronize(sm));
         if (cs.stages[0].generic == 2) {
             // hit-mask launches run ahead of the trace: a failed interpolation
@@ -757,13 +815,22 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         uint64_t n_out = 0;
         for (uint64_t c = 0; c < n_cur; c += launch_cap) {
             int n = (int)std::min<uint64_t>(launch_cap, n_cur - c);
-            if (!trace_chunk((int)si, n, nullptr, 0, cur + c, last_prev, nxt, &n_out)) return -1;
+            inflight.push_back(trace_launch((int)si, n, nullptr, 0, cur + c, last_prev, nxt, &n_out, nullptr));
+            if (!finish_to(depth - 1)) {
+                HC(hipStreamSynchronize(cs_));
+                return -1;
+            }
             if (cb && !cb((st_uint_t)traced_total, (st_uint_t)(c + n), (st_uint_t)last_prev, (st_uint_t)si + 1,
                           (st_uint_t)nst, cbdata)) {
                 cancelled = true;
                 break;
             }
         }
+        if (!finish_to(0)) {
+            HC(hipStreamSynchronize(cs_));
+            return -1;
+        }
+        join_comp();
         if (si == 1 && save_data && !cancelled) {
             // the reference's stage-2 ray array as stage 2 leaves it
             // (coretrace/raytrace.cpp:1065-1078): the array is reused in
@@ -787,7 +854,13 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         prev_has = n_cur > 0;
         if (prev_has) last_prev = last_raynum_of(cur, n_cur);
     }
-    HC(hipStreamSynchronize(s));
+    {
+        // the sunshape sampler's error flags
+        uint32_t e = 0;
+        HC(hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, s));
+        HC(hipStreamSynchronize(s));
+        if (!check_err(e)) return -1;
+    }
     for (const auto &m : ev_marks) {
         float ms = 0;
         HC(hipEventElapsedTime(&ms, ev_pool[m.second], ev_pool[m.second + 1]));

@@ -115,8 +115,15 @@ struct Context {
     std::vector<std::string> messages;
     hipStream_t stream = nullptr;
     hipStream_t mask_stream = nullptr;  // next batch's sun mask, beside the current batch's trace
-    hipEvent_t ev_hits[2] = {nullptr, nullptr}, ev_ready = nullptr, ev_tot = nullptr;
-    uint32_t *h_tot = nullptr;  // pinned: per-launch totals and error flags
+    // record and ray compaction of a trace launch, beside the next launch
+    hipStream_t comp_stream = nullptr;
+    hipEvent_t ev_hits[2] = {nullptr, nullptr}, ev_ready = nullptr, ev_join = nullptr;
+    // per launch-buffer set: its totals reached the host; its compaction is done
+    hipEvent_t ev_tot[2] = {nullptr, nullptr}, ev_comp[2] = {nullptr, nullptr};
+    // per sun-ray batch parity: the batch's trace launches are done (its
+    // sunshape angles and ray order may be overwritten)
+    hipEvent_t ev_traced[2] = {nullptr, nullptr};
+    uint32_t *h_tot = nullptr;  // pinned: per-launch totals and error flags (4 words per set)
     int device = 0;
 
     CompiledScene cs;
@@ -126,14 +133,17 @@ struct Context {
     DevBuf d_elems, d_faces, d_aux, d_stages, d_bvh, d_bvh_items, d_always_items, d_grid_cells, d_grid_items, d_grid_always, d_foot, d_beam, d_beam_items;
     // run scratch
     DevBuf d_mask_scan_tmp, d_mask_total;
-    DevBuf d_mask, d_popc, d_offs, d_scan_tmp, d_scan_tmp2, d_total, d_hits, d_rec_slots, d_rec_count, d_rec_offs, d_ovf_rec,
-        d_ovf_meta, d_ovf_count, d_out_slots, d_out_flag, d_out_offs, d_flux;
+    DevBuf d_mask, d_popc, d_offs, d_total, d_hits, d_ovf_count, d_flux;
+    // per-launch buffers, two sets: launch k uses set k mod 2, so a launch
+    // runs while the one before it is compacted
+    DevBuf d_scan_tmp[2], d_scan_tmp2[2], d_rec_slots[2], d_rec_count[2], d_rec_offs[2], d_ovf_rec[2], d_ovf_meta[2],
+        d_out_slots[2], d_out_flag[2], d_out_offs[2];
     DevBuf d_rays_a, d_rays_b, d_sunpre, d_sunpre_batch[2];
     DevBuf d_keyrank, d_hist, d_hoffs, d_htmp, d_htot, d_order[2];  // stage-0 ray order by sun-plane bucket
     // trace-launch arguments in device memory: a ring of slots, each filled
-    // from its pinned host twin by a copy queued before its launch (a run
-    // finishes each launch before the next, so a slot is free again long
-    // before the ring comes round)
+    // from its pinned host twin by a copy queued before its launch (at most
+    // two launches are queued ahead of the host, so a slot is free again
+    // long before the ring comes round)
     static constexpr int kArgsRing = 8;
     TraceArgs *h_args = nullptr;
     DevBuf d_args;
@@ -178,9 +188,13 @@ struct Context {
         scratch_ptr = scratch_on(device);
         HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HC(hipStreamCreateWithFlags(&mask_stream, hipStreamNonBlocking));
+        HC(hipStreamCreateWithFlags(&comp_stream, hipStreamNonBlocking));
         for (auto &e : ev_hits) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (auto &e : ev_tot) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (auto &e : ev_comp) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (auto &e : ev_traced) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         HC(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
-        HC(hipEventCreateWithFlags(&ev_tot, hipEventDisableTiming));
+        HC(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
         HC(hipHostMalloc((void **)&h_tot, 16 * sizeof(uint32_t), hipHostMallocDefault));
     }
     ~Context() {
@@ -188,13 +202,20 @@ struct Context {
         for (auto e : ev_hits)
             if (e) (void)hipEventDestroy(e);
         if (ev_ready) (void)hipEventDestroy(ev_ready);
-        if (ev_tot) (void)hipEventDestroy(ev_tot);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        for (auto e : ev_tot)
+            if (e) (void)hipEventDestroy(e);
+        for (auto e : ev_comp)
+            if (e) (void)hipEventDestroy(e);
+        for (auto e : ev_traced)
+            if (e) (void)hipEventDestroy(e);
         if (h_tot) (void)hipHostFree(h_tot);
         if (h_stage) (void)hipHostFree(h_stage);
         if (h_args) (void)hipHostFree(h_args);
         for (hipEvent_t e : ev_stage)
             if (e) (void)hipEventDestroy(e);
         if (mask_stream) (void)hipStreamDestroy(mask_stream);
+        if (comp_stream) (void)hipStreamDestroy(comp_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void errlog(const char *fmt, ...) {
