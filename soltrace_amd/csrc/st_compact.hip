@@ -290,6 +290,66 @@ void launch_order_scatter(const uint32_t *key_rank, int n, const uint32_t *offse
                        order);
 }
 
+// Ray order (launch_sun_order): column b of table (nblocks rows of
+// kOrderBuckets counts) -> each row's exclusive prefix within the column,
+// and colsum[b] = the column's total.  Four lanes share a column, each
+// scanning a quarter of the rows; a shuffle joins the quarters.
+__global__ __launch_bounds__(256) void order_colscan_kernel(uint32_t *table, uint32_t nblocks, uint32_t *colsum) {
+    const uint32_t b = blockIdx.x * 64 + (threadIdx.x >> 2), q = threadIdx.x & 3;
+    const uint32_t seg = (nblocks + 3) / 4, r0 = min(nblocks, q * seg), r1 = min(nblocks, r0 + seg);
+    uint32_t sum = 0;
+    if (b < kOrderBuckets)
+        for (uint32_t r = r0; r < r1; r++) sum += table[(size_t)r * kOrderBuckets + b];
+    uint32_t incl = sum;
+    uint32_t v = (uint32_t)__shfl_up((int)incl, 1, 4);
+    if (q >= 1) incl += v;
+    v = (uint32_t)__shfl_up((int)incl, 2, 4);
+    if (q >= 2) incl += v;
+    const uint32_t total = (uint32_t)__shfl((int)incl, 3, 4);
+    uint32_t run = incl - sum;
+    if (b < kOrderBuckets) {
+        for (uint32_t r = r0; r < r1; r++) {
+            uint32_t *p = table + (size_t)r * kOrderBuckets + b;
+            const uint32_t c = *p;
+            *p = run;
+            run += c;
+        }
+        if (q == 3) colsum[b] = total;
+    }
+}
+void launch_order_colscan(uint32_t *table, uint32_t nblocks, uint32_t *colsum, hipStream_t s) {
+    hipLaunchKernelGGL(order_colscan_kernel, dim3(kOrderBuckets / 64), dim3(256), 0, s, table, nblocks, colsum);
+}
+
+// Ray order, last pass: ray i of block i / kOrderRays goes to its bucket's
+// start, plus its block's start within the bucket, plus its rank.  A block
+// of the pass covers one block of the count pass and reads that block's
+// row of starts into LDS first.
+__global__ __launch_bounds__(1024) void order_place_kernel(const uint32_t *key_rank, int n, const uint32_t *table,
+                                                           const uint32_t *cbase, const uint32_t *ctile,
+                                                           uint32_t *order) {
+    __shared__ uint32_t start[kOrderBuckets];
+    const uint32_t *row = table + (size_t)blockIdx.x * kOrderBuckets;
+    for (uint32_t b = threadIdx.x; b < kOrderBuckets; b += 1024)
+        start[b] = row[b] + cbase[b] + ctile[b / kScanTile];
+    __syncthreads();
+    const int base = (int)blockIdx.x * (int)kOrderRays;
+    for (int k = 0; k < (int)(kOrderRays / 1024); k++) {
+        const int i = base + k * 1024 + (int)threadIdx.x;
+        if (i < n) {
+            const uint32_t kr = key_rank[i];
+            const uint32_t key = kr & (kOrderBuckets - 1u), rank = kr >> (2 * kOrderLog2);
+            order[start[key] + rank] = (uint32_t)i;
+        }
+    }
+}
+void launch_order_place(const uint32_t *key_rank, int n, const uint32_t *table, const uint32_t *cbase,
+                        const uint32_t *ctile, uint32_t *order, hipStream_t s) {
+    if (n <= 0) return;
+    const uint32_t nblocks = (uint32_t)((n + kOrderRays - 1) / kOrderRays);
+    hipLaunchKernelGGL(order_place_kernel, dim3(nblocks), dim3(1024), 0, s, key_rank, n, table, cbase, ctile, order);
+}
+
 // ------------------------------------------------ merged record columns ---
 // The getters' view of a run's records (coretrace/stapi.cpp:564-659 over
 // TRayData::Merge, types.cpp:311-355): output record i comes from the piece

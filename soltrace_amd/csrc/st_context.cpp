@@ -612,15 +612,15 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             // the batch's rays in sun-plane bucket order, for a single launch
             r.order = nullptr;
             if (sort_rays && r.take && r.take <= launch_cap) {
-                uint32_t *kr = (uint32_t *)d_keyrank.ensure((size_t)r.take * 2 * sizeof(uint32_t));
-                uint32_t *hist = (uint32_t *)d_hist.ensure(kSunBuckets * sizeof(uint32_t));
-                uint32_t *hoffs = (uint32_t *)d_hoffs.ensure(kSunBuckets * sizeof(uint32_t));
-                uint32_t *htmp = (uint32_t *)d_htmp.ensure(scan_tmp_words(kSunBuckets) * sizeof(uint32_t));
-                uint32_t *htot = (uint32_t *)d_htot.ensure(sizeof(uint32_t));
+                OrderWork w;
+                w.key_rank = (uint32_t *)d_keyrank.ensure((size_t)r.take * sizeof(uint32_t));
+                w.table = (uint32_t *)d_hist.ensure(order_table_words((int)r.take) * sizeof(uint32_t));
+                w.colsum = (uint32_t *)d_hoffs.ensure(2 * kOrderBuckets * sizeof(uint32_t));
+                w.cbase = w.colsum + kOrderBuckets;
+                w.ctile = (uint32_t *)d_htmp.ensure(scan_tmp_words(kOrderBuckets) * sizeof(uint32_t));
+                w.ctot = (uint32_t *)d_htot.ensure(sizeof(uint32_t));
                 uint32_t *ord = (uint32_t *)d_order[parity].ensure((size_t)r.take * sizeof(uint32_t));
-                launch_sun_buckets<R>(sv, seed, r.hit_list, (int)r.take, kr, hist, sm);
-                launch_tile_scan(hist, hoffs, kSunBuckets, htmp, htot, sm);
-                launch_order_scatter(kr, (int)r.take, hoffs, htmp, ord, sm);
+                launch_sun_order<R>(sv, seed, r.hit_list, (int)r.take, w, ord, sm);
                 r.order = ord;
             }
             if (r.take == want) {

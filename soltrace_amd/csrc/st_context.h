@@ -139,7 +139,9 @@ struct Context {
     DevBuf d_scan_tmp[2], d_scan_tmp2[2], d_rec_slots[2], d_rec_count[2], d_rec_offs[2], d_ovf_rec[2], d_ovf_meta[2],
         d_out_slots[2], d_out_flag[2], d_out_offs[2];
     DevBuf d_rays_a, d_rays_b, d_sunpre, d_sunpre_batch[2];
-    DevBuf d_keyrank, d_hist, d_hoffs, d_htmp, d_htot, d_order[2];  // stage-0 ray order by sun-plane bucket
+    // stage-0 ray order by sun-plane bucket (OrderWork: key_rank, table,
+    // column totals and starts, tile starts, total; the order of each parity)
+    DevBuf d_keyrank, d_hist, d_hoffs, d_htmp, d_htot, d_order[2];
     // trace-launch arguments in device memory: a ring of slots, each filled
     // from its pinned host twin by a copy queued before its launch (at most
     // two launches are queued ahead of the host, so a slot is free again

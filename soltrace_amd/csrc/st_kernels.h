@@ -213,20 +213,35 @@ void launch_stats_bin(const DevRecord *rec, uint32_t n, int32_t elem, const Flux
                       uint32_t *bins, unsigned long long *acc, hipStream_t s);
 void launch_compact_rays(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, const uint32_t *tile_off,
                          int n_rays, DevRay *out, hipStream_t s);
-// Ray order of a stage-0 launch by sun-plane bucket (a counting sort):
-// the sun-ray bucket of each of the n rays (launch_sun_buckets, per
-// precision file) counted into hist (kSunBuckets words, zeroed here) with its
-// rank, then order[start of the bucket + rank] = ray.  Ranks come from
-// atomics, so the order within a bucket varies between runs; the results
-// do not.
-constexpr uint32_t kSunBucketSide = 256;
-constexpr uint32_t kSunBuckets = kSunBucketSide * kSunBucketSide;
+// Ray order of a stage-0 launch by sun-plane bucket, a counting sort with
+// no global atomics (launch_sun_order, per precision file).  Blocks of
+// kOrderRays rays rank their rays' buckets (a 2^kOrderLog2-square Morton
+// grid over the sun box) with LDS atomics and store the block's bucket
+// counts as a row of `table`; a column scan turns each column into the
+// blocks' starts within the bucket and the bucket's total, a tile scan of
+// the totals gives the buckets' starts, and order[bucket start + block
+// start + rank] = ray.  The order within a bucket varies between runs; the
+// results do not.
+constexpr uint32_t kOrderLog2 = 7;
+constexpr uint32_t kOrderBuckets = 1u << (2 * kOrderLog2);  // 16384 (a 64 KiB LDS histogram)
+constexpr uint32_t kOrderRays = 8192;                        // rays per block (ranks fit 13 bits)
+struct OrderWork {
+    uint32_t *key_rank;  // n words: key | rank << 2 kOrderLog2
+    uint32_t *table;     // order_table_words(n)
+    uint32_t *colsum, *cbase, *ctile, *ctot;  // kOrderBuckets, kOrderBuckets, scan_tmp_words(kOrderBuckets), 1
+};
+inline size_t order_table_words(int n) { return (size_t)((n + kOrderRays - 1) / kOrderRays) * kOrderBuckets; }
 template <typename R>
-void launch_sun_buckets(const SceneView &sv, uint32_t seed, const uint64_t *sun_index, int n, uint32_t *key_rank,
-                        uint32_t *hist, hipStream_t s);
-// The same over the segments of a batched launch: ray i of item k (q.prefix
-// in rays) gets key items[k].key0 + its bucket on a 2^side_log2-square grid;
-// hist (nkeys words) is zeroed here.
+void launch_sun_order(const SceneView &sv, uint32_t seed, const uint64_t *sun_index, int n, const OrderWork &w,
+                      uint32_t *order, hipStream_t s);
+// the generic passes of launch_sun_order (st_compact.hip)
+void launch_order_colscan(uint32_t *table, uint32_t nblocks, uint32_t *colsum, hipStream_t s);
+void launch_order_place(const uint32_t *key_rank, int n, const uint32_t *table, const uint32_t *cbase,
+                        const uint32_t *ctile, uint32_t *order, hipStream_t s);
+// Bucket order over the segments of a batched launch: ray i of item k
+// (q.prefix in rays) gets key items[k].key0 + its bucket on a
+// 2^side_log2-square grid, counted into hist (nkeys words, zeroed here) with
+// its rank from a global atomic; then launch_order_scatter.
 struct BucketItem {
     DevSun sun;
     uint32_t seed;

@@ -923,24 +923,40 @@ STD uint32_t sun_bucket(const DevSun &sun, uint32_t seed, uint64_t sun_index, ui
     return key;
 }
 
+// Ray order, first pass: block b ranks rays [b kOrderRays, (b + 1)
+// kOrderRays) within their buckets in LDS and writes its bucket counts as
+// row b of table.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void sun_bucket_kernel(DevSun sun, uint32_t seed, const uint64_t *sun_index,
-                                                            int n, uint32_t *key_rank, uint32_t *hist) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    constexpr uint32_t kLog2 = __builtin_ctz(kSunBucketSide);
-    const uint32_t key = sun_bucket(sun, seed, sun_index[i], kLog2);
-    key_rank[2 * i] = key;
-    key_rank[2 * i + 1] = atomicAdd(&hist[key], 1u);
+__global__ __launch_bounds__(1024) void sun_order_count_kernel(DevSun sun, uint32_t seed, const uint64_t *sun_index,
+                                                               int n, uint32_t *key_rank, uint32_t *table) {
+    __shared__ uint32_t hist[kOrderBuckets];
+    for (uint32_t b = threadIdx.x; b < kOrderBuckets; b += 1024) hist[b] = 0u;
+    __syncthreads();
+    const int base = (int)blockIdx.x * (int)kOrderRays;
+    for (int k = 0; k < (int)(kOrderRays / 1024); k++) {
+        const int i = base + k * 1024 + (int)threadIdx.x;
+        if (i < n) {
+            const uint32_t key = sun_bucket(sun, seed, sun_index[i], kOrderLog2);
+            const uint32_t rank = atomicAdd(&hist[key], 1u);
+            key_rank[i] = key | (rank << (2 * kOrderLog2));
+        }
+    }
+    __syncthreads();
+    uint32_t *row = table + (size_t)blockIdx.x * kOrderBuckets;
+    for (uint32_t b = threadIdx.x; b < kOrderBuckets; b += 1024) row[b] = hist[b];
 }
 
 template <typename R>
-void launch_sun_buckets(const SceneView &sv, uint32_t seed, const uint64_t *sun_index, int n, uint32_t *key_rank,
-                        uint32_t *hist, hipStream_t s) {
-    launch_fill_u32(hist, kSunBuckets, 0u, s);
+void launch_sun_order(const SceneView &sv, uint32_t seed, const uint64_t *sun_index, int n, const OrderWork &w,
+                      uint32_t *order, hipStream_t s) {
+    static_assert(kOrderRays <= (1u << (32 - 2 * kOrderLog2)), "rank bits");
     if (n <= 0) return;
-    hipLaunchKernelGGL(sun_bucket_kernel<R>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sv.sun, seed,
-                       sun_index, n, key_rank, hist);
+    const uint32_t nblocks = (uint32_t)((n + kOrderRays - 1) / kOrderRays);
+    hipLaunchKernelGGL(sun_order_count_kernel<R>, dim3(nblocks), dim3(1024), 0, s, sv.sun, seed, sun_index, n,
+                       w.key_rank, w.table);
+    launch_order_colscan(w.table, nblocks, w.colsum, s);
+    launch_tile_scan(w.colsum, w.cbase, kOrderBuckets, w.ctile, w.ctot, s);
+    launch_order_place(w.key_rank, n, w.table, w.cbase, w.ctile, order, s);
 }
 
 // ------------------------------------------------------ batched scenes ---
