@@ -1070,9 +1070,11 @@ static void compile_beams(CompiledScene &out, const double sun_dir[3]) {
         if (dist == 'g' || dist == 'G') { *th = 5.0 * mrad / 1000.0; return true; }
         return false;
     };
-    std::vector<int> near(s0.nelem, -1);
-    for (size_t q = 0; q < out.s0_boxes.size(); q++)
-        if (out.s0_boxes[q].box == 0) near[out.s0_boxes[q].j] = (int)q;
+    // element -> its near and far box in s0_boxes (-1: none)
+    std::vector<int> box_of[2] = {std::vector<int>(s0.nelem, -1), std::vector<int>(s0.nelem, -1)};
+    for (size_t q = 0; q < out.s0_boxes.size(); q++) box_of[out.s0_boxes[q].box][out.s0_boxes[q].j] = (int)q;
+    const std::vector<int> &near = box_of[0];
+    std::vector<int> stamp(out.s0_boxes.size(), -1);  // box already taken for element i
     auto centre = [](const DevBeamItem &b, double c[3]) {
         double r2 = 0.0;
         for (int k = 0; k < 3; k++) {
@@ -1115,22 +1117,57 @@ static void compile_beams(CompiledScene &out, const double sun_dir[3]) {
         double c[3];
         const double r = centre(out.s0_boxes[near[i]], c);
         const double r_host = r * (1.0 + 1e-6) + 1e-3, alpha_host = alpha + 1e-4;
-        std::vector<std::pair<double, int>> keep;
-        for (size_t q = 0; q < out.s0_boxes.size(); q++) {
-            double cq[3];
-            const double rho = centre(out.s0_boxes[q], cq) + r_host;
-            double v[3], dist = 0.0, va = 0.0;
-            for (int k = 0; k < 3; k++) { v[k] = cq[k] - c[k]; dist += v[k] * v[k]; va += v[k] * a[k]; }
-            dist = sqrt(dist);
-            bool in = dist <= rho;
-            if (!in) {
-                const double ang = acos(std::max(-1.0, std::min(1.0, va / dist)));
-                in = ang <= alpha_host + asin(std::min(1.0, rho / dist));
+        // whether a ray from S within the cone can reach the sphere (cq, rq);
+        // dist: from S's centre
+        auto reach = [&](const double cq[3], double rq, double *dist_out) {
+            const double rho = rq + r_host;
+            double dist = 0.0, va = 0.0;
+            for (int k = 0; k < 3; k++) {
+                const double v = cq[k] - c[k];
+                dist += v * v;
+                va += v * a[k];
             }
-            if (in) keep.emplace_back(dist, (int)q);
+            dist = sqrt(dist);
+            *dist_out = dist;
+            if (dist <= rho) return true;
+            const double ang = acos(std::max(-1.0, std::min(1.0, va / dist)));
+            return ang <= alpha_host + asin(std::min(1.0, rho / dist));
+        };
+        // the stage's BVH, pruned by the same test on each node's bounding
+        // sphere (a box inside a node lies inside the node's sphere, so a
+        // node that fails holds no box that passes)
+        std::vector<std::pair<double, int>> keep;
+        bool too_long = false;
+        for (int k = 0; k < s0.nbvh && !too_long;) {
+            const DevBvhNode &nd = out.bvh[(size_t)s0.bvh0 + k];
+            double cn[3], rn2 = 0.0, dist;
+            for (int q = 0; q < 3; q++) {
+                cn[q] = 0.5 * ((double)nd.lo[q] + (double)nd.hi[q]);
+                const double h = 0.5 * ((double)nd.hi[q] - (double)nd.lo[q]);
+                rn2 += h * h;
+            }
+            if (!reach(cn, sqrt(rn2), &dist)) {
+                k = nd.skip;
+            } else if (nd.leaf) {
+                for (int t = nd.leaf >> 3, te = t + (nd.leaf & 7); t < te; t++) {
+                    const int j = out.bvh_items[t];
+                    for (int q = 0; q < 2; q++) {
+                        const int bq = box_of[q][j];
+                        if (bq < 0 || stamp[bq] == i) continue;  // j sits in a leaf once per box
+                        stamp[bq] = i;
+                        double cq[3];
+                        const double rq = centre(out.s0_boxes[bq], cq);
+                        if (reach(cq, rq, &dist)) keep.emplace_back(dist, bq);
+                    }
+                }
+                k = nd.skip;
+            } else {
+                k = k + 1;
+            }
+            // a long list walks no faster than the BVH
+            too_long = keep.size() > 32;
         }
-        // a long list walks no faster than the BVH
-        if (keep.size() > 32) continue;
+        if (too_long) continue;
         std::sort(keep.begin(), keep.end());
         for (int k = 0; k < 3; k++) { bm.c[k] = c[k]; bm.a[k] = (float)a[k]; }
         bm.r2 = (float)(r * r);
