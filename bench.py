@@ -536,7 +536,7 @@ def main():
 
     t32, tm32 = _timed_steps(lib, c32, args.steps, args.warmup, args.seed, sync)
     f32 = _flux(lib, c32)
-    t64, tm64 = _timed_steps(lib, c64, max(1, args.steps // 2), 1, args.seed, sync)
+    t64, tm64 = _timed_steps(lib, c64, args.steps, max(1, args.warmup), args.seed, sync)
     f64 = _flux(lib, c64)
     nrmse = float(np.sqrt(np.mean((f32 - f64) ** 2)) / max(1.0, f64.max()))
 

@@ -47,6 +47,32 @@ void *scratch_on(int dev) {
     return scratch_dev[dev];
 }
 
+// The streams of every context on GPU `dev`, made once per device with that
+// device current and kept until the process ends.  A process gets few
+// hardware queues (four by default) and streams beyond them share queues,
+// where one stream's work waits behind another's: contexts share one set so
+// that two or more contexts of a process keep their streams on queues of
+// their own.
+static std::mutex streams_mu;
+static DevStreams streams_dev[kMaxDevices];
+const DevStreams &streams_on(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device ordinal out of range");
+    std::lock_guard<std::mutex> lk(streams_mu);
+    DevStreams &d = streams_dev[dev];
+    if (!d.trace) {
+        int cur = 0;
+        HC(hipGetDevice(&cur));
+        if (cur != dev) HC(hipSetDevice(dev));
+        DevStreams n{};
+        hipError_t e = hipStreamCreateWithFlags(&n.trace, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&n.mask, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&n.comp, hipStreamNonBlocking);
+        if (cur != dev) HC(hipSetDevice(cur));
+        HC(e);
+        d = n;
+    }
+    return d;
+}
 
 void Context::materialize() {
     if (cols_valid) return;

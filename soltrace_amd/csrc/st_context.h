@@ -66,6 +66,13 @@ inline T *upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
 // (st_context.cpp).
 void *scratch_on(int dev);
 
+// The trace, sun-mask and compaction streams shared by the contexts on GPU
+// `dev` (st_context.cpp).
+struct DevStreams {
+    hipStream_t trace, mask, comp;
+};
+const DevStreams &streams_on(int dev);
+
 // A stage's records of one run: chunks of device records, each a range of a
 // buffer.  Buffers a context allocated return to its pool when the results
 // are cleared; a batched run's buffers are shared by its contexts and are
@@ -113,9 +120,11 @@ struct Context {
     bool count_events = false;
     DevBuf d_counters;
     std::vector<std::string> messages;
+    // the device's shared streams (streams_on): trace; the next batch's sun
+    // mask, beside the current batch's trace; record and ray compaction of a
+    // trace launch, beside the next launch
     hipStream_t stream = nullptr;
-    hipStream_t mask_stream = nullptr;  // next batch's sun mask, beside the current batch's trace
-    // record and ray compaction of a trace launch, beside the next launch
+    hipStream_t mask_stream = nullptr;
     hipStream_t comp_stream = nullptr;
     hipEvent_t ev_hits[2] = {nullptr, nullptr}, ev_ready = nullptr, ev_join = nullptr;
     // per launch-buffer set: its totals reached the host; its compaction is done
@@ -188,9 +197,10 @@ struct Context {
     Context() {
         HC(hipGetDevice(&device));
         scratch_ptr = scratch_on(device);
-        HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        HC(hipStreamCreateWithFlags(&mask_stream, hipStreamNonBlocking));
-        HC(hipStreamCreateWithFlags(&comp_stream, hipStreamNonBlocking));
+        const DevStreams &ds = streams_on(device);
+        stream = ds.trace;
+        mask_stream = ds.mask;
+        comp_stream = ds.comp;
         for (auto &e : ev_hits) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (auto &e : ev_tot) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (auto &e : ev_comp) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -216,9 +226,6 @@ struct Context {
         if (h_args) (void)hipHostFree(h_args);
         for (hipEvent_t e : ev_stage)
             if (e) (void)hipEventDestroy(e);
-        if (mask_stream) (void)hipStreamDestroy(mask_stream);
-        if (comp_stream) (void)hipStreamDestroy(comp_stream);
-        if (stream) (void)hipStreamDestroy(stream);
     }
     void errlog(const char *fmt, ...) {
         char b[513];
