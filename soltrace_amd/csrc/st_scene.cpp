@@ -1118,7 +1118,10 @@ static void compile_beams(CompiledScene &out, const double sun_dir[3]) {
         const double r = centre(out.s0_boxes[near[i]], c);
         const double r_host = r * (1.0 + 1e-6) + 1e-3, alpha_host = alpha + 1e-4;
         // whether a ray from S within the cone can reach the sphere (cq, rq);
-        // dist: from S's centre
+        // dist: from S's centre.  angle(v, a) <= alpha + asin(rho / |v|) is
+        // tested as cos(angle) >= cos(alpha + beta), beta = asin(rho / |v|)
+        // (alpha + beta < pi here).
+        const double ca = cos(alpha_host), sa = sin(alpha_host);
         auto reach = [&](const double cq[3], double rq, double *dist_out) {
             const double rho = rq + r_host;
             double dist = 0.0, va = 0.0;
@@ -1130,8 +1133,8 @@ static void compile_beams(CompiledScene &out, const double sun_dir[3]) {
             dist = sqrt(dist);
             *dist_out = dist;
             if (dist <= rho) return true;
-            const double ang = acos(std::max(-1.0, std::min(1.0, va / dist)));
-            return ang <= alpha_host + asin(std::min(1.0, rho / dist));
+            const double sb = rho / dist, cb = sqrt(1.0 - sb * sb);
+            return va / dist >= ca * cb - sa * sb;
         };
         // the stage's BVH, pruned by the same test on each node's bounding
         // sphere (a box inside a node lies inside the node's sphere, so a
