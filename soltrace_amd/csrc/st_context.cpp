@@ -426,6 +426,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         DevRay *next_rays;
         uint64_t *n_next;
         std::chrono::steady_clock::time_point t0;
+        bool last;  // the stage's last launch: compacted on the trace stream, nothing to overlap
     };
     hipStream_t cs_ = comp_stream;
     uint64_t launch_seq = 0;
@@ -495,7 +496,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             launch_fill_u32(tot_set + 1, 1, 0u, s);
         HC(hipMemcpyAsync(h_tot + 4 * set, tot_set, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HC(hipEventRecord(ev_tot[set], s));
-        return Pending{a, si, n, set, next_rays, n_next, std::chrono::steady_clock::now()};
+        return Pending{a, si, n, set, next_rays, n_next, std::chrono::steady_clock::now(), false};
     };
     auto trace_finish = [&](Pending &p) -> bool {
         const int set = p.set;
@@ -506,16 +507,17 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         const TraceArgs &a = p.a;
         const int n = p.n, si = p.si;
         std::shared_ptr<DevBuf> chunk = take_buffer((size_t)std::max<uint32_t>(1, tot[0]) * sizeof(DevRecord));
-        HC(hipStreamWaitEvent(cs_, ev_tot[set], 0));
+        const hipStream_t cst = p.last ? s : cs_;
+        if (!p.last) HC(hipStreamWaitEvent(cs_, ev_tot[set], 0));
         launch_compact_records(a.rec_slots, a.rec_count, d_rec_offs[set].as<uint32_t>(), d_scan_tmp[set].as<uint32_t>(),
-                               n, kSlots, chunk->as<DevRecord>(), cs_);
+                               n, kSlots, chunk->as<DevRecord>(), cst);
         launch_compact_overflow(a.ovf_rec, a.ovf_meta, a.ovf_count, ovf_cap, d_rec_offs[set].as<uint32_t>(),
-                                d_scan_tmp[set].as<uint32_t>(), chunk->as<DevRecord>(), cs_);
+                                d_scan_tmp[set].as<uint32_t>(), chunk->as<DevRecord>(), cst);
         if (tot[1] && p.next_rays)
             launch_compact_rays(a.out_slots, a.out_flag, d_out_offs[set].as<uint32_t>(),
-                                d_scan_tmp2[set].as<uint32_t>(), n, p.next_rays + *p.n_next, cs_);
-        HC(hipEventRecord(ev_comp[set], cs_));
-        comp_queued[set] = true;
+                                d_scan_tmp2[set].as<uint32_t>(), n, p.next_rays + *p.n_next, cst);
+        if (!p.last) HC(hipEventRecord(ev_comp[set], cs_));
+        comp_queued[set] = !p.last;
         recs[si].add(chunk, 0, tot[0], true);
         timing.records[si] += tot[0];
         if (p.n_next) *p.n_next += tot[1];
@@ -694,6 +696,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
                                                 0, nst > 1 ? next : nullptr, &n_next,
                                                 cur_b.pre != nullptr ? cur_b.pre + c : nullptr,
                                                 c == 0 && (uint32_t)n == cur_b.take ? cur_b.order : nullptr));
+                inflight.back().last = !cur_b.more && c + (uint32_t)n >= cur_b.take;
                 if (c == 0 && cur_b.more) {
                     next_b = prepare();  // overlaps the trace just queued
                     have_next = true;
@@ -842,6 +845,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         for (uint64_t c = 0; c < n_cur; c += launch_cap) {
             int n = (int)std::min<uint64_t>(launch_cap, n_cur - c);
             inflight.push_back(trace_launch((int)si, n, nullptr, 0, cur + c, last_prev, nxt, &n_out, nullptr));
+            inflight.back().last = c + (uint64_t)n >= n_cur;
             if (!finish_to(depth - 1)) {
                 HC(hipStreamSynchronize(cs_));
                 return -1;
@@ -880,7 +884,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         prev_has = n_cur > 0;
         if (prev_has) last_prev = last_raynum_of(cur, n_cur);
     }
-    {
+    if (presample) {
         // the sunshape sampler's error flags
         uint32_t e = 0;
         HC(hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, s));

@@ -892,9 +892,10 @@ __device__ __forceinline__ void trace_one(const TraceArgs &a, const int t) {
 // Waves per SIMD (WV): the generic variant runs 2 (its registers spill,
 // but a second wave hides more latency than the spills cost; the one with
 // finite elements keeps the whole register file); the basic FP32 variant
-// runs 3 (168 registers, no spills: with beam lists the rays of the sun
-// stage wait little on the BVH, and 4 waves in 128 registers spill more
-// than the fourth wave hides); the basic FP64 variant 2 (3 is slower).
+// runs 3 (168 registers, 20 B of spills per lane: with beam lists the rays
+// of the sun stage wait little on the BVH, and 4 waves in 128 registers
+// spill more than the fourth wave hides); the basic FP64 variant 2 (3 is
+// slower).
 template <typename R, int G, int WV>
 __global__ __launch_bounds__(kBlock, WV) void trace_stage_kernel(const TraceArgs *__restrict__ ap) {
     const TraceArgs &a = *ap;
