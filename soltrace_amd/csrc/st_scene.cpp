@@ -595,6 +595,13 @@ static int element_boxes(const HElement &e, double lo[2][3], double hi[2][3]) {
             const double pad = 0.05 * (zh[b] - zl[b]) + 1e-3;
             zl[b] -= pad; zh[b] += pad;
         }
+        // The trace refuses a hit above the aperture plane, z > zap, of
+        // these surfaces (coretrace/raytrace.cpp:747-751): a far branch
+        // wholly above it, the far cap of a concave mirror, bounds no hit.
+        if (zl[1] <= zh[1] && zl[1] > e.zap) {
+            zl[1] = HUGE_VAL;
+            zh[1] = -HUGE_VAL;
+        }
         if (zl[1] <= zh[1]) {
             if (zl[1] <= zh[0] && zl[0] <= zh[1]) {  // the branches meet: one box
                 zl[0] = std::min(zl[0], zl[1]);
