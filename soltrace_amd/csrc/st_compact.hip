@@ -156,13 +156,32 @@ void launch_mask_compact(const uint64_t *mask, const uint32_t *offsets, const ui
 
 // Slots are slot-major (record j of ray t at slots[j * n_rays + t]), so the
 // first records of a wave's rays, most of the data, are read contiguously.
-__global__ void compact_records_kernel(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets,
-                                       const uint32_t *tile_off, int n_rays, int nslots, DevRecord *out) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+// A wave whose 64 rays have one record each copies them as one block: its
+// 64 records are contiguous in the slots and in the output, and the lanes
+// move the block in 8-byte words, each load and store instruction covering
+// 512 contiguous bytes (a record per lane would touch 56 lines per
+// instruction).  Other waves copy per lane.
+static_assert(sizeof(DevRecord) == 56, "record layout");
+__global__ __launch_bounds__(256) void compact_records_kernel(const DevRecord *slots, const uint32_t *count,
+                                                              const uint32_t *offsets, const uint32_t *tile_off,
+                                                              int n_rays, int nslots, DevRecord *out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const uint32_t c = t < n_rays ? count[t] : 0u;
+    const uint32_t o = t < n_rays ? offsets[t] + tile_off[t / kScanTile] : 0u;
+    if (__ballot(c == 1u) == ~0ull) {
+        // every lane is in range with one record: out[o0 + l] = slots[t0 + l]
+        const int t0 = t - lane;
+        const uint32_t o0 = (uint32_t)__shfl((int)o, 0);
+        const uint2 *src = reinterpret_cast<const uint2 *>(slots + t0);
+        uint2 *dst = reinterpret_cast<uint2 *>(out + o0);
+        constexpr int kWords = 64 * (int)sizeof(DevRecord) / 8;  // 448
+#pragma unroll
+        for (int k = 0; k < kWords / 64; ++k) dst[k * 64 + lane] = src[k * 64 + lane];
+        return;
+    }
     if (t >= n_rays) return;
-    uint32_t c = count[t];
-    uint32_t m = c < (uint32_t)nslots ? c : (uint32_t)nslots;
-    uint32_t o = offsets[t] + tile_off[t / kScanTile];
+    const uint32_t m = c < (uint32_t)nslots ? c : (uint32_t)nslots;
     for (uint32_t j = 0; j < m; ++j) out[o + j] = slots[(size_t)j * n_rays + t];
 }
 void launch_compact_records(const DevRecord *slots, const uint32_t *count, const uint32_t *offsets,
@@ -189,11 +208,31 @@ void launch_compact_overflow(const DevRecord *ovf, const uint32_t *meta, const u
                        cap, offsets, tile_off, out);
 }
 
-__global__ void compact_rays_kernel(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets,
-                                    const uint32_t *tile_off, int n_rays, DevRay *out) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_rays) return;
-    if (flag[t]) out[offsets[t] + tile_off[t / kScanTile]] = slots[t];
+// A wave's passed rays land contiguously in the output (their offsets are
+// consecutive), so the wave copies them as one block of 16-byte words: word
+// q of the block is word q mod 4 of the (q / 4)-th passed ray, whose lane
+// the wave keeps in LDS by rank.
+static_assert(sizeof(DevRay) == 64, "ray layout");
+__global__ __launch_bounds__(256) void compact_rays_kernel(const DevRay *slots, const uint32_t *flag,
+                                                           const uint32_t *offsets, const uint32_t *tile_off,
+                                                           int n_rays, DevRay *out) {
+    __shared__ int src_of[256 / 64][64];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const bool f = t < n_rays && flag[t] != 0u;
+    const uint64_t m = __ballot(f);
+    if (m == 0) return;
+    const int first = __ffsll((unsigned long long)m) - 1;
+    const uint32_t o = f ? offsets[t] + tile_off[t / kScanTile] : 0u;
+    const uint32_t o0 = (uint32_t)__shfl((int)o, first);
+    if (f) src_of[wv][__popcll(m & ((1ull << lane) - 1ull))] = t;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int words = __popcll(m) * 4;
+    const uint4 *src = reinterpret_cast<const uint4 *>(slots);
+    uint4 *dst = reinterpret_cast<uint4 *>(out + o0);
+    for (int q = lane; q < words; q += 64) dst[q] = src[(size_t)src_of[wv][q >> 2] * 4 + (q & 3)];
 }
 void launch_compact_rays(const DevRay *slots, const uint32_t *flag, const uint32_t *offsets, const uint32_t *tile_off,
                          int n_rays, DevRay *out, hipStream_t s) {
