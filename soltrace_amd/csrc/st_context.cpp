@@ -322,7 +322,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
 
     // kernel times: events recorded around the launches, read once the run
     // has finished (no synchronisation per launch)
-    std::vector<std::pair<double *, size_t>> ev_marks;
+    marks.clear();
     size_t ev_used = 0;
     auto timed_on = [&](hipStream_t st, double &acc, auto &&fn) {
         while (ev_used + 2 > ev_pool.size()) {
@@ -333,7 +333,7 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         HC(hipEventRecord(ev_pool[ev_used], st));
         fn();
         HC(hipEventRecord(ev_pool[ev_used + 1], st));
-        ev_marks.emplace_back(&acc, ev_used);
+        marks.emplace_back(&acc, ev_used);
         ev_used += 2;
     };
     auto timed = [&](double &acc, auto &&fn) { timed_on(s, acc, fn); };
@@ -828,7 +828,11 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
             return -1;
         }
     } else if (!cancelled && n_cur > 0) {
-        last_prev = last_raynum_of(cur, n_cur);
+        // the last ray number handed on matters to the pass-on rule only for
+        // a single ray, to a trace-through stage 2 and to the callback
+        // (reading it costs a round trip while the GPU waits)
+        if (n_cur == 1 || cb != nullptr || (nst > 1 && cs.stages[1].tracethrough))
+            last_prev = last_raynum_of(cur, n_cur);
         uint64_t below = (last_prev == N) ? n_cur - 1 : n_cur;
         prev_has = below > 0;
     }
@@ -882,33 +886,26 @@ int Context::run_t(uint32_t seed, st_callback_t cb, void *cbdata) {
         cur = nxt;
         n_cur = n_out;
         prev_has = n_cur > 0;
-        if (prev_has) last_prev = last_raynum_of(cur, n_cur);
+        if (prev_has && (cb != nullptr || (si + 1 < nst && cs.stages[si + 1].tracethrough)))
+            last_prev = last_raynum_of(cur, n_cur);
     }
-    if (presample) {
-        // the sunshape sampler's error flags
-        uint32_t e = 0;
-        HC(hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, s));
-        HC(hipStreamSynchronize(s));
-        if (!check_err(e)) return -1;
-    }
-    for (const auto &m : ev_marks) {
-        float ms = 0;
-        HC(hipEventElapsedTime(&ms, ev_pool[m.second], ev_pool[m.second + 1]));
-        *m.first += ms;
-    }
+    // the run's last copies behind one synchronisation: the sunshape
+    // sampler's error flags, the flux map, the event counters
+    uint32_t sampler_err = 0;
+    if (presample) HC(hipMemcpyAsync(&sampler_err, err, sizeof sampler_err, hipMemcpyDeviceToHost, s));
     if (fs.active) {
         flux_host.assign((size_t)fs.nx * fs.ny, 0u);
         HC(hipMemcpyAsync(flux_host.data(), d_flux.p, flux_host.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
                           s));
-        HC(hipStreamSynchronize(s));
         flux_src_words = src_words;  // per-source maps stay in HBM until asked for
     }
     if (ctrs) {
         HC(hipMemcpyAsync(&timing.ev[0][0], ctrs, sizeof(timing.ev), hipMemcpyDeviceToHost, s));
         HC(hipMemcpyAsync(timing.ev_sun, ctrs + 5 * (1 + kMaxStages), sizeof(timing.ev_sun), hipMemcpyDeviceToHost,
                           s));
-        HC(hipStreamSynchronize(s));
     }
+    HC(hipStreamSynchronize(s));
+    if (!check_err(sampler_err)) return -1;
     timing.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return (int)total_records();
 }
@@ -1758,6 +1755,7 @@ STCORE_API int st_amd_events(st_context_t p, double *out, int n) {
 // [36..] records per stage, [52] sampler ms.
 STCORE_API int st_amd_timing(st_context_t p, double *out, int n) {
     CTX(p, -1);
+    c->settle_timing();
     std::vector<double> v;
     v.push_back(c->timing.mask_ms);
     for (int i = 0; i < kMaxStages; i++) v.push_back(c->timing.trace_ms[i]);

@@ -166,6 +166,16 @@ struct Context {
     // memory once the buffers have grown to the scene's size
     std::vector<std::shared_ptr<DevBuf>> rec_pool;
     std::vector<hipEvent_t> ev_pool;  // kernel timing events, reused
+    // the last run's kernel intervals (a time field, the first of two pool
+    // events), turned into times when the timing is asked for
+    std::vector<std::pair<double *, size_t>> marks;
+    void settle_timing() {
+        for (const auto &m : marks) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, ev_pool[m.second], ev_pool[m.second + 1]) == hipSuccess) *m.first += ms;
+        }
+        marks.clear();
+    }
     uint64_t sun_ray_count = 0;
     // the records in merged order as the getters' columns, in HBM
     bool cols_valid = false;
