@@ -1043,8 +1043,6 @@ static bool foot_hull(std::vector<std::pair<double, double>> pts, double pad, De
     return true;
 }
 
-// Sun frame, sun-plane box and the stage-1 sun-plane grid
-// (coretrace/suntoprimarystage.cpp:59-293).
 // Beam lists of the stage-1 elements (DevBeam).  Element i's list holds
 // every stage-1 hit box that a ray starting in the sphere S about i's near
 // box, in a direction within angle alpha of the axis a, can reach: for a
@@ -1193,6 +1191,8 @@ static void compile_beams(CompiledScene &out, const double sun_dir[3]) {
     }
 }
 
+// Sun frame, sun-plane box and the stage-1 sun-plane grid
+// (coretrace/suntoprimarystage.cpp:59-293).
 bool compile_sun(SceneModel &m, CompiledScene &out, std::string &msg) {
     HStage &st = m.stages[0];
     HSun &sun = m.sun;

@@ -68,13 +68,17 @@ CULL_SCENES = {
     "field": dict(n_helio=300, raycount=20000, r_max=700.0),
     "dense": dict(n_helio=200, raycount=40000, r_max=200.0, sun_elev=30.0),
     "dense_wide": dict(n_helio=200, raycount=40000, r_max=200.0, sun_elev=60.0, slope=12.0, sun="pillbox"),
+    # flat 2 x 2-facet heliostats around a cylindrical receiver: flat beam
+    # lists (no curvature term), Gaussian sun
+    "faceted": dict(n_helio=150, raycount=40000, r_max=260.0, facet=5.0),
 }
 
 
 @pytest.mark.parametrize("bits", [64, 32])
 @pytest.mark.parametrize("name", sorted(CULL_SCENES))
 def test_cull_equals_full_scan(hip_lib, name, bits):
-    sc = workloads.tower(**CULL_SCENES[name])
+    kw = dict(CULL_SCENES[name])
+    sc = workloads.faceted_field(**kw) if name == "faceted" else workloads.tower(**kw)
     a = run_scene(hip_lib, sc, 4, setup=_setup(bits, 1))
     b = run_scene(hip_lib, sc, 4, setup=_setup(bits, 0))
     assert a[0] == b[0] > 0
