@@ -919,6 +919,37 @@ inline void torus_start_sphere(Element &e) {
                                  (hi[2] - lo[2]) * (hi[2] - lo[2]));
 }
 
+// The torus of torus_closed_form lies in the shell |z - Rs| <= |Rs|,
+// rho <= |Ra| + |Rs| and, for a ring torus, rho >= |Ra| - |Rs|; a line that
+// stays 1e-3 (|Ra| + |Rs|) outside it has no real root, and the solve is
+// skipped (the no-root outcome: error 1, path 0).  Only with the start
+// sphere (the GPU library's arithmetic): the reference's own solve from a
+// far origin is ill-conditioned enough to return roots off the torus, which
+// the reference-stream mode keeps.
+inline bool torus_shell(const Element &e, const double P[3], const double D[3]) {
+    const double Rs = e.cross_r, Ra = e.annular_r;
+    const double m = 1e-3 * (fabs(Ra) + fabs(Rs));
+    const double z0 = Rs - fabs(Rs) - m, z1 = Rs + fabs(Rs) + m;
+    double ta = -1e300, tb = 1e300;
+    if (D[2] != 0.0) {
+        const double u = (z0 - P[2]) / D[2], v = (z1 - P[2]) / D[2];
+        ta = fmin(u, v);
+        tb = fmax(u, v);
+    } else if (P[2] < z0 || P[2] > z1) {
+        return false;
+    }
+    const double a = D[0] * D[0] + D[1] * D[1], b = P[0] * D[0] + P[1] * D[1], c = P[0] * P[0] + P[1] * P[1];
+    const double tv = fmin(fmax(a > 0.0 ? -b / a : ta, ta), tb);
+    const double hi = fabs(Ra) + fabs(Rs) + m;
+    if ((a * tv + 2.0 * b) * tv + c > hi * hi) return false;
+    const double lo = fabs(Ra) - fabs(Rs) - m;
+    if (lo > 0.0 && ta > -1e300 && tb < 1e300) {
+        const double ra = (a * ta + 2.0 * b) * ta + c, rb = (a * tb + 2.0 * b) * tb + c;
+        if (fmax(ra, rb) < lo * lo) return false;
+    }
+    return true;
+}
+
 // coretrace/intersect.cpp:140-1011
 inline void intersect(const double P[3], const double D[3], const Element &e, double X[3], double Cout[3], double DF[3],
                       double *path, int *err) {
@@ -953,10 +984,13 @@ inline void intersect(const double P[3], const double D[3], const Element &e, do
             const double rr = 2.0 * e.torus_start_r + 1.0;
             if (dist2 > rr * rr && tc > rr) t0 = tc - rr;
         }
-        if (t0 == 0.0) {
+        const double Ps[3] = {P[0] + t0 * D[0], P[1] + t0 * D[1], P[2] + t0 * D[2]};
+        if (e.torus_start_r > 0.0 && !torus_shell(e, t0 == 0.0 ? P : Ps, D)) {
+            *path = 0.0;
+            *err = 1;
+        } else if (t0 == 0.0) {
             torus_closed_form(e, P, D, X, DF, path, err);
         } else {
-            const double Ps[3] = {P[0] + t0 * D[0], P[1] + t0 * D[1], P[2] + t0 * D[2]};
             torus_closed_form(e, Ps, D, X, DF, path, err);
             *path = t0 + *path;
         }

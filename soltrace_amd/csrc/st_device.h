@@ -438,6 +438,37 @@ struct QuarticState {
     }
 };
 
+// The torus (x^2 + y^2 + (z - Rs)^2 + Ra^2 - Rs^2)^2 = 4 Ra^2 (x^2 + y^2) of
+// the quartic below lies in the shell |z - Rs| <= |Rs|, rho <= |Ra| + |Rs|
+// and, for a ring torus, rho >= |Ra| - |Rs| (rho^2 = x^2 + y^2).  A line
+// that stays a margin outside the shell has no real root; its quartic solve
+// is skipped.  The margin, 1e-3 (|Ra| + |Rs|), keeps near-tangent lines,
+// whose root pairs rounding could turn real or complex, on the solve.
+STD bool torus_shell(const DevElem &e, const double P[3], const double D[3]) {
+    const double Rs = e.cross_r, Ra = e.ann_r;
+    const double m = 1e-3 * (::fabs(Ra) + ::fabs(Rs));
+    const double z0 = Rs - ::fabs(Rs) - m, z1 = Rs + ::fabs(Rs) + m;
+    double ta = -1e300, tb = 1e300;  // the line's parameters inside the slab z0 <= z <= z1
+    if (D[2] != 0.0) {
+        const double u = (z0 - P[2]) / D[2], v = (z1 - P[2]) / D[2];
+        ta = ::fmin(u, v);
+        tb = ::fmax(u, v);
+    } else if (P[2] < z0 || P[2] > z1) {
+        return false;
+    }
+    // rho^2(t) = a t^2 + 2 b t + c on [ta, tb]: least at the vertex (clamped), most at an end
+    const double a = D[0] * D[0] + D[1] * D[1], b = P[0] * D[0] + P[1] * D[1], c = P[0] * P[0] + P[1] * P[1];
+    const double tv = ::fmin(::fmax(a > 0.0 ? -b / a : ta, ta), tb);
+    const double hi = ::fabs(Ra) + ::fabs(Rs) + m;
+    if ((a * tv + 2.0 * b) * tv + c > hi * hi) return false;
+    const double lo = ::fabs(Ra) - ::fabs(Rs) - m;
+    if (lo > 0.0 && ta > -1e300 && tb < 1e300) {
+        const double ra = (a * ta + 2.0 * b) * ta + c, rb = (a * tb + 2.0 * b) * tb + c;
+        if (::fmax(ra, rb) < lo * lo) return false;
+    }
+    return true;
+}
+
 STD bool torus(const DevElem &e, const double P[3], const double D[3], double X[3], double DF[3], double *path) {
     const double Rs = e.cross_r, Ra = e.ann_r;
     const double Xo = P[0], Yo = P[1], Zo = P[2];
@@ -1124,8 +1155,8 @@ STD bool element_hit(const DevElem &e, const double *aux, const double P[3], con
             if (dist2 > rr * rr && tc > rr) t0 = tc - rr;
         }
         const double Ps[3] = {P[0] + t0 * D[0], P[1] + t0 * D[1], P[2] + t0 * D[2]};
-        double Xd[3], DFd[3], L;
-        ok = torus(e, t0 != 0.0 ? Ps : P, D, Xd, DFd, &L);
+        double Xd[3] = {0.0, 0.0, 0.0}, DFd[3] = {0.0, 0.0, 0.0}, L = 0.0;
+        ok = torus_shell(e, t0 != 0.0 ? Ps : P, D) && torus(e, t0 != 0.0 ? Ps : P, D, Xd, DFd, &L);
         path = t0 + L;
         if (ok && path < 0.0) ok = false;  // coretrace/determineelementintersectionnew.cpp:86-100
         for (int i = 0; i < 3; i++) { X[i] = (R)Xd[i]; DF[i] = (R)DFd[i]; }

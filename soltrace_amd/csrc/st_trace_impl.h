@@ -1001,7 +1001,10 @@ STD uint32_t wave_take(uint32_t *counter) {
 }
 
 // Sun masks of several scenes: a tile is one wave's 64 * kMaskRays sun
-// rays of one scene.
+// rays of one scene, and a wave takes kMaskChunk tiles in a row per atomic
+// on the queue counter (one atomic per 256 sun rays made the counter's
+// address the bottleneck of a launch).
+constexpr uint32_t kMaskChunk = 8;
 // Waves per SIMD held to those of the single-scene mask kernels (4 basic, 2
 // generic): the queue loop would otherwise cost registers and occupancy.
 template <typename R, int G>
@@ -1009,16 +1012,19 @@ __global__ __launch_bounds__(kBlock, G ? 2 : 4) void mask_batch_kernel(const Mas
                                                                        BatchQueue q) {
     const uint32_t total = q.prefix[q.n];
     for (;;) {
-        const uint32_t tile = wave_take(q.counter);
-        if (tile >= total) break;
-        const int k = __builtin_amdgcn_readfirstlane(batch_item(q, tile));
-        const MaskArgs &a = items[k];
-        const uint32_t local = tile - q.prefix[k];
-        if (a.compact) {
-            mask_grid_wave<R, G>(a, local);
-        } else {
-            for (int r = 0; r < kMaskRays; r++)
-                mask_lane<R, G>(a, (local * kMaskRays + (uint32_t)r) * 64u + (threadIdx.x & 63));
+        const uint32_t t0 = wave_take(q.counter) * kMaskChunk;
+        if (t0 >= total) break;
+        const uint32_t t1 = min(t0 + kMaskChunk, total);
+        for (uint32_t tile = t0; tile < t1; tile++) {
+            const int k = __builtin_amdgcn_readfirstlane(batch_item(q, tile));
+            const MaskArgs &a = items[k];
+            const uint32_t local = tile - q.prefix[k];
+            if (a.compact) {
+                mask_grid_wave<R, G>(a, local);
+            } else {
+                for (int r = 0; r < kMaskRays; r++)
+                    mask_lane<R, G>(a, (local * kMaskRays + (uint32_t)r) * 64u + (threadIdx.x & 63));
+            }
         }
     }
 }
